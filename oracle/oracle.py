@@ -1,0 +1,288 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+The oracle is the C restatement of the reference's gridFFT operator and its
+CG/FISTA callers in ``gridfft_oracle.c`` (built by ``oracle/Makefile`` into
+``liboracle_f32.so`` and ``liboracle_f64.so``).  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this
+module, as the checker; the product path (``mri_recon_amd``) never does.
+
+Array conventions follow the reference (Blitz ColumnMajorArray, x fastest):
+images are numpy arrays of shape (Nz, Ny, Nx); coil maps (C, Nz, Ny, Nx);
+k-space (C, M); the trajectory kx/ky/kz/kw are float32 (M,) in the
+reference's units (cycles per FOV, |k| <= N/2; src/mri_data.h:50-57).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+KERNELS = {"triangle": 0, "kaiser": 1, "sinc": 2, "poly": 3}
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32 * 3),
+        ("overgrid", ctypes.c_float),
+        ("grid", ctypes.c_float * 3),
+        ("dwin", ctypes.c_float * 3),
+        ("kernel", ctypes.c_int32),
+        ("grid_in", ctypes.c_int32 * 3),
+        ("fft_in", ctypes.c_int32 * 3),
+        ("grid_scale", ctypes.c_float * 3),
+        ("k_rad", ctypes.c_float),
+    ]
+
+
+_LIBS = {}
+
+
+def build():
+    """Compile both oracle libraries (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _lib(precision):
+    if precision not in _LIBS:
+        path = os.path.join(_HERE, "liboracle_%s.so" % precision)
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        lib.orc_create.restype = vp
+        lib.orc_create.argtypes = [ctypes.POINTER(OrcParams)]
+        lib.orc_destroy.argtypes = [vp]
+        lib.orc_geometry.argtypes = [vp] + [vp] * 6
+        lib.orc_table.argtypes = [vp, ctypes.c_int, vp]
+        lib.orc_table.restype = ctypes.c_int
+        lib.orc_window.argtypes = [vp, ctypes.c_int, vp]
+        lib.orc_window.restype = ctypes.c_int
+        lib.orc_bessi0_eval.argtypes = [dbl]
+        lib.orc_bessi0_eval.restype = dbl
+        lib.orc_fft1d.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
+        lib.orc_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]
+        lib.orc_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i64]
+        lib.orc_stage_pre_fft.argtypes = [vp, vp, vp, vp]
+        lib.orc_stage_gather.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, vp]
+        lib.orc_stage_scatter.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64]
+        lib.orc_stage_ifft_post.argtypes = [vp, vp, vp, vp]
+        lib.orc_fft3.argtypes = [vp, vp, ctypes.c_int]
+        lib.orc_image_to_kspace.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, vp]
+        lib.orc_kspace_to_image.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64]
+        lib.orc_normal.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64]
+        lib.orc_cg.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int, vp]
+        lib.orc_power_method.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, i64, ctypes.c_int, vp]
+        lib.orc_power_method.restype = dbl
+        lib.orc_fista.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int,
+                                  dbl, vp]
+        lib.orc_threads.restype = ctypes.c_int
+        _LIBS[precision] = lib
+    return _LIBS[precision]
+
+
+def threads():
+    return _lib("f32").orc_threads()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def bessi0(x, precision="f64"):
+    return _lib(precision).orc_bessi0_eval(float(x))
+
+
+def fft1d(x, sign=-1, precision="f64"):
+    """Unitary 1-D DFT along the last axis (sign -1 forward, +1 backward)."""
+    ct = np.complex64 if precision == "f32" else np.complex128
+    a = np.ascontiguousarray(x, dtype=ct).copy()
+    n = a.shape[-1]
+    _lib(precision).orc_fft1d(n, sign, a.ctypes.data, a.size // n)
+    return a
+
+
+class OracleOp:
+    """One gridFFT instance (geometry, tables, windows) at one precision."""
+
+    def __init__(self, N, overgrid=1.5, dwin=-1.0, kernel="kaiser", grid_in=(1, 1, 1),
+                 fft_in=(1, 1, 1), grid=-1.0, grid_scale=1.0, k_rad=9999.0, precision="f64"):
+        if np.isscalar(N):
+            N = (N, N, N)
+        self.precision = precision
+        self.ctype = np.complex64 if precision == "f32" else np.complex128
+        self.lib = _lib(precision)
+        p = OrcParams()
+        trip = lambda v: v if not np.isscalar(v) else (v, v, v)
+        for a in range(3):
+            p.N[a] = int(N[a])
+            p.grid[a] = float(trip(grid)[a])
+            p.dwin[a] = float(trip(dwin)[a])
+            p.grid_in[a] = int(grid_in[a])
+            p.fft_in[a] = int(fft_in[a])
+            p.grid_scale[a] = float(trip(grid_scale)[a])
+        p.overgrid = float(overgrid)
+        p.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        p.k_rad = float(k_rad)
+        self.params = p
+        self.h = self.lib.orc_create(ctypes.byref(p))
+        self.N = tuple(int(n) for n in N)
+        g = self.geometry()
+        self.S = tuple(g["S"])
+        self.og = tuple(g["og"])
+
+    def __del__(self):
+        try:
+            self.lib.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def geometry(self):
+        S = np.zeros(3, np.int32)
+        og = np.zeros(3, np.int32)
+        L = np.zeros(3, np.int32)
+        grid = np.zeros(3)
+        dwin = np.zeros(3)
+        beta = np.zeros(3)
+        self.lib.orc_geometry(self.h, S.ctypes.data, og.ctypes.data, L.ctypes.data,
+                              grid.ctypes.data, dwin.ctypes.data, beta.ctypes.data)
+        return dict(S=S, og=og, L=L, grid=grid, dwin=dwin, beta=beta)
+
+    def table(self, axis=0):
+        out = np.zeros(4096)
+        n = self.lib.orc_table(self.h, axis, out.ctypes.data)
+        return out[:n].copy()
+
+    def window(self, axis=0):
+        out = np.zeros(8192)
+        n = self.lib.orc_window(self.h, axis, out.ctypes.data)
+        return out[:n].copy()
+
+    # --- helpers -------------------------------------------------------
+    def _c(self, a):
+        return None if a is None else np.ascontiguousarray(a, dtype=self.ctype)
+
+    @staticmethod
+    def _traj(traj):
+        return [np.ascontiguousarray(t, dtype=np.float32) for t in traj]
+
+    def grid_shape(self):
+        return (self.S[2], self.S[1], self.S[0])
+
+    # --- per-coil operator (reference names) ---------------------------
+    def backward(self, X, smap, traj, sub=None):
+        """A_c X (gridFFT::backward); minus sub when given (backward_residual)."""
+        kx, ky, kz, kw = self._traj(traj)
+        X, smap, sub = self._c(X), self._c(smap), self._c(sub)
+        out = np.zeros(kx.shape[0], self.ctype)
+        self.lib.orc_backward(self.h, X.ctypes.data, _p(smap), out.ctypes.data, kx.ctypes.data,
+                              ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0], _p(sub))
+        return out
+
+    def forward(self, X, smap, data, traj):
+        """X += A_c^H (kw taper data) (gridFFT::forward); X updated in place."""
+        kx, ky, kz, kw = self._traj(traj)
+        assert X.dtype == self.ctype and X.flags.c_contiguous
+        smap, data = self._c(smap), self._c(data)
+        self.lib.orc_forward(self.h, X.ctypes.data, _p(smap), data.ctypes.data, kx.ctypes.data,
+                             ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0])
+        return X
+
+    # --- stages ---------------------------------------------------------
+    def stage_pre_fft(self, X, smap):
+        g = np.zeros(self.grid_shape(), self.ctype)
+        X, smap = self._c(X), self._c(smap)
+        self.lib.orc_stage_pre_fft(self.h, X.ctypes.data, _p(smap), g.ctypes.data)
+        return g
+
+    def stage_gather(self, grid, traj, sub=None):
+        kx, ky, kz, kw = self._traj(traj)
+        grid, sub = self._c(grid), self._c(sub)
+        out = np.zeros(kx.shape[0], self.ctype)
+        self.lib.orc_stage_gather(self.h, grid.ctypes.data, out.ctypes.data, kx.ctypes.data,
+                                  ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0],
+                                  _p(sub))
+        return out
+
+    def stage_scatter(self, data, traj):
+        kx, ky, kz, kw = self._traj(traj)
+        g = np.zeros(self.grid_shape(), self.ctype)
+        data = self._c(data)
+        self.lib.orc_stage_scatter(self.h, g.ctypes.data, data.ctypes.data, kx.ctypes.data,
+                                   ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0])
+        return g
+
+    def stage_ifft_post(self, grid, smap, X=None):
+        g = self._c(grid).copy()
+        if X is None:
+            X = np.zeros((self.N[2], self.N[1], self.N[0]), self.ctype)
+        smap = self._c(smap)
+        self.lib.orc_stage_ifft_post(self.h, g.ctypes.data, X.ctypes.data, _p(smap))
+        return X
+
+    def fft3(self, grid, sign):
+        g = self._c(grid).copy()
+        self.lib.orc_fft3(self.h, g.ctypes.data, int(sign))
+        return g
+
+    # --- multi-coil -----------------------------------------------------
+    def image_to_kspace(self, X, smaps, traj, sub=None):
+        kx, ky, kz, kw = self._traj(traj)
+        X, smaps, sub = self._c(X), self._c(smaps), self._c(sub)
+        C = 1 if smaps is None else smaps.shape[0]
+        out = np.zeros((C, kx.shape[0]), self.ctype)
+        self.lib.orc_image_to_kspace(self.h, X.ctypes.data, _p(smaps), C, out.ctypes.data,
+                                     kx.ctypes.data, ky.ctypes.data, kz.ctypes.data,
+                                     kw.ctypes.data, kx.shape[0], _p(sub))
+        return out
+
+    def kspace_to_image(self, data, smaps, traj, X=None):
+        kx, ky, kz, kw = self._traj(traj)
+        data, smaps = self._c(data), self._c(smaps)
+        C = data.shape[0]
+        if X is None:
+            X = np.zeros((self.N[2], self.N[1], self.N[0]), self.ctype)
+        self.lib.orc_kspace_to_image(self.h, X.ctypes.data, _p(smaps), C, data.ctypes.data,
+                                     kx.ctypes.data, ky.ctypes.data, kz.ctypes.data,
+                                     kw.ctypes.data, kx.shape[0])
+        return X
+
+    def normal(self, P, smaps, traj):
+        kx, ky, kz, kw = self._traj(traj)
+        P, smaps = self._c(P), self._c(smaps)
+        C = 1 if smaps is None else smaps.shape[0]
+        out = np.zeros_like(P)
+        self.lib.orc_normal(self.h, P.ctypes.data, _p(smaps), C, out.ctypes.data, kx.ctypes.data,
+                            ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0])
+        return out
+
+    def cg(self, smaps, data, traj, iters):
+        """SENSE-CG (recon_lib.cpp:1523-1759); returns (X, trace[iters,4])."""
+        kx, ky, kz, kw = self._traj(traj)
+        data, smaps = self._c(data), self._c(smaps)
+        X = np.zeros((self.N[2], self.N[1], self.N[0]), self.ctype)
+        trace = np.zeros((iters, 4))
+        self.lib.orc_cg(self.h, X.ctypes.data, _p(smaps), data.shape[0], data.ctypes.data,
+                        kx.ctypes.data, ky.ctypes.data, kz.ctypes.data, kw.ctypes.data,
+                        kx.shape[0], iters, trace.ctypes.data)
+        return X, trace
+
+    def power_method(self, smaps, traj, iters=30):
+        kx, ky, kz, kw = self._traj(traj)
+        smaps = self._c(smaps)
+        C = 1 if smaps is None else smaps.shape[0]
+        eig = np.zeros(iters)
+        step = self.lib.orc_power_method(self.h, _p(smaps), C, kx.ctypes.data, ky.ctypes.data,
+                                         kz.ctypes.data, kw.ctypes.data, kx.shape[0], iters,
+                                         eig.ctypes.data)
+        return step, eig
+
+    def fista(self, smaps, data, traj, iters, step):
+        kx, ky, kz, kw = self._traj(traj)
+        data, smaps = self._c(data), self._c(smaps)
+        X = np.zeros((self.N[2], self.N[1], self.N[0]), self.ctype)
+        resid = np.zeros(iters)
+        self.lib.orc_fista(self.h, X.ctypes.data, _p(smaps), data.shape[0], data.ctypes.data,
+                           kx.ctypes.data, ky.ctypes.data, kz.ctypes.data, kw.ctypes.data,
+                           kx.shape[0], iters, float(step), resid.ctypes.data)
+        return X, resid
