@@ -1,0 +1,167 @@
+/*
+ * mrn.h — C ABI of the MI355X-native NUFFT/SENSE encoding operator.
+ *
+ * Drop-in boundary for uwmri/mri_recon's gridFFT (src/gridFFT.h:28-194) and the
+ * SENSE-CG / FISTA loops that call it (src/recon_lib.cpp:1523-2212).
+ *
+ * Conventions (those of the reference):
+ *  - arrays are column-major with x fastest (Blitz ColumnMajorArray<3>): an
+ *    image is complex64 [Nz][Ny][Nx] in C order; coil maps [C][Nz][Ny][Nx];
+ *    k-space of one frame [C][M]; a trajectory is four float32 [M] arrays
+ *    kx, ky, kz, kw in the reference's units (cycles/FOV, |k| <= N/2,
+ *    src/mri_data.h:50-57).  complex64 = interleaved float re, im.
+ *  - "image_to_kspace" is A (the reference's gridFFT::backward, a gather);
+ *    "kspace_to_image" is A^H (the reference's gridFFT::forward, a scatter).
+ *  - every pointer passed to an apply call is a HIP device pointer; the op owns
+ *    its grids, bins, tables and workspaces, and apply calls allocate nothing.
+ *  - every call returns an mrn_status and never exits the process.
+ *  - calls are asynchronous on the given stream (NULL = the op's own stream);
+ *    one op must not be used from two streams at once.
+ */
+#ifndef MRN_H
+#define MRN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MRN_OK = 0,
+  MRN_EINVAL = 1,       /* bad argument (null pointer, size, frame index) */
+  MRN_ENOMEM = 2,       /* device allocation failed */
+  MRN_EHIP = 3,         /* a HIP runtime call failed */
+  MRN_EUNSUPPORTED = 4, /* valid request this build does not implement */
+  MRN_ESTATE = 5        /* call order (e.g. apply before set_traj) */
+} mrn_status;
+
+/* Kernel types, same values as the reference enum (src/gridFFT.h:19-22). */
+enum { MRN_KERNEL_TRIANGLE = 0, MRN_KERNEL_KAISER = 1, MRN_KERNEL_SINC = 2, MRN_KERNEL_POLY = 3 };
+/* Kernel evaluation: the reference's 600-point lookup table with linear
+ * interpolation, or direct Kaiser-Bessel evaluation (smooth in k, used for
+ * trajectory gradients). */
+enum { MRN_KMODE_TABLE = 0, MRN_KMODE_EXACT_KB = 1 };
+/* Execution form: batched (all coils resident, binned gridding, fused and
+ * pruned FFT passes) or plain (one coil grid, the reference's per-coil
+ * structure on the GPU). */
+enum { MRN_FORM_BATCHED = 0, MRN_FORM_PLAIN = 1 };
+
+typedef struct mrn_params {
+  int32_t N[3];          /* image size Nx, Ny, Nz (precalc_gridding, gridFFT.cpp:475) */
+  float overgrid;        /* -overgrid, default 1.5 (gridFFT.cpp:32) */
+  float grid[3];         /* -grid_x/y/z; -1 = 16*ceil(overgrid*N/16)/N (503) */
+  float dwin[3];         /* -dwinX/Y/Z kernel half width; -1 = kernel default */
+  int32_t kernel;        /* MRN_KERNEL_* (-kaiser -triangle -sinc -poly_kernel) */
+  int32_t grid_in[3];    /* 1 = non-Cartesian axis (trajectory_type, 484-486) */
+  int32_t fft_in[3];     /* 1 = DFT along the axis (dft_needed, 488-498) */
+  float grid_scale[3];   /* -grid_scale_x/y/z (343-345) */
+  float k_rad;           /* A^H taper radius, default 9999 (51, 1096-1108) */
+  int32_t ncoils;        /* coils C */
+  int32_t nframes;       /* frames (gated bins / encodes), >= 1 */
+  int32_t kernel_mode;   /* MRN_KMODE_* */
+  int32_t form;          /* MRN_FORM_* */
+  int32_t coil_chunk;    /* coils per grid cell block (1,2,4,8,16); 0 = auto */
+  int32_t tile[3];       /* scatter tile edge (x,y,z); 0 = auto */
+  int32_t max_visits;    /* samples per scatter work item; 0 = auto */
+  int32_t reserved[7];
+} mrn_params;
+
+typedef struct mrn_op mrn_op;
+
+/* Defaults of gridFFT::gridFFT() (gridFFT.cpp:28-60) for a 3-D
+ * non-Cartesian acquisition. */
+void mrn_params_default(mrn_params *p);
+
+/* gridFFT::read_commandline (gridFFT.cpp:187-245): the same flags
+ * (-overgrid -dwinX/Y/Z -grid_x/y/z -grid_scale_x/y/z -grid_in_x/y/z
+ * -fft_in_x/y/z -kaiser -triangle -sinc -poly_kernel -fast_grid). */
+int mrn_parse_args(mrn_params *p, int argc, const char **argv);
+
+/* gridFFT::precalc_gridding + alloc_grid (gridFFT.cpp:66-75, 475-643):
+ * geometry, kernel tables, deapodisation windows, device buffers. */
+int mrn_op_create(const mrn_params *p, mrn_op **out);
+int mrn_op_destroy(mrn_op *op);
+
+/* Sx,Sy,Sz; og_sx,og_sy,og_sz (gridFFT.h:53-75); betaX/Y/Z (80-82). */
+int mrn_get_geometry(const mrn_op *op, int32_t S[3], int32_t og[3], float beta[3]);
+/* Host copies of the kernel table (grid_filterX/Y/Z, gridFFT.h:94-96) and
+ * deapodisation window (winx/y/z, 85-87) of one axis; *len receives the
+ * length; returns MRN_EINVAL when cap is too small. */
+int mrn_get_table(const mrn_op *op, int axis, float *host_out, int cap, int *len);
+int mrn_get_window(const mrn_op *op, int axis, float *host_out, int cap, int *len);
+
+/* Trajectory of one frame (the kx, ky, kz, kw arguments of gridFFT::forward /
+ * backward, gridFFT.h:118-141).  Sorts samples into tiles once; kept until
+ * replaced.  Device pointers, M samples. */
+int mrn_set_traj(mrn_op *op, int frame, int64_t M, const float *kx, const float *ky,
+                 const float *kz, const float *kw, void *stream);
+/* Coil sensitivities smaps[C][Nz][Ny][Nx] (the smap argument, gridFFT.h:119). */
+int mrn_set_smaps(mrn_op *op, const void *smaps, void *stream);
+
+/* A for every coil of one frame: data[C][M] = A_c X, overwriting data
+ * (gridFFT::backward, gridFFT.cpp:824-844); with sub != NULL,
+ * data = A_c X - sub (backward_residual, 856-876).  Samples with kw == 0 or
+ * outside the grid give 0 (chop_grid_backward, 1196-1271). */
+int mrn_image_to_kspace(mrn_op *op, int frame, const void *X, void *data, const void *sub,
+                        void *stream);
+/* A^H for every coil of one frame: X (+)= sum_c A_c^H (kw * taper * data_c)
+ * (gridFFT::forward, gridFFT.cpp:791-812, summed over the caller's coil loop,
+ * recon_lib.cpp:1565-1567).  accumulate = 1 reproduces the reference's +=. */
+int mrn_kspace_to_image(mrn_op *op, int frame, const void *data, void *X, int accumulate,
+                        void *stream);
+/* out = sum_c A_c^H W A_c X for one frame (the E'E of recon_lib.cpp:1614-1622),
+ * k-space kept on chip in the op's internal sample order. */
+int mrn_normal(mrn_op *op, int frame, const void *X, void *out, void *stream);
+
+/* One coil, one grid, trajectory per call: gridFFT::backward /
+ * backward_residual / forward with a smap (gridFFT.h:118-141).  These run
+ * the plain per-coil form (no binning); X accumulates in forward. */
+int mrn_coil_backward(mrn_op *op, const void *X, const void *smap, void *data, int64_t M,
+                      const float *kx, const float *ky, const float *kz, const float *kw,
+                      const void *sub, void *stream);
+int mrn_coil_forward(mrn_op *op, void *X, const void *smap, const void *data, int64_t M,
+                     const float *kx, const float *ky, const float *kz, const float *kw,
+                     void *stream);
+
+/* Measured k-space of one frame data[C][M] for the solvers below. */
+int mrn_set_kdata(mrn_op *op, int frame, const void *data, void *stream);
+
+/* SENSE-CG (recon_lib.cpp:1523-1759) over all frames: X[F][Nz][Ny][Nx] is
+ * zeroed and receives the image after `iters` iterations.  FP32 vectors, FP64
+ * inner products and step sizes on the device.  use_graph = 1 captures one
+ * iteration in a hipGraph and replays it.  trace (host, nullable) receives
+ * 4 doubles per iteration: sum|R|^2, Re alpha, Im alpha, sum|R_new|^2. */
+int mrn_cg(mrn_op *op, void *X, int iters, int use_graph, double *trace, void *stream);
+
+/* Power method for the FISTA step (recon_lib.cpp:1791-1915): iters
+ * applications of E'E starting from X = -0.5-0.5i; *step = 1/max_eig. */
+int mrn_power_method(mrn_op *op, int iters, double *step, void *stream);
+/* FISTA without regulariser (recon_lib.cpp:1925-2105, fista_update
+ * 2177-2212): X[F][Nz][Ny][Nx] starts at 0; resid (host, nullable) receives
+ * sum|R|^2 per iteration. */
+int mrn_fista(mrn_op *op, void *X, int iters, double step, int use_graph, double *resid,
+              void *stream);
+
+/* Trajectory gradient of 0.5*||A(x;k) - y||^2-type losses: given X and the
+ * upstream gradient g[C][M] of the k-space output of one frame, writes
+ * grad_k[M][3] (float32, reference k units) = 2 Re sum_c conj(g_cm) dA_cm/dk.
+ * No counterpart in the reference (its trajectory is fixed). */
+int mrn_traj_grad(mrn_op *op, int frame, const void *X, const void *g, float *grad_k,
+                  void *stream);
+
+/* Time the tunable launch shapes of one frame and keep the fastest
+ * (scatter tile, samples per work item, gather block).  No counterpart in
+ * the reference. */
+int mrn_autotune(mrn_op *op, int frame, void *stream);
+
+/* Wait for the op's work on `stream` (NULL = its own stream). */
+int mrn_sync(mrn_op *op, void *stream);
+
+const char *mrn_status_string(int status);
+/* Message of the op's last failure ("" if none). */
+const char *mrn_last_error(const mrn_op *op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MRN_H */

@@ -1,0 +1,112 @@
+"""ctypes binding of libmrn.so (declarations in include/mrn.h).
+
+The library is the only compute path: importing this module on a host where
+libmrn.so is missing or cannot be loaded raises, it never falls back to a
+CPU implementation.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmrn.so")
+
+MRN_OK, MRN_EINVAL, MRN_ENOMEM, MRN_EHIP, MRN_EUNSUPPORTED, MRN_ESTATE = range(6)
+KERNELS = {"triangle": 0, "kaiser": 1, "sinc": 2, "poly": 3}
+KMODES = {"table": 0, "exact_kb": 1}
+FORMS = {"batched": 0, "plain": 1}
+
+# every symbol include/mrn.h declares
+EXPORTS = [
+    "mrn_params_default", "mrn_parse_args", "mrn_op_create", "mrn_op_destroy",
+    "mrn_get_geometry", "mrn_get_table", "mrn_get_window", "mrn_set_traj", "mrn_set_smaps",
+    "mrn_image_to_kspace", "mrn_kspace_to_image", "mrn_normal", "mrn_coil_backward",
+    "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_power_method", "mrn_fista",
+    "mrn_traj_grad", "mrn_autotune", "mrn_sync", "mrn_status_string", "mrn_last_error",
+]
+
+
+class MrnParams(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32 * 3),
+        ("overgrid", ctypes.c_float),
+        ("grid", ctypes.c_float * 3),
+        ("dwin", ctypes.c_float * 3),
+        ("kernel", ctypes.c_int32),
+        ("grid_in", ctypes.c_int32 * 3),
+        ("fft_in", ctypes.c_int32 * 3),
+        ("grid_scale", ctypes.c_float * 3),
+        ("k_rad", ctypes.c_float),
+        ("ncoils", ctypes.c_int32),
+        ("nframes", ctypes.c_int32),
+        ("kernel_mode", ctypes.c_int32),
+        ("form", ctypes.c_int32),
+        ("coil_chunk", ctypes.c_int32),
+        ("tile", ctypes.c_int32 * 3),
+        ("max_visits", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class MrnError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("mrn status %d (%s): %s" % (status, status_string(status), msg))
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libmrn.so not built (run `python -m mri_recon_amd.build`): %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        P = ctypes.POINTER(MrnParams)
+        sig = {
+            "mrn_params_default": (None, [P]),
+            "mrn_parse_args": (i32, [P, i32, ctypes.POINTER(ctypes.c_char_p)]),
+            "mrn_op_create": (i32, [P, ctypes.POINTER(vp)]),
+            "mrn_op_destroy": (i32, [vp]),
+            "mrn_get_geometry": (i32, [vp, vp, vp, vp]),
+            "mrn_get_table": (i32, [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_int)]),
+            "mrn_get_window": (i32, [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_int)]),
+            "mrn_set_traj": (i32, [vp, i32, i64, vp, vp, vp, vp, vp]),
+            "mrn_set_smaps": (i32, [vp, vp, vp]),
+            "mrn_image_to_kspace": (i32, [vp, i32, vp, vp, vp, vp]),
+            "mrn_kspace_to_image": (i32, [vp, i32, vp, vp, i32, vp]),
+            "mrn_normal": (i32, [vp, i32, vp, vp, vp]),
+            "mrn_coil_backward": (i32, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]),
+            "mrn_coil_forward": (i32, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
+            "mrn_set_kdata": (i32, [vp, i32, vp, vp]),
+            "mrn_cg": (i32, [vp, vp, i32, i32, vp, vp]),
+            "mrn_power_method": (i32, [vp, i32, ctypes.POINTER(dbl), vp]),
+            "mrn_fista": (i32, [vp, vp, i32, dbl, i32, vp, vp]),
+            "mrn_traj_grad": (i32, [vp, i32, vp, vp, vp, vp]),
+            "mrn_autotune": (i32, [vp, i32, vp]),
+            "mrn_sync": (i32, [vp, vp]),
+            "mrn_status_string": (ctypes.c_char_p, [i32]),
+            "mrn_last_error": (ctypes.c_char_p, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(s):
+    try:
+        return lib().mrn_status_string(int(s)).decode()
+    except Exception:
+        return "?"
+
+
+def check(status, handle=None):
+    if status != MRN_OK:
+        msg = ""
+        if handle:
+            msg = (lib().mrn_last_error(handle) or b"").decode()
+        raise MrnError(status, msg)

@@ -1,0 +1,581 @@
+// C-ABI entry points: trajectory/map/data setup, the operator A, A^H, E'E in
+// the batched and plain forms, and the CG / power-method / FISTA drivers.
+//
+// Batched form, one frame (all coil blocks per launch):
+//   A   : pre+z FFT (fused set_image) -> y FFT -> x FFT -> gather
+//   A^H : tiled scatter -> x IFFT -> y IFFT -> z IFFT+accumulate (per block)
+// with the FFT pruned to the lines and points that are non-zero or read.
+// Plain form, one coil at a time (the reference's structure on the GPU,
+// gridFFT.cpp:791-906): zero grid, set_image, per-axis FFT, per-sample gather;
+// zero grid, per-sample atomic scatter, per-axis IFFT, accumulate.
+#include <math.h>
+#include <string.h>
+
+#include <cstdio>
+
+#include "mrn_internal.hpp"
+
+namespace mrn {
+int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, hipStream_t);
+int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
+int launch_gather_plain(mrn_op *, const float2 *, float2 *, int64_t, const float *, const float *,
+                        const float *, const float *, const float2 *, hipStream_t);
+int launch_scatter_plain(mrn_op *, float2 *, const float2 *, int64_t, const float *, const float *,
+                         const float *, const float *, hipStream_t);
+int build_bins(mrn_op *, FrameData &, int64_t, const float *, const float *, const float *,
+               const float *, hipStream_t);
+int op_zero(mrn_op *, float2 *, int64_t, hipStream_t);
+int op_fill(mrn_op *, float2 *, int64_t, float2, hipStream_t);
+int op_copy(mrn_op *, float2 *, const float2 *, int64_t, hipStream_t);
+int op_smaps_in(mrn_op *, float2 *, const float2 *, hipStream_t);
+int op_permute_in(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
+int op_permute_out(mrn_op *, float2 *, const float2 *, const FrameData &, const float2 *, hipStream_t);
+int op_set_image_plain(mrn_op *, float2 *, const float2 *, const float2 *, const PrePostArgs &, hipStream_t);
+int op_accumulate_plain(mrn_op *, const float2 *, float2 *, const float2 *, const PrePostArgs &, hipStream_t);
+int op_cg_alpha(mrn_op *, const float2 *, const float2 *, const float2 *, int64_t, double *, hipStream_t);
+int op_cg_update(mrn_op *, float2 *, float2 *, float2 *, const float2 *, int64_t, double *, hipStream_t);
+int op_power_step(mrn_op *, float2 *, const float2 *, int64_t, double *, hipStream_t);
+int op_fista_momentum(mrn_op *, float2 *, float2 *, int64_t, hipStream_t);
+int op_fista_finish(mrn_op *, float2 *, float2 *, int64_t, float, double *, hipStream_t);
+int traj_grad_impl(mrn_op *, int, const float2 *, const float2 *, float *, hipStream_t);
+int autotune_impl(mrn_op *, int, hipStream_t);
+
+static inline bool batched(const mrn_op *op) { return op->p.form == MRN_FORM_BATCHED; }
+static inline int64_t ncell(const mrn_op *op) {
+  return (int64_t)op->g.S[0] * op->g.S[1] * op->g.S[2];
+}
+static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+static inline float fft_scale(int S) { return (float)(1. / sqrt((float)S)); }
+
+static PrePostArgs prepost(const mrn_op *op, int ch, int overwrite, int CC) {
+  PrePostArgs a;
+  for (int q = 0; q < 3; q++) {
+    a.N[q] = op->g.N[q];
+    a.S[q] = op->g.S[q];
+    a.og[q] = op->g.og[q];
+  }
+  a.CC = CC;
+  a.chunk_grid = ncell(op) * CC;
+  a.chunk_smap = op->nvox * CC;
+  a.ch = ch;
+  a.overwrite = overwrite;
+  a.scale = fft_scale(op->g.S[2]);
+  return a;
+}
+
+// ---------------------------------------------------- batched transforms
+// image -> full k-space grid of every coil block
+static int fwd_grid(mrn_op *op, const float2 *X, hipStream_t st) {
+  const Geom &g = op->g;
+  const int CC = g.CC;
+  const int64_t cg = ncell(op) * CC;
+  MRN_TRY(fft_pre_z(op, op->grid, X, op->smaps, prepost(op, 0, 0, CC), st));
+  LinesArgs y{};
+  y.off0 = (int64_t)g.og[0] * CC;
+  y.ystride = (int64_t)g.S[1] * g.S[0] * CC;
+  y.zstride = cg;
+  y.pstride = (int64_t)g.S[0] * CC;
+  y.rstride = 0;
+  y.run = g.N[0] * CC;
+  y.nlines = g.N[0] * CC;
+  y.ld0 = g.og[1];
+  y.ld1 = g.og[1] + g.N[1];
+  y.st0 = 0;
+  y.st1 = g.S[1];
+  y.scale = fft_scale(g.S[1]);
+  MRN_TRY(fft_lines(op, g.S[1], -1, op->grid, y, dim3(cdiv(y.nlines, FFT_LINES), g.S[2], op->nch),
+                    op->tw_d[1], st));
+  LinesArgs x{};
+  x.off0 = 0;
+  x.ystride = (int64_t)g.S[1] * g.S[0] * CC;
+  x.zstride = cg;
+  x.pstride = CC;
+  x.rstride = (int64_t)g.S[0] * CC;
+  x.run = CC;
+  x.nlines = g.S[1] * CC;
+  x.ld0 = g.og[0];
+  x.ld1 = g.og[0] + g.N[0];
+  x.st0 = 0;
+  x.st1 = g.S[0];
+  x.scale = fft_scale(g.S[0]);
+  MRN_TRY(fft_lines(op, g.S[0], -1, op->grid, x, dim3(cdiv(x.nlines, FFT_LINES), g.S[2], op->nch),
+                    op->tw_d[0], st));
+  return MRN_OK;
+}
+
+int fwd_grid_pub(mrn_op *op, const float2 *X, hipStream_t st) { return fwd_grid(op, X, st); }
+
+// full grids of every coil block -> image (X = or += sum over coils)
+static int adj_grid(mrn_op *op, float2 *X, bool overwrite, hipStream_t st) {
+  const Geom &g = op->g;
+  const int CC = g.CC;
+  const int64_t cg = ncell(op) * CC;
+  LinesArgs x{};
+  x.off0 = 0;
+  x.ystride = (int64_t)g.S[1] * g.S[0] * CC;
+  x.zstride = cg;
+  x.pstride = CC;
+  x.rstride = (int64_t)g.S[0] * CC;
+  x.run = CC;
+  x.nlines = g.S[1] * CC;
+  x.ld0 = 0;
+  x.ld1 = g.S[0];
+  x.st0 = g.og[0];
+  x.st1 = g.og[0] + g.N[0];
+  x.scale = fft_scale(g.S[0]);
+  MRN_TRY(fft_lines(op, g.S[0], 1, op->grid, x, dim3(cdiv(x.nlines, FFT_LINES), g.S[2], op->nch),
+                    op->tw_d[0], st));
+  LinesArgs y{};
+  y.off0 = (int64_t)g.og[0] * CC;
+  y.ystride = (int64_t)g.S[1] * g.S[0] * CC;
+  y.zstride = cg;
+  y.pstride = (int64_t)g.S[0] * CC;
+  y.run = g.N[0] * CC;
+  y.nlines = g.N[0] * CC;
+  y.ld0 = 0;
+  y.ld1 = g.S[1];
+  y.st0 = g.og[1];
+  y.st1 = g.og[1] + g.N[1];
+  y.scale = fft_scale(g.S[1]);
+  MRN_TRY(fft_lines(op, g.S[1], 1, op->grid, y, dim3(cdiv(y.nlines, FFT_LINES), g.S[2], op->nch),
+                    op->tw_d[1], st));
+  for (int ch = 0; ch < op->nch; ch++)
+    MRN_TRY(fft_post_z(op, op->grid, X, op->smaps, prepost(op, ch, overwrite && ch == 0, CC), st));
+  return MRN_OK;
+}
+
+// ---------------------------------------------------- plain transforms
+static int plain_passes(mrn_op *op, int dir, hipStream_t st) {
+  const Geom &g = op->g;
+  float2 *G = op->cgrid;
+  const int64_t plane = (int64_t)g.S[1] * g.S[0];
+  auto zpass = [&](bool pruned) -> int {
+    if (!g.fft_in[2]) return MRN_OK;
+    LinesArgs a{};
+    const int nx = pruned ? g.N[0] : g.S[0], ny = pruned ? g.N[1] : g.S[1];
+    a.off0 = pruned ? (int64_t)g.og[1] * g.S[0] + g.og[0] : 0;
+    a.ystride = g.S[0];
+    a.zstride = 0;
+    a.pstride = plane;
+    a.run = nx;
+    a.nlines = nx;
+    a.ld0 = 0;
+    a.ld1 = g.S[2];
+    a.st0 = 0;
+    a.st1 = g.S[2];
+    a.scale = fft_scale(g.S[2]);
+    return fft_lines(op, g.S[2], dir, G, a, dim3(cdiv(nx, FFT_LINES), ny, 1), op->tw_d[2], st);
+  };
+  auto ypass = [&](bool pruned) -> int {
+    if (!g.fft_in[1]) return MRN_OK;
+    LinesArgs a{};
+    const int nx = pruned ? g.N[0] : g.S[0];
+    a.off0 = pruned ? g.og[0] : 0;
+    a.ystride = plane;
+    a.pstride = g.S[0];
+    a.run = nx;
+    a.nlines = nx;
+    a.ld0 = 0;
+    a.ld1 = g.S[1];
+    a.st0 = 0;
+    a.st1 = g.S[1];
+    a.scale = fft_scale(g.S[1]);
+    return fft_lines(op, g.S[1], dir, G, a, dim3(cdiv(nx, FFT_LINES), g.S[2], 1), op->tw_d[1], st);
+  };
+  auto xpass = [&]() -> int {
+    if (!g.fft_in[0]) return MRN_OK;
+    LinesArgs a{};
+    a.ystride = plane;
+    a.pstride = 1;
+    a.rstride = g.S[0];
+    a.run = 1;
+    a.nlines = g.S[1];
+    a.ld0 = 0;
+    a.ld1 = g.S[0];
+    a.st0 = 0;
+    a.st1 = g.S[0];
+    a.scale = fft_scale(g.S[0]);
+    return fft_lines(op, g.S[0], dir, G, a, dim3(cdiv(g.S[1], FFT_LINES), g.S[2], 1), op->tw_d[0], st);
+  };
+  if (dir < 0) {  // do_fft: z over crop lines, y over crop x, x over all (gridFFT.cpp:645-671)
+    MRN_TRY(zpass(true));
+    MRN_TRY(ypass(true));
+    MRN_TRY(xpass());
+  } else {  // do_ifft: x, y, z over all lines (gridFFT.cpp:673-711)
+    MRN_TRY(xpass());
+    MRN_TRY(ypass(false));
+    MRN_TRY(zpass(false));
+  }
+  return MRN_OK;
+}
+
+static int plain_A_coil(mrn_op *op, const float2 *X, const float2 *smap, float2 *out, int64_t M,
+                        const float *kx, const float *ky, const float *kz, const float *kw,
+                        const float2 *sub, hipStream_t st) {
+  MRN_TRY(op_zero(op, op->cgrid, ncell(op), st));
+  MRN_TRY(op_set_image_plain(op, op->cgrid, X, smap, prepost(op, 0, 0, 1), st));
+  MRN_TRY(plain_passes(op, -1, st));
+  return launch_gather_plain(op, op->cgrid, out, M, kx, ky, kz, kw, sub, st);
+}
+
+static int plain_AH_coil(mrn_op *op, float2 *X, const float2 *smap, const float2 *data, int64_t M,
+                         const float *kx, const float *ky, const float *kz, const float *kw,
+                         hipStream_t st) {
+  MRN_TRY(op_zero(op, op->cgrid, ncell(op), st));
+  MRN_TRY(launch_scatter_plain(op, op->cgrid, data, M, kx, ky, kz, kw, st));
+  MRN_TRY(plain_passes(op, 1, st));
+  return op_accumulate_plain(op, op->cgrid, X, smap, prepost(op, 0, 0, 1), st);
+}
+
+static const float *traw(const FrameData &fr, int q) { return fr.traw + (int64_t)q * fr.M; }
+
+// ---------------------------------------------------- frame-level engine
+static int check_frame(mrn_op *op, int f, bool need_kdata) {
+  if (f < 0 || f >= op->F) return fail(op, MRN_EINVAL, "frame index out of range");
+  if (!op->frames[f].has_traj) return fail(op, MRN_ESTATE, "trajectory of this frame not set");
+  if (need_kdata && !op->frames[f].has_kdata) return fail(op, MRN_ESTATE, "k-space data of this frame not set");
+  return MRN_OK;
+}
+
+// out = E'E X of frame f (out overwritten)
+static int normal_frame(mrn_op *op, int f, const float2 *X, float2 *out, hipStream_t st) {
+  FrameData &fr = op->frames[f];
+  if (batched(op)) {
+    MRN_TRY(fwd_grid(op, X, st));
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, st));
+    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
+    return adj_grid(op, out, true, st);
+  }
+  MRN_TRY(op_zero(op, out, op->nvox, st));
+  for (int c = 0; c < op->C; c++) {
+    const float2 *sm = op->smaps + (int64_t)c * op->nvox;
+    MRN_TRY(plain_A_coil(op, X, sm, op->kplain, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2), traw(fr, 3), nullptr, st));
+    MRN_TRY(plain_AH_coil(op, out, sm, op->kplain, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2), traw(fr, 3), st));
+  }
+  return MRN_OK;
+}
+
+// R = E^H W (E X - d) of frame f (R overwritten)
+static int grad_frame(mrn_op *op, int f, const float2 *X, float2 *R, hipStream_t st) {
+  FrameData &fr = op->frames[f];
+  if (batched(op)) {
+    MRN_TRY(fwd_grid(op, X, st));
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, fr.kdata, st));
+    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
+    return adj_grid(op, R, true, st);
+  }
+  MRN_TRY(op_zero(op, R, op->nvox, st));
+  for (int c = 0; c < op->C; c++) {
+    const float2 *sm = op->smaps + (int64_t)c * op->nvox;
+    MRN_TRY(plain_A_coil(op, X, sm, op->kplain, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2), traw(fr, 3),
+                         fr.kraw + (int64_t)c * fr.M, st));
+    MRN_TRY(plain_AH_coil(op, R, sm, op->kplain, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2), traw(fr, 3), st));
+  }
+  return MRN_OK;
+}
+
+// R = E^H W d of frame f (R overwritten)
+static int adjoint_data_frame(mrn_op *op, int f, float2 *R, hipStream_t st) {
+  FrameData &fr = op->frames[f];
+  if (batched(op)) {
+    MRN_TRY(launch_scatter(op, op->grid, fr.kdata, fr, st));
+    return adj_grid(op, R, true, st);
+  }
+  MRN_TRY(op_zero(op, R, op->nvox, st));
+  for (int c = 0; c < op->C; c++)
+    MRN_TRY(plain_AH_coil(op, R, op->smaps + (int64_t)c * op->nvox, fr.kraw + (int64_t)c * fr.M, fr.M,
+                          traw(fr, 0), traw(fr, 1), traw(fr, 2), traw(fr, 3), st));
+  return MRN_OK;
+}
+
+static int ensure_solver(mrn_op *op, int trace_len) {
+  const int64_t n = (int64_t)op->F * op->nvox;
+  if (!op->vR) {
+    MRN_HIP(op, hipMalloc(&op->vR, sizeof(float2) * n));
+    MRN_HIP(op, hipMalloc(&op->vP, sizeof(float2) * n));
+    MRN_HIP(op, hipMalloc(&op->vL, sizeof(float2) * n));
+    MRN_HIP(op, hipMalloc(&op->vX0, sizeof(float2) * n));
+  }
+  if (trace_len > op->dtrace_cap) {
+    hipFree(op->dtrace);
+    op->dtrace = nullptr;
+    MRN_HIP(op, hipMalloc(&op->dtrace, sizeof(double) * trace_len));
+    op->dtrace_cap = trace_len;
+  }
+  return MRN_OK;
+}
+
+// Run `body` iters times on the caller's stream, or capture one call of it
+// in a hipGraph on the op's stream and replay it.
+template <class Body>
+static int run_iters(mrn_op *op, int iters, bool graph, hipStream_t st, Body body) {
+  if (!graph) {
+    for (int it = 0; it < iters; it++) MRN_TRY(body(st));
+    return MRN_OK;
+  }
+  hipEvent_t ev0, ev1;
+  MRN_HIP(op, hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+  MRN_HIP(op, hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+  MRN_HIP(op, hipEventRecord(ev0, st));
+  MRN_HIP(op, hipStreamWaitEvent(op->stream, ev0, 0));
+  hipGraph_t graph_h = nullptr;
+  hipGraphExec_t exec = nullptr;
+  MRN_HIP(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+  int rc = body(op->stream);
+  hipError_t ec = hipStreamEndCapture(op->stream, &graph_h);
+  if (rc != MRN_OK) {
+    if (graph_h) hipGraphDestroy(graph_h);
+    return rc;
+  }
+  MRN_HIP(op, ec);
+  MRN_HIP(op, hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0));
+  for (int it = 0; it < iters; it++) MRN_HIP(op, hipGraphLaunch(exec, op->stream));
+  MRN_HIP(op, hipEventRecord(ev1, op->stream));
+  MRN_HIP(op, hipStreamWaitEvent(st, ev1, 0));
+  MRN_HIP(op, hipStreamSynchronize(op->stream));
+  hipGraphExecDestroy(exec);
+  hipGraphDestroy(graph_h);
+  hipEventDestroy(ev0);
+  hipEventDestroy(ev1);
+  return MRN_OK;
+}
+
+}  // namespace mrn
+
+using namespace mrn;
+
+extern "C" {
+
+int mrn_set_traj(mrn_op *op, int f, int64_t M, const float *kx, const float *ky, const float *kz,
+                 const float *kw, void *stream) {
+  if (!op) return MRN_EINVAL;
+  if (f < 0 || f >= op->F) return fail(op, MRN_EINVAL, "frame index out of range");
+  if (M < 0 || M > INT32_MAX) return fail(op, MRN_EINVAL, "sample count out of range");
+  if (M > 0 && (!kx || !ky || !kz || !kw)) return fail(op, MRN_EINVAL, "null trajectory pointer");
+  hipStream_t st = pick(op, stream);
+  FrameData &fr = op->frames[f];
+  hipFree(fr.traw);
+  fr.traw = nullptr;
+  MRN_HIP(op, hipMalloc(&fr.traw, sizeof(float) * 4 * (M > 0 ? M : 1)));
+  const float *src[4] = {kx, ky, kz, kw};
+  for (int q = 0; q < 4; q++)
+    if (M > 0) MRN_HIP(op, hipMemcpyAsync(fr.traw + q * M, src[q], sizeof(float) * M, hipMemcpyDeviceToDevice, st));
+  fr.M = M;
+  if (batched(op)) {
+    MRN_TRY(build_bins(op, fr, M, fr.traw, fr.traw + M, fr.traw + 2 * M, fr.traw + 3 * M, st));
+    const int64_t need = fr.Mv * op->g.CC * op->nch;
+    if (need > op->kwork_cap) {
+      hipFree(op->kwork);
+      op->kwork = nullptr;
+      MRN_HIP(op, hipMalloc(&op->kwork, sizeof(float2) * (need > 0 ? need : 1)));
+      op->kwork_cap = need;
+    }
+    if (fr.has_kdata) {  // re-sort stored data for the new trajectory
+      hipFree(fr.kdata);
+      fr.kdata = nullptr;
+      MRN_HIP(op, hipMalloc(&fr.kdata, sizeof(float2) * (need > 0 ? need : 1)));
+      MRN_TRY(op_permute_in(op, fr.kdata, fr.kraw, fr, st));
+    }
+  }
+  if (M > op->kplain_cap) {
+    hipFree(op->kplain);
+    op->kplain = nullptr;
+    MRN_HIP(op, hipMalloc(&op->kplain, sizeof(float2) * M));
+    op->kplain_cap = M;
+  }
+  fr.has_traj = true;
+  MRN_HIP(op, hipStreamSynchronize(st));
+  return MRN_OK;
+}
+
+int mrn_set_smaps(mrn_op *op, const void *smaps, void *stream) {
+  if (!op || !smaps) return MRN_EINVAL;
+  hipStream_t st = pick(op, stream);
+  if (batched(op)) {
+    MRN_TRY(op_smaps_in(op, op->smaps, (const float2 *)smaps, st));
+  } else {
+    MRN_HIP(op, hipMemcpyAsync(op->smaps, smaps, sizeof(float2) * op->nvox * op->C, hipMemcpyDeviceToDevice, st));
+  }
+  return MRN_OK;
+}
+
+int mrn_set_kdata(mrn_op *op, int f, const void *data, void *stream) {
+  if (!op || !data) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  hipStream_t st = pick(op, stream);
+  FrameData &fr = op->frames[f];
+  hipFree(fr.kraw);
+  fr.kraw = nullptr;
+  MRN_HIP(op, hipMalloc(&fr.kraw, sizeof(float2) * op->C * (fr.M > 0 ? fr.M : 1)));
+  MRN_HIP(op, hipMemcpyAsync(fr.kraw, data, sizeof(float2) * op->C * fr.M, hipMemcpyDeviceToDevice, st));
+  if (batched(op)) {
+    const int64_t need = fr.Mv * op->g.CC * op->nch;
+    hipFree(fr.kdata);
+    fr.kdata = nullptr;
+    MRN_HIP(op, hipMalloc(&fr.kdata, sizeof(float2) * (need > 0 ? need : 1)));
+    MRN_TRY(op_permute_in(op, fr.kdata, fr.kraw, fr, st));
+  }
+  fr.has_kdata = true;
+  MRN_HIP(op, hipStreamSynchronize(st));
+  return MRN_OK;
+}
+
+int mrn_image_to_kspace(mrn_op *op, int f, const void *X, void *data, const void *sub, void *stream) {
+  if (!op || !X || !data) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  hipStream_t st = pick(op, stream);
+  FrameData &fr = op->frames[f];
+  if (batched(op)) {
+    MRN_TRY(fwd_grid(op, (const float2 *)X, st));
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, st));
+    return op_permute_out(op, (float2 *)data, op->kwork, fr, (const float2 *)sub, st);
+  }
+  for (int c = 0; c < op->C; c++)
+    MRN_TRY(plain_A_coil(op, (const float2 *)X, op->smaps + (int64_t)c * op->nvox,
+                         (float2 *)data + (int64_t)c * fr.M, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2),
+                         traw(fr, 3), sub ? (const float2 *)sub + (int64_t)c * fr.M : nullptr, st));
+  return MRN_OK;
+}
+
+int mrn_kspace_to_image(mrn_op *op, int f, const void *data, void *X, int accumulate, void *stream) {
+  if (!op || !X || !data) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  hipStream_t st = pick(op, stream);
+  FrameData &fr = op->frames[f];
+  if (batched(op)) {
+    MRN_TRY(op_permute_in(op, op->kwork, (const float2 *)data, fr, st));
+    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
+    return adj_grid(op, (float2 *)X, !accumulate, st);
+  }
+  if (!accumulate) MRN_TRY(op_zero(op, (float2 *)X, op->nvox, st));
+  for (int c = 0; c < op->C; c++)
+    MRN_TRY(plain_AH_coil(op, (float2 *)X, op->smaps + (int64_t)c * op->nvox,
+                          (const float2 *)data + (int64_t)c * fr.M, fr.M, traw(fr, 0), traw(fr, 1),
+                          traw(fr, 2), traw(fr, 3), st));
+  return MRN_OK;
+}
+
+int mrn_normal(mrn_op *op, int f, const void *X, void *out, void *stream) {
+  if (!op || !X || !out) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  return normal_frame(op, f, (const float2 *)X, (float2 *)out, pick(op, stream));
+}
+
+int mrn_coil_backward(mrn_op *op, const void *X, const void *smap, void *data, int64_t M,
+                      const float *kx, const float *ky, const float *kz, const float *kw,
+                      const void *sub, void *stream) {
+  if (!op || !X || !data || (M > 0 && (!kx || !ky || !kz || !kw))) return MRN_EINVAL;
+  return plain_A_coil(op, (const float2 *)X, (const float2 *)smap, (float2 *)data, M, kx, ky, kz, kw,
+                      (const float2 *)sub, pick(op, stream));
+}
+
+int mrn_coil_forward(mrn_op *op, void *X, const void *smap, const void *data, int64_t M,
+                     const float *kx, const float *ky, const float *kz, const float *kw, void *stream) {
+  if (!op || !X || !data || (M > 0 && (!kx || !ky || !kz || !kw))) return MRN_EINVAL;
+  return plain_AH_coil(op, (float2 *)X, (const float2 *)smap, (const float2 *)data, M, kx, ky, kz, kw,
+                       pick(op, stream));
+}
+
+int mrn_cg(mrn_op *op, void *Xv, int iters, int use_graph, double *trace, void *stream) {
+  if (!op || !Xv || iters < 0) return MRN_EINVAL;
+  for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, true));
+  MRN_TRY(ensure_solver(op, 4 * (iters > 0 ? iters : 1)));
+  hipStream_t st = pick(op, stream);
+  float2 *X = (float2 *)Xv;
+  const int64_t n = (int64_t)op->F * op->nvox;
+  double *tr = trace ? op->dtrace : nullptr;
+  MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
+  for (int f = 0; f < op->F; f++) MRN_TRY(adjoint_data_frame(op, f, op->vR + f * op->nvox, st));
+  MRN_TRY(op_copy(op, op->vP, op->vR, n, st));
+  MRN_TRY(op_zero(op, X, n, st));
+  const bool plain = !batched(op);
+  double host_scal[16];
+  auto body = [&](hipStream_t s) -> int {
+    for (int f = 0; f < op->F; f++)
+      MRN_TRY(normal_frame(op, f, op->vP + f * op->nvox, op->vL + f * op->nvox, s));
+    MRN_TRY(op_cg_alpha(op, op->vR, op->vP, op->vL, n, tr, s));
+    if (plain) {  // the reference's host-side scalar round trip
+      MRN_HIP(op, hipMemcpyAsync(host_scal, op->dscal, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+      MRN_HIP(op, hipStreamSynchronize(s));
+    }
+    MRN_TRY(op_cg_update(op, X, op->vR, op->vP, op->vL, n, tr, s));
+    if (plain) {
+      MRN_HIP(op, hipMemcpyAsync(host_scal, op->dscal, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+      MRN_HIP(op, hipStreamSynchronize(s));
+    }
+    return MRN_OK;
+  };
+  MRN_TRY(run_iters(op, iters, use_graph && !plain, st, body));
+  if (trace && iters > 0) {
+    MRN_HIP(op, hipMemcpyAsync(trace, op->dtrace, sizeof(double) * 4 * iters, hipMemcpyDeviceToHost, st));
+    MRN_HIP(op, hipStreamSynchronize(st));
+  }
+  return MRN_OK;
+}
+
+int mrn_power_method(mrn_op *op, int iters, double *step, void *stream) {
+  if (!op || iters <= 0) return MRN_EINVAL;
+  for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, false));
+  MRN_TRY(ensure_solver(op, iters));
+  hipStream_t st = pick(op, stream);
+  const int64_t n = (int64_t)op->F * op->nvox;
+  MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
+  MRN_TRY(op_fill(op, op->vX0, n, make_float2(-0.5f, -0.5f), st));
+  for (int it = 0; it < iters; it++) {
+    for (int f = 0; f < op->F; f++)
+      MRN_TRY(normal_frame(op, f, op->vX0 + f * op->nvox, op->vR + f * op->nvox, st));
+    MRN_TRY(op_power_step(op, op->vX0, op->vR, n, op->dtrace, st));
+  }
+  double sc[16];
+  MRN_HIP(op, hipMemcpyAsync(sc, op->dscal, sizeof(double) * 16, hipMemcpyDeviceToHost, st));
+  MRN_HIP(op, hipStreamSynchronize(st));
+  if (step) *step = (double)(float)(1. / sc[4]);
+  return MRN_OK;
+}
+
+int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, double *resid, void *stream) {
+  if (!op || !Xv || iters < 0) return MRN_EINVAL;
+  for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, true));
+  MRN_TRY(ensure_solver(op, iters > 0 ? iters : 1));
+  hipStream_t st = pick(op, stream);
+  float2 *X = (float2 *)Xv;
+  const int64_t n = (int64_t)op->F * op->nvox;
+  MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
+  MRN_TRY(op_zero(op, X, n, st));
+  MRN_TRY(op_zero(op, op->vP, n, st));  // X_old
+  const float stf = (float)step;
+  auto body = [&](hipStream_t s) -> int {
+    MRN_TRY(op_fista_momentum(op, X, op->vP, n, s));
+    for (int f = 0; f < op->F; f++)
+      MRN_TRY(grad_frame(op, f, X + f * op->nvox, op->vR + f * op->nvox, s));
+    return op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s);
+  };
+  MRN_TRY(run_iters(op, iters, use_graph && batched(op), st, body));
+  if (resid && iters > 0) {
+    MRN_HIP(op, hipMemcpyAsync(resid, op->dtrace, sizeof(double) * iters, hipMemcpyDeviceToHost, st));
+    MRN_HIP(op, hipStreamSynchronize(st));
+  }
+  return MRN_OK;
+}
+
+int mrn_traj_grad(mrn_op *op, int f, const void *X, const void *g, float *grad_k, void *stream) {
+  if (!op || !X || !g || !grad_k) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  if (!batched(op)) return fail(op, MRN_EUNSUPPORTED, "trajectory gradient needs the batched form");
+  return traj_grad_impl(op, f, (const float2 *)X, (const float2 *)g, grad_k, pick(op, stream));
+}
+
+int mrn_autotune(mrn_op *op, int f, void *stream) {
+  if (!op) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  if (!batched(op)) return MRN_OK;
+  return autotune_impl(op, f, pick(op, stream));
+}
+
+int mrn_sync(mrn_op *op, void *stream) {
+  if (!op) return MRN_EINVAL;
+  MRN_HIP(op, hipStreamSynchronize(pick(op, stream)));
+  return MRN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,440 @@
+// LDS-staged mixed-radix pencil FFT on the oversampled grid.
+//
+// Stands in for fft3 over FFTW 1-D plans (ArrayTemplates.cpp:276-427) as
+// called by do_fft / do_ifft (gridFFT.cpp:645-711): unitary per-axis DFT,
+// sign -1 forward (image -> k) and +1 inverse, scaled by 1/sqrt(S) per axis.
+//
+// A workgroup owns FFT_LINES (16) lines of one axis.  The lines are staged in
+// LDS (one padded row per line), transformed by an in-place Stockham
+// autosort (radix 8/4/2/3/5 butterflies in registers, twiddles from an LDS
+// table computed in double on the host) and written back.  Loads skip the
+// points known to be zero (pruning) and stores skip the points nobody reads.
+// The first forward pass is fused with set_image (deapodisation x coil map x
+// zero pad, gridFFT.cpp:972-986) and the last inverse pass with accumulate
+// (deapodisation x conj(coil map), summed over the coils of a block,
+// gridFFT.cpp:926-941).
+#include "mrn_internal.hpp"
+
+namespace mrn {
+
+__host__ __device__ constexpr int radix_at(int S, int i) {
+  int m = S, r = 0;
+  for (int k = 0; k <= i; k++) {
+    if (m % 8 == 0) r = 8;
+    else if (m % 4 == 0) r = 4;
+    else if (m % 2 == 0) r = 2;
+    else if (m % 3 == 0) r = 3;
+    else if (m % 5 == 0) r = 5;
+    else return 0;
+    m /= r;
+  }
+  return r;
+}
+
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {  // (DIR * i) * a
+  return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+__device__ __forceinline__ float2 add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 scl(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2 *x);
+
+template <>
+__device__ __forceinline__ void dft<2, -1>(float2 *x) {
+  float2 a = x[0], b = x[1];
+  x[0] = add(a, b);
+  x[1] = sub(a, b);
+}
+template <>
+__device__ __forceinline__ void dft<2, 1>(float2 *x) { dft<2, -1>(x); }
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2, float2 &x3) {
+  float2 t0 = add(x0, x2), t1 = sub(x0, x2), t2 = add(x1, x3), t3 = mul_i<DIR>(sub(x1, x3));
+  x0 = add(t0, t2);
+  x2 = sub(t0, t2);
+  x1 = add(t1, t3);
+  x3 = sub(t1, t3);
+}
+template <>
+__device__ __forceinline__ void dft<4, -1>(float2 *x) { dft4<-1>(x[0], x[1], x[2], x[3]); }
+template <>
+__device__ __forceinline__ void dft<4, 1>(float2 *x) { dft4<1>(x[0], x[1], x[2], x[3]); }
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2 *x) {
+  const float r2 = 0.70710678118654752f;
+  float2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+  float2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  // twiddles W8^k, W8 = exp(DIR * 2 pi i / 8)
+  o1 = scl(add(o1, mul_i<DIR>(o1)), r2);
+  o2 = mul_i<DIR>(o2);
+  o3 = scl(sub(mul_i<DIR>(o3), o3), r2);
+  x[0] = add(e0, o0);
+  x[4] = sub(e0, o0);
+  x[1] = add(e1, o1);
+  x[5] = sub(e1, o1);
+  x[2] = add(e2, o2);
+  x[6] = sub(e2, o2);
+  x[3] = add(e3, o3);
+  x[7] = sub(e3, o3);
+}
+template <>
+__device__ __forceinline__ void dft<8, -1>(float2 *x) { dft8<-1>(x); }
+template <>
+__device__ __forceinline__ void dft<8, 1>(float2 *x) { dft8<1>(x); }
+
+template <int DIR>
+__device__ __forceinline__ void dft3(float2 *x) {
+  const float s = 0.86602540378443865f;
+  float2 t = add(x[1], x[2]), u = mul_i<DIR>(sub(x[1], x[2]));
+  float2 m = sub(x[0], scl(t, 0.5f));
+  x[0] = add(x[0], t);
+  u = scl(u, s);
+  x[1] = add(m, u);
+  x[2] = sub(m, u);
+}
+template <>
+__device__ __forceinline__ void dft<3, -1>(float2 *x) { dft3<-1>(x); }
+template <>
+__device__ __forceinline__ void dft<3, 1>(float2 *x) { dft3<1>(x); }
+
+template <int DIR>
+__device__ __forceinline__ void dft5(float2 *x) {
+  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+  const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+  float2 a1 = add(x[1], x[4]), b1 = sub(x[1], x[4]);
+  float2 a2 = add(x[2], x[3]), b2 = sub(x[2], x[3]);
+  float2 x0 = x[0];
+  float2 m1 = add(x0, add(scl(a1, c1), scl(a2, c2)));
+  float2 m2 = add(x0, add(scl(a1, c2), scl(a2, c1)));
+  float2 n1 = mul_i<DIR>(add(scl(b1, s1), scl(b2, s2)));
+  float2 n2 = mul_i<DIR>(sub(scl(b1, s2), scl(b2, s1)));
+  x[0] = add(x0, add(a1, a2));
+  x[1] = add(m1, n1);
+  x[4] = sub(m1, n1);
+  x[2] = add(m2, n2);
+  x[3] = sub(m2, n2);
+}
+template <>
+__device__ __forceinline__ void dft<5, -1>(float2 *x) { dft5<-1>(x); }
+template <>
+__device__ __forceinline__ void dft<5, 1>(float2 *x) { dft5<1>(x); }
+
+// One Stockham stage (radix R, NS = product of earlier radices) over all
+// FFT_LINES lines in LDS, in place: every thread reads its butterflies into
+// registers, the workgroup synchronises, then the results are written back.
+template <int S, int R, int NS, int DIR>
+__device__ __forceinline__ void fft_stage(float2 *buf, const float2 *tws) {
+  constexpr int NBL = S / R;
+  constexpr int NB = FFT_LINES * NBL;
+  constexpr int PER = (NB + FFT_THREADS - 1) / FFT_THREADS;
+  constexpr int TSTEP = S / (NS * R);
+  float2 v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int b = tid + q * FFT_THREADS;
+    if ((NB % FFT_THREADS) == 0 || b < NB) {
+      const int line = b / NBL, j = b - line * NBL;
+      const float2 *ln = buf + line * (S + FFT_PAD);
+#pragma unroll
+      for (int k = 0; k < R; k++) v[q][k] = ln[j + k * NBL];
+      if (NS > 1) {
+        const int jm = j % NS;
+#pragma unroll
+        for (int k = 1; k < R; k++) {
+          float2 t = tws[jm * k * TSTEP];
+          if (DIR > 0) t.y = -t.y;
+          v[q][k] = cmul(v[q][k], t);
+        }
+      }
+      dft<R, DIR>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int b = tid + q * FFT_THREADS;
+    if ((NB % FFT_THREADS) == 0 || b < NB) {
+      const int line = b / NBL, j = b - line * NBL;
+      float2 *ln = buf + line * (S + FFT_PAD);
+      const int idxD = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int k = 0; k < R; k++) ln[idxD + k * NS] = v[q][k];
+    }
+  }
+  __syncthreads();
+}
+
+template <int S, int DIR, int STAGE, int NS>
+__device__ __forceinline__ void fft_stages(float2 *buf, const float2 *tws) {
+  constexpr int R = radix_at(S, STAGE);
+  if constexpr (R != 0) {
+    fft_stage<S, R, NS, DIR>(buf, tws);
+    fft_stages<S, DIR, STAGE + 1, NS * R>(buf, tws);
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void load_twiddles(float2 *tws, const float2 *tw) {
+  for (int q = threadIdx.x; q < S; q += FFT_THREADS) tws[q] = tw[q];
+}
+
+// Zero the points of every line outside [lo, hi).
+template <int S>
+__device__ __forceinline__ void zero_outside(float2 *buf, int lo, int hi) {
+  const int nz = S - (hi - lo);
+  if (nz <= 0) return;
+  for (int idx = threadIdx.x; idx < FFT_LINES * nz; idx += FFT_THREADS) {
+    const int l = idx / nz, q = idx - l * nz;
+    const int p = q < lo ? q : q + (hi - lo);
+    buf[l * (S + FFT_PAD) + p] = make_float2(0.f, 0.f);
+  }
+}
+
+// Generic grid -> grid pass over FFT_LINES lines (see LinesArgs).
+template <int S, int DIR>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(float2 *__restrict__ grid, LinesArgs a,
+                                                           const float2 *__restrict__ tw) {
+  __shared__ float2 buf[FFT_LINES * (S + FFT_PAD)];
+  __shared__ float2 tws[S];
+  load_twiddles<S>(tws, tw);
+  const int64_t base = a.off0 + (int64_t)blockIdx.y * a.ystride + (int64_t)blockIdx.z * a.zstride;
+  const int g0 = blockIdx.x * FFT_LINES;
+  const int W = a.run < FFT_LINES ? a.run : FFT_LINES;
+  zero_outside<S>(buf, a.ld0, a.ld1);
+  const int np = a.ld1 - a.ld0;
+  for (int idx = threadIdx.x; idx < FFT_LINES * np; idx += FFT_THREADS) {
+    const int c = idx % W, t = idx / W;
+    const int p = t % np, l = (t / np) * W + c;
+    const int gl = g0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (gl < a.nlines) {
+      const int r = gl / a.run, cc = gl - r * a.run;
+      v = grid[base + (int64_t)r * a.rstride + cc + (int64_t)(a.ld0 + p) * a.pstride];
+    }
+    buf[l * (S + FFT_PAD) + a.ld0 + p] = v;
+  }
+  __syncthreads();
+  fft_stages<S, DIR, 0, 1>(buf, tws);
+  const int ns = a.st1 - a.st0;
+  for (int idx = threadIdx.x; idx < FFT_LINES * ns; idx += FFT_THREADS) {
+    const int c = idx % W, t = idx / W;
+    const int p = t % ns, l = (t / ns) * W + c;
+    const int gl = g0 + l;
+    if (gl < a.nlines) {
+      const int r = gl / a.run, cc = gl - r * a.run;
+      grid[base + (int64_t)r * a.rstride + cc + (int64_t)(a.st0 + p) * a.pstride] =
+          scl(buf[l * (S + FFT_PAD) + a.st0 + p], a.scale);
+    }
+  }
+}
+
+// Fused set_image + forward z pass.  Workgroup: 16 lines (x, coil) of one
+// image row y, coil block blockIdx.z.  Loads the crop z range straight from
+// the image and the coil maps, writes the whole z line of the grid.
+template <int S>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z(
+    float2 *__restrict__ grid, const float2 *__restrict__ X, const float2 *__restrict__ smaps,
+    const float *__restrict__ wx, const float *__restrict__ wy, const float *__restrict__ wz,
+    PrePostArgs a, const float2 *__restrict__ tw) {
+  __shared__ float2 buf[FFT_LINES * (S + FFT_PAD)];
+  __shared__ float2 tws[S];
+  load_twiddles<S>(tws, tw);
+  const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
+  const int nl = a.N[0] * CC, g0 = blockIdx.x * FFT_LINES;
+  const int ogz = a.og[2];
+  zero_outside<S>(buf, ogz, ogz + a.N[2]);
+  const float wyv = wy[a.og[1] + y];
+  const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
+  for (int idx = threadIdx.x; idx < FFT_LINES * a.N[2]; idx += FFT_THREADS) {
+    const int l = idx % FFT_LINES, zi = idx / FFT_LINES;
+    const int gl = g0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (gl < nl) {
+      const int xi = gl / CC;
+      const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];
+      const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
+      const float2 xv = X[r + xi];
+      v = cmul(make_float2(wt * xv.x, wt * xv.y), sm[r * CC + gl]);
+    }
+    buf[l * (S + FFT_PAD) + ogz + zi] = v;
+  }
+  __syncthreads();
+  fft_stages<S, -1, 0, 1>(buf, tws);
+  float2 *gch = grid + (int64_t)ch * a.chunk_grid;
+  for (int idx = threadIdx.x; idx < FFT_LINES * S; idx += FFT_THREADS) {
+    const int l = idx % FFT_LINES, p = idx / FFT_LINES;
+    const int gl = g0 + l;
+    if (gl < nl)
+      gch[(((int64_t)p * a.S[1] + a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl] =
+          scl(buf[l * (S + FFT_PAD) + p], a.scale);
+  }
+}
+
+// Inverse z pass fused with accumulate: X(x,y,z) (+)= sum_c w * img_c * conj(S_c)
+// over the CC coils of block a.ch, for the crop z range.
+template <int S>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
+    const float2 *__restrict__ grid, float2 *__restrict__ X, const float2 *__restrict__ smaps,
+    const float *__restrict__ wx, const float *__restrict__ wy, const float *__restrict__ wz,
+    PrePostArgs a, const float2 *__restrict__ tw) {
+  __shared__ float2 buf[FFT_LINES * (S + FFT_PAD)];
+  __shared__ float2 tws[S];
+  load_twiddles<S>(tws, tw);
+  const int CC = a.CC, y = blockIdx.y, ch = a.ch;
+  const int nl = a.N[0] * CC, g0 = blockIdx.x * FFT_LINES;
+  const float2 *gch = grid + (int64_t)ch * a.chunk_grid;
+  for (int idx = threadIdx.x; idx < FFT_LINES * S; idx += FFT_THREADS) {
+    const int l = idx % FFT_LINES, p = idx / FFT_LINES;
+    const int gl = g0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (gl < nl) v = gch[(((int64_t)p * a.S[1] + a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl];
+    buf[l * (S + FFT_PAD) + p] = v;
+  }
+  __syncthreads();
+  fft_stages<S, 1, 0, 1>(buf, tws);
+  const int nxl = FFT_LINES / CC;
+  const int ogz = a.og[2];
+  const float wyv = wy[a.og[1] + y];
+  const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
+  for (int pr = threadIdx.x; pr < nxl * a.N[2]; pr += FFT_THREADS) {
+    const int xl = pr % nxl, zi = pr / nxl;
+    const int xi = blockIdx.x * nxl + xl;
+    if (xi >= a.N[0]) continue;
+    const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];
+    const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0] + xi;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int c = 0; c < CC; c++) {
+      const float2 im = scl(buf[(xl * CC + c) * (S + FFT_PAD) + ogz + zi], a.scale);
+      const float2 t = cmulc(make_float2(wt * im.x, wt * im.y), sm[r * CC + c]);
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    if (a.overwrite) {
+      X[r] = acc;
+    } else {
+      float2 o = X[r];
+      X[r] = make_float2(o.x + acc.x, o.y + acc.y);
+    }
+  }
+}
+
+// Direct O(S^2) DFT for sizes without a Stockham plan (plain form only).
+__global__ __launch_bounds__(FFT_THREADS) void k_dft_lines(float2 *__restrict__ grid, LinesArgs a,
+                                                           const float2 *__restrict__ tw, int S,
+                                                           int dir) {
+  extern __shared__ float2 dsm[];
+  float2 *tws = dsm, *buf = dsm + S;
+  for (int q = threadIdx.x; q < S; q += FFT_THREADS) tws[q] = tw[q];
+  const int64_t base = a.off0 + (int64_t)blockIdx.y * a.ystride + (int64_t)blockIdx.z * a.zstride;
+  const int g0 = blockIdx.x * FFT_LINES;
+  for (int idx = threadIdx.x; idx < FFT_LINES * S; idx += FFT_THREADS) {
+    const int l = idx / S, p = idx % S, gl = g0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (gl < a.nlines && p >= a.ld0 && p < a.ld1) {
+      const int r = gl / a.run, cc = gl - r * a.run;
+      v = grid[base + (int64_t)r * a.rstride + cc + (int64_t)p * a.pstride];
+    }
+    buf[l * S + p] = v;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < FFT_LINES * (a.st1 - a.st0); idx += FFT_THREADS) {
+    const int ns = a.st1 - a.st0;
+    const int l = idx / ns, p = a.st0 + idx % ns, gl = g0 + l;
+    if (gl >= a.nlines) continue;
+    float sr = 0.f, si = 0.f;
+    for (int q = 0; q < S; q++) {
+      float2 t = tws[(int)(((int64_t)p * q) % S)];
+      if (dir > 0) t.y = -t.y;
+      const float2 x = buf[l * S + q];
+      sr += x.x * t.x - x.y * t.y;
+      si += x.x * t.y + x.y * t.x;
+    }
+    const int r = gl / a.run, cc = gl - r * a.run;
+    grid[base + (int64_t)r * a.rstride + cc + (int64_t)p * a.pstride] =
+        make_float2(sr * a.scale, si * a.scale);
+  }
+}
+
+#define MRN_FFT_SIZES(X) \
+  X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(144) X(160) \
+  X(192) X(240) X(256) X(288) X(320) X(384) X(400) X(432) X(480) X(512)
+
+bool fft_size_supported(int S) {
+  switch (S) {
+#define MRN_CASE_TRUE(n) case n:
+    MRN_FFT_SIZES(MRN_CASE_TRUE)
+    return true;
+#undef MRN_CASE_TRUE
+    default:
+      return false;
+  }
+}
+
+int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 g3,
+              const float2 *tw, hipStream_t st) {
+  if (g3.x == 0 || g3.y == 0 || g3.z == 0) return MRN_OK;
+  switch (S) {
+#define MRN_CASE_LINES(n)                                                                  \
+  case n:                                                                                  \
+    if (dir < 0)                                                                           \
+      hipLaunchKernelGGL((k_fft_lines<n, -1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw); \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_fft_lines<n, 1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw);  \
+    break;
+    MRN_FFT_SIZES(MRN_CASE_LINES)
+#undef MRN_CASE_LINES
+    default: {
+      size_t sh = sizeof(float2) * (size_t)S * (FFT_LINES + 1);
+      if (sh > 160 * 1024) return fail(op, MRN_EUNSUPPORTED, "FFT size too large for the DFT fallback");
+      hipLaunchKernelGGL(k_dft_lines, g3, dim3(FFT_THREADS), sh, st, grid, a, tw, S, dir);
+    }
+  }
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
+              const PrePostArgs &a, hipStream_t st) {
+  dim3 g3((a.N[0] * a.CC + FFT_LINES - 1) / FFT_LINES, a.N[1], op->nch);
+  switch (a.S[2]) {
+#define MRN_CASE_PRE(n)                                                                    \
+  case n:                                                                                  \
+    hipLaunchKernelGGL((k_fft_pre_z<n>), g3, dim3(FFT_THREADS), 0, st, grid, X, smaps,     \
+                       op->win_d[0], op->win_d[1], op->win_d[2], a, op->tw_d[2]);          \
+    break;
+    MRN_FFT_SIZES(MRN_CASE_PRE)
+#undef MRN_CASE_PRE
+    default:
+      return fail(op, MRN_EUNSUPPORTED, "no fused FFT for this grid size");
+  }
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+int fft_post_z(mrn_op *op, const float2 *grid, float2 *X, const float2 *smaps,
+               const PrePostArgs &a, hipStream_t st) {
+  dim3 g3((a.N[0] * a.CC + FFT_LINES - 1) / FFT_LINES, a.N[1], 1);
+  switch (a.S[2]) {
+#define MRN_CASE_POST(n)                                                                   \
+  case n:                                                                                  \
+    hipLaunchKernelGGL((k_fft_post_z<n>), g3, dim3(FFT_THREADS), 0, st, grid, X, smaps,    \
+                       op->win_d[0], op->win_d[1], op->win_d[2], a, op->tw_d[2]);          \
+    break;
+    MRN_FFT_SIZES(MRN_CASE_POST)
+#undef MRN_CASE_POST
+    default:
+      return fail(op, MRN_EUNSUPPORTED, "no fused FFT for this grid size");
+  }
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+}  // namespace mrn

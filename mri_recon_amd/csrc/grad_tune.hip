@@ -1,0 +1,206 @@
+// Trajectory gradient of the operator and the launch-shape autotuner.
+//
+// Neither has a counterpart in the reference (its trajectory is an input and
+// its loops have no tunables).  The gradient differentiates the gather of
+// chop_grid_backward (gridFFT.cpp:1276-1313) with respect to the sample
+// coordinate: d/dk_a of prod_b w_b(dk_b - l_b) = grid_a * w_a'(dk_a - l_a) *
+// prod_{b != a} w_b.  In table mode w' is the slope of the table segment (the
+// exact derivative of the piecewise-linear kernel); in exact-KB mode it is the
+// analytic derivative of the Kaiser-Bessel function, using I1 from Abramowitz
+// & Stegun 9.8.3 / 9.8.4.
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "mrn_internal.hpp"
+
+namespace mrn {
+int fwd_grid_pub(mrn_op *op, const float2 *X, hipStream_t st);
+int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, hipStream_t);
+int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
+int build_bins(mrn_op *, FrameData &, int64_t, const float *, const float *, const float *,
+               const float *, hipStream_t);
+int op_permute_in(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
+int op_fill(mrn_op *, float2 *, int64_t, float2, hipStream_t);
+
+// I1(z)/z for z >= 0 (A&S 9.8.3 below 3.75, 9.8.4 above)
+__device__ __forceinline__ float i1_over_z(float z) {
+  if (z < 3.75f) {
+    float t = z / 3.75f;
+    t *= t;
+    return 0.5f + t * (0.87890594f + t * (0.51498869f + t * (0.15084934f + t * (0.02658733f +
+                  t * (0.00301532f + t * 0.00032411f)))));
+  }
+  float u = 3.75f / z;
+  float i1 = (expf(z) * rsqrtf(z)) *
+             (0.39894228f + u * (-0.03988024f + u * (-0.00362018f + u * (0.00163801f + u * (-0.01031555f +
+              u * (0.02282967f + u * (-0.02895312f + u * (0.01787654f - u * 0.00420059f))))))));
+  return i1 / z;
+}
+
+// d w(dk - l) / d dk, with the same fftshift sign as tap_weight
+__device__ __forceinline__ float tap_dweight(const Geom &g, int a, const float *tab, float dk, int l) {
+  if (a == 2 && !g.grid_in[2]) return 0.0f;
+  const float u = dk - (float)l;
+  const float su = u < 0.f ? -1.f : 1.f;
+  float d;
+  if (g.kmode == MRN_KMODE_EXACT_KB && g.grid_in[a]) {
+    const float aw = g.dwin[a];
+    const float x = fabsf(u);
+    if (x >= aw) return 0.0f;
+    const float s = sqrtf(fmaxf(1.0f - (x / aw) * (x / aw), 0.0f));
+    const float z = g.beta[a] * s;
+    // dK/dx = knorm/I0(beta) * I1(beta s) * beta * ds/dx, ds/dx = -x/(aw^2 s)
+    d = -g.knorm[a] / g.i0beta[a] * g.beta[a] * g.beta[a] * i1_over_z(z) * x / (aw * aw);
+  } else {
+    const float del = fabsf(600.0f * u);
+    const int i0 = (int)del;
+    const float f0 = (i0 < g.tlen[a]) ? tab[i0] : 0.0f;
+    const float f1 = (i0 + 1 < g.tlen[a]) ? tab[i0 + 1] : 0.0f;
+    d = 600.0f * (f1 - f0);
+  }
+  d *= su;
+  if (g.fft_in[a] && !(l & 1)) d = -d;
+  return d;
+}
+
+// grad_k[m][a] += Re sum_c conj(g_cm) dY_cm/dk_a over the CC coils of a block
+__global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ grid,
+                                                   const float2 *__restrict__ gin,
+                                                   const float4 *__restrict__ coords,
+                                                   const int32_t *__restrict__ perm, int64_t Mv,
+                                                   Geom g, const float *__restrict__ tables,
+                                                   int64_t chunk_grid, int64_t chunk_k,
+                                                   float *grad_k) {
+  extern __shared__ float tab_s[];
+  for (int q = threadIdx.x; q < g.ntab; q += blockDim.x) tab_s[q] = tables[q];
+  __syncthreads();
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (int)(gid & (g.CC - 1));
+  const int64_t i = gid >> g.lgCC;
+  if (i >= Mv) return;
+  const int ch = blockIdx.y;
+  const float4 k = coords[i];
+  const float dk[3] = {k.x, k.y, k.z};
+  int s[3], e[3];
+  for (int a = 0; a < 3; a++) axis_range(g, a, dk[a], s[a], e[a]);
+  const float2 *G = grid + ch * chunk_grid + c;
+  float dr[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
+  for (int lz = s[2]; lz <= e[2]; lz++) {
+    const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
+    const float dz = tap_dweight(g, 2, tab_s + g.toff[2], dk[2], lz);
+    for (int ly = s[1]; ly <= e[1]; ly++) {
+      const float wy = tap_weight(g, 1, tab_s + g.toff[1], dk[1], ly);
+      const float dy = tap_dweight(g, 1, tab_s + g.toff[1], dk[1], ly);
+      const float2 *row = G + ((int64_t)lz * g.S[1] + ly) * g.S[0] * g.CC;
+      for (int lx = s[0]; lx <= e[0]; lx++) {
+        const float wx = tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
+        const float dx = tap_dweight(g, 0, tab_s + g.toff[0], dk[0], lx);
+        const float2 v = row[(int64_t)lx * g.CC];
+        const float q0 = dx * wy * wz, q1 = wx * dy * wz, q2 = wx * wy * dz;
+        dr[0] += q0 * v.x;
+        di[0] += q0 * v.y;
+        dr[1] += q1 * v.x;
+        di[1] += q1 * v.y;
+        dr[2] += q2 * v.x;
+        di[2] += q2 * v.y;
+      }
+    }
+  }
+  const float2 gv = gin[ch * chunk_k + i * g.CC + c];
+  const int m = perm[i];
+  for (int a = 0; a < 3; a++) {
+    // Re(conj(g) * dY) * d(dk)/dk
+    float val = (gv.x * dr[a] + gv.y * di[a]) * g.grid[a];
+    // sum the CC lanes of this sample before one atomic
+    for (int off = 1; off < g.CC; off <<= 1) val += __shfl_xor(val, off);
+    if (c == 0) atomicAdd(&grad_k[3 * m + a], val);
+  }
+}
+
+int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float *grad_k,
+                   hipStream_t st) {
+  FrameData &fr = op->frames[f];
+  MRN_HIP(op, hipMemsetAsync(grad_k, 0, sizeof(float) * 3 * fr.M, st));
+  if (fr.Mv == 0) return MRN_OK;
+  MRN_TRY(fwd_grid_pub(op, X, st));
+  MRN_TRY(op_permute_in(op, op->kwork, gup, fr, st));
+  const Geom &g = op->g;
+  dim3 g3((unsigned)((fr.Mv * g.CC + 255) / 256), op->nch);
+  hipLaunchKernelGGL(k_traj_grad, g3, dim3(256), sizeof(float) * g.ntab, st, op->grid, op->kwork,
+                     fr.coords, fr.perm, fr.Mv, g, op->tables_d,
+                     (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, fr.Mv * g.CC, grad_k);
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+// Candidate scatter shapes (tile edge, coils per workgroup, visits per work
+// item) are timed on the frame's own trajectory; the fastest is kept and the
+// bins are rebuilt for it.  Every candidate computes the same result.
+int autotune_impl(mrn_op *op, int f, hipStream_t st) {
+  FrameData &fr = op->frames[f];
+  const Tiles keep = op->t;
+  int ntap = 1;
+  for (int a = 0; a < 3; a++)
+    if (op->g.grid_in[a]) ntap = std::max(ntap, (int)floorf(2 * op->g.dwin[a]) + 1);
+  struct Cand {
+    int T, CCs, maxv;
+  };
+  std::vector<Cand> cands;
+  for (int T : {8, 12, 16})
+    for (int CCs : {1, 2, 4, 8})
+      for (int maxv : {1024, 2048, 4096}) {
+        if (T < ntap - 1 || CCs > op->g.CC) continue;
+        if ((int64_t)T * T * T * CCs * 8 + 4 * (op->g.ntab + 4) > 64 * 1024) continue;
+        cands.push_back({T, CCs, maxv});
+      }
+  // non-zero samples so that no candidate skips work
+  MRN_TRY(op_fill(op, op->kwork, fr.Mv * op->g.CC * op->nch, make_float2(1.f, 0.f), st));
+  hipEvent_t e0, e1;
+  MRN_HIP(op, hipEventCreate(&e0));
+  MRN_HIP(op, hipEventCreate(&e1));
+  float best = 1e30f;
+  Cand bestc{keep.T[0], keep.CCs, keep.maxv};
+  for (const Cand &cd : cands) {
+    for (int a = 0; a < 3; a++) {
+      op->t.T[a] = cd.T;
+      op->t.nt[a] = (op->g.S[a] + cd.T - 1) / cd.T;
+    }
+    op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
+    op->t.maxv = cd.maxv;
+    op->t.CCs = cd.CCs;
+    MRN_TRY(build_bins(op, fr, fr.M, fr.traw, fr.traw + fr.M, fr.traw + 2 * fr.M, fr.traw + 3 * fr.M, st));
+    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));  // warm-up
+    MRN_HIP(op, hipEventRecord(e0, st));
+    for (int r = 0; r < 3; r++) MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
+    MRN_HIP(op, hipEventRecord(e1, st));
+    MRN_HIP(op, hipEventSynchronize(e1));
+    float ms = 0.f;
+    MRN_HIP(op, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bestc = cd;
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    op->t.T[a] = bestc.T;
+    op->t.nt[a] = (op->g.S[a] + bestc.T - 1) / bestc.T;
+  }
+  op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
+  op->t.maxv = bestc.maxv;
+  op->t.CCs = bestc.CCs;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  // rebuild every frame's bins for the chosen shape (kdata follows the new order)
+  for (int q = 0; q < op->F; q++) {
+    FrameData &fq = op->frames[q];
+    if (!fq.has_traj) continue;
+    MRN_TRY(build_bins(op, fq, fq.M, fq.traw, fq.traw + fq.M, fq.traw + 2 * fq.M, fq.traw + 3 * fq.M, st));
+    if (fq.has_kdata) MRN_TRY(op_permute_in(op, fq.kdata, fq.kraw, fq, st));
+  }
+  MRN_HIP(op, hipStreamSynchronize(st));
+  return MRN_OK;
+}
+
+}  // namespace mrn

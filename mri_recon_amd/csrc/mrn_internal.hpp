@@ -1,0 +1,211 @@
+// Internal declarations shared by the HIP sources of libmrn.
+//
+// Data layout in HBM (all complex64 = float2):
+//   image X            [Nz][Ny][Nx]                     (x fastest, like the reference)
+//   coil maps (op)     [nch][Nz][Ny][Nx][CC]            coil-interleaved blocks of CC coils
+//   grids (op)         [nch][Sz][Sy][Sx][CC]            one oversampled grid per coil block
+//   k-space (op)       [nch][Mv][CC]                    valid samples in tile order
+// CC coils share one grid cell (CC*8 bytes), so a gridding tap reads or adds
+// one contiguous CC*8-byte run and the kernel weights are computed once per
+// sample for all CC coils of the block.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mrn.h"
+
+#define MRN_MAXT 8           // max taps per axis (dwin <= 3.5)
+#define FFT_LINES 16         // 1-D lines per FFT workgroup
+#define FFT_THREADS 256
+#define FFT_PAD 1            // LDS line padding (float2) against bank conflicts
+
+namespace mrn {
+
+// ------------------------------------------------------------ geometry
+struct Geom {
+  int N[3], S[3], og[3];
+  int CC, lgCC;
+  int grid_in[3], fft_in[3];
+  float grid[3];   // oversampling ratio per axis
+  float dwin[3];
+  int tlen[3];     // kernel table length (L + 10)
+  int toff[3];     // table offset in the packed table buffer
+  int ntab;        // packed table length (floats)
+  int kmode;       // MRN_KMODE_*
+  float beta[3];   // KB beta (exact mode)
+  float knorm[3];  // table normalisation (exact mode)
+  float i0beta[3]; // I0(beta) (exact mode)
+  float k_rad;
+};
+
+// Scatter tiles (owner-computes): tile t covers cells [t*T, t*T+T) per axis
+// and receives the samples whose first tap lies in tile t or t-1 per axis.
+struct Tiles {
+  int T[3];
+  int nt[3];
+  int ntiles;
+  int maxv;   // sample visits per scatter work item
+  int CCs;    // coils per scatter workgroup (LDS accumulator depth)
+};
+
+struct FrameData {
+  int64_t M = 0;               // samples (caller order)
+  int64_t Mv = 0;              // valid samples
+  float4 *coords = nullptr;    // [Mv] dkx, dky, dkz, kw in tile order
+  float *taper = nullptr;      // [Mv] exp(-|k|^2 / (2 k_rad^2))
+  int32_t *perm = nullptr;     // [Mv] tile order -> caller index
+  int32_t *inv = nullptr;      // [M] caller index -> tile order, -1 if skipped
+  int32_t *bin_start = nullptr;// [ntiles + 1]
+  int4 *subs = nullptr;        // [nsub] {tile, first visit, end visit, heavy}
+  int32_t *heavy = nullptr;    // [nheavy] tiles split over several work items
+  int nsub = 0, nheavy = 0;
+  float2 *kdata = nullptr;     // measured k-space, internal layout [nch][Mv][CC]
+  float2 *kraw = nullptr;      // measured k-space, caller layout [C][M] (plain form)
+  float *traw = nullptr;       // trajectory kx|ky|kz|kw, caller order [4][M]
+  bool has_traj = false, has_kdata = false;
+};
+
+}  // namespace mrn
+
+struct mrn_op {
+  mrn_params p;
+  mrn::Geom g;
+  mrn::Tiles t;
+  int C = 0, F = 1, nch = 1;
+  std::vector<float> table_h[3], win_h[3];
+  float *tables_d = nullptr;   // packed tables
+  float *win_d[3] = {nullptr, nullptr, nullptr};
+  float2 *tw_d[3] = {nullptr, nullptr, nullptr};  // forward twiddles exp(-2 pi i q / S)
+  float2 *smaps = nullptr;     // [nch][Nz][Ny][Nx][CC]
+  float2 *grid = nullptr;      // [nch][Sz][Sy][Sx][CC]
+  float2 *cgrid = nullptr;     // plain form: one coil grid [Sz][Sy][Sx]
+  float2 *kwork = nullptr;     // [nch][Mv_max][CC]
+  int64_t kwork_cap = 0;       // samples
+  float2 *kplain = nullptr;    // plain form: one coil of k-space [M_max]
+  int64_t kplain_cap = 0;
+  std::vector<mrn::FrameData> frames;
+  hipStream_t stream = nullptr;
+  // solver workspace
+  float2 *vR = nullptr, *vP = nullptr, *vL = nullptr, *vX0 = nullptr;
+  double *dpart = nullptr;     // reduction partials
+  double *dscal = nullptr;     // device scalars (see solvers)
+  double *dtrace = nullptr;    // per-iteration trace
+  int dtrace_cap = 0;
+  int *dcounter = nullptr;
+  int64_t nvox = 0;
+  std::string err;
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+
+namespace mrn {
+
+int fail(mrn_op *op, int code, const char *msg);
+#define MRN_HIP(op, call)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      return mrn::fail(op, MRN_EHIP, (std::string(#call) + ": " + hipGetErrorString(e_)).c_str()); \
+    }                                                                              \
+  } while (0)
+#define MRN_TRY(expr)          \
+  do {                         \
+    int s_ = (expr);           \
+    if (s_ != MRN_OK) return s_; \
+  } while (0)
+
+inline hipStream_t pick(mrn_op *, void *s) { return (hipStream_t)s; }  // NULL = null stream
+
+// ------------------------------------------------------ device helpers
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// First/last tap on one axis (chop_grid_backward, gridFFT.cpp:1211-1271).
+// Returns false when the sample lies outside the grid on this axis.
+__device__ __forceinline__ bool axis_range(const Geom &g, int a, float dk, int &s, int &e) {
+  if (g.grid_in[a]) {
+    s = max((int)ceilf(dk - g.dwin[a]), 0);
+    e = min((int)floorf(dk + g.dwin[a]), g.S[a] - 1);
+  } else {
+    s = (int)dk;
+    e = s;
+  }
+  return !(s >= g.S[a] || e < 0);
+}
+
+// Modified Bessel I0 (Abramowitz & Stegun 9.8.1 / 9.8.2), float result.
+__device__ __forceinline__ float dev_i0(float x) {
+  float ax = fabsf(x);
+  if (ax < 3.75f) {
+    float y = x / 3.75f;
+    y *= y;
+    return 1.0f + y * (3.5156229f + y * (3.0899424f + y * (1.2067492f + y * (0.2659732f +
+                  y * (0.360768e-1f + y * 0.45813e-2f)))));
+  }
+  float y = 3.75f / ax;
+  return (expf(ax) * rsqrtf(ax)) *
+         (0.39894228f + y * (0.1328592e-1f + y * (0.225319e-2f + y * (-0.157565e-2f +
+          y * (0.916281e-2f + y * (-0.2057706e-1f + y * (0.2635537e-1f + y * (-0.1647633e-1f +
+          y * 0.392377e-2f))))))));
+}
+
+// Kernel weight of tap l (with the fftshift sign), table or exact-KB mode.
+// tab points at this axis' table (in LDS).  Mirrors the tap loops of
+// gridFFT.cpp:1276-1306: the z weight is 1 on a Cartesian z axis.
+__device__ __forceinline__ float tap_weight(const Geom &g, int a, const float *tab, float dk,
+                                            int l) {
+  float w;
+  if (g.kmode == MRN_KMODE_EXACT_KB && g.grid_in[a]) {
+    float x = fabsf(dk - (float)l) / g.dwin[a];
+    w = (x <= 1.0f) ? g.knorm[a] * dev_i0(g.beta[a] * sqrtf(fmaxf(1.0f - x * x, 0.0f))) / g.i0beta[a]
+                    : 0.0f;
+  } else {
+    float del = fabsf(600.0f * (dk - (float)l));
+    int i0 = (int)del;
+    float fr = del - (float)i0;
+    float f0 = (i0 < g.tlen[a]) ? tab[i0] : 0.0f;
+    float f1 = (i0 + 1 < g.tlen[a]) ? tab[i0 + 1] : 0.0f;
+    w = f0 * (1.0f - fr) + f1 * fr;
+  }
+  if (a == 2 && !g.grid_in[2]) w = 1.0f;
+  if (g.fft_in[a] && !(l & 1)) w = -w;
+  return w;
+}
+
+// ------------------------------------------------------ FFT pass args
+struct LinesArgs {
+  int64_t off0, ystride, zstride;  // base of the workgroup's lines
+  int64_t pstride, rstride;        // point stride along the axis; row stride
+  int run;                         // contiguous lines per row
+  int nlines;                      // valid lines in the group dimension
+  int ld0, ld1, st0, st1;          // loaded / stored point ranges
+  float scale;
+};
+
+struct PrePostArgs {
+  int N[3], S[3], og[3];
+  int CC;
+  int64_t chunk_grid;   // grid elements per coil block
+  int64_t chunk_smap;   // smap elements per coil block
+  int ch;               // coil block (post: one launch per block)
+  int overwrite;        // post: X = sum instead of X += sum
+  float scale;
+};
+
+// host-side launchers (defined in the .hip files)
+int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 grid3,
+              const float2 *tw, hipStream_t st);
+int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
+              const PrePostArgs &a, hipStream_t st);
+int fft_post_z(mrn_op *op, const float2 *grid, float2 *X, const float2 *smaps,
+               const PrePostArgs &a, hipStream_t st);
+bool fft_size_supported(int S);
+
+}  // namespace mrn

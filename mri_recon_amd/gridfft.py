@@ -1,0 +1,239 @@
+"""GridFFT: the NUFFT/SENSE encoding operator on the HIP library.
+
+Mirrors the reference's gridFFT (src/gridFFT.h:28-194) for torch tensors on a
+HIP device: ``backward`` is A (image -> k-space, gridFFT::backward,
+gridFFT.cpp:824-844), ``backward_residual`` is A X - d (856-876), ``forward``
+is A^H (k-space -> image, gridFFT::forward, 791-812) and accumulates into X
+like the reference.  Unlike the reference, one call covers every coil of a
+frame (the caller's coil loop, recon_lib.cpp:1565-1567 and 1614-1622, moves
+into the library).
+
+Shapes: image (Nz, Ny, Nx) complex64; coil maps (C, Nz, Ny, Nx); k-space of a
+frame (C, M); trajectory kx, ky, kz, kw float32 (M,) in the reference's units
+(cycles/FOV, |k| <= N/2).
+"""
+import ctypes
+
+import torch
+
+from . import _ffi
+from ._ffi import MrnParams, check, lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _need(t, dtype, shape=None, name="tensor"):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor" % name)
+    if t.device.type != "cuda":
+        raise ValueError("%s must be on a HIP device" % name)
+    if t.dtype != dtype:
+        raise TypeError("%s must be %s, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError("%s must have shape %s, got %s" % (name, tuple(shape), tuple(t.shape)))
+    return t
+
+
+def _trip(v):
+    return (v, v, v) if not isinstance(v, (tuple, list)) else tuple(v)
+
+
+class GridFFT:
+    """One operator instance: geometry, tables, windows, coil maps, per-frame
+    trajectories (and measured data for the solvers)."""
+
+    def __init__(self, N, ncoils=1, nframes=1, overgrid=1.5, dwin=-1.0, kernel="kaiser",
+                 grid_in=(1, 1, 1), fft_in=(1, 1, 1), grid=-1.0, grid_scale=1.0, k_rad=9999.0,
+                 kernel_mode="table", form="batched", coil_chunk=0, tile=0, max_visits=0,
+                 args=None):
+        L = lib()
+        p = MrnParams()
+        L.mrn_params_default(ctypes.byref(p))
+        N = _trip(N)
+        for a in range(3):
+            p.N[a] = int(N[a])
+            p.grid[a] = float(_trip(grid)[a])
+            p.dwin[a] = float(_trip(dwin)[a])
+            p.grid_in[a] = int(grid_in[a])
+            p.fft_in[a] = int(fft_in[a])
+            p.grid_scale[a] = float(_trip(grid_scale)[a])
+            p.tile[a] = int(_trip(tile)[a])
+        p.overgrid = float(overgrid)
+        p.kernel = _ffi.KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        p.k_rad = float(k_rad)
+        p.ncoils = int(ncoils)
+        p.nframes = int(nframes)
+        p.kernel_mode = _ffi.KMODES[kernel_mode] if isinstance(kernel_mode, str) else int(kernel_mode)
+        p.form = _ffi.FORMS[form] if isinstance(form, str) else int(form)
+        p.coil_chunk = int(coil_chunk)
+        p.max_visits = int(max_visits)
+        if args:  # gridFFT::read_commandline flags
+            arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
+            check(L.mrn_parse_args(ctypes.byref(p), len(args), arr))
+        h = ctypes.c_void_p()
+        check(L.mrn_op_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.params = p
+        self.N = (int(p.N[0]), int(p.N[1]), int(p.N[2]))
+        self.C = int(ncoils)
+        self.F = int(nframes)
+        self.form = form
+        self.M = [0] * self.F
+        g = self.geometry()
+        self.S, self.og, self.beta = g["S"], g["og"], g["beta"]
+
+    # ---------------------------------------------------------------- life
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mrn_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, s):
+        check(s, self._h)
+
+    @property
+    def image_shape(self):
+        return (self.N[2], self.N[1], self.N[0])
+
+    # ------------------------------------------------------------ geometry
+    def geometry(self):
+        S = (ctypes.c_int32 * 3)()
+        og = (ctypes.c_int32 * 3)()
+        beta = (ctypes.c_float * 3)()
+        self._c(lib().mrn_get_geometry(self._h, S, og, beta))
+        return dict(S=tuple(S), og=tuple(og), beta=tuple(beta))
+
+    def _host_vec(self, fn, axis):
+        n = ctypes.c_int()
+        self._c(fn(self._h, axis, None, 0, ctypes.byref(n)))
+        out = (ctypes.c_float * n.value)()
+        self._c(fn(self._h, axis, out, n.value, ctypes.byref(n)))
+        return torch.tensor(list(out), dtype=torch.float32)
+
+    def table(self, axis=0):
+        return self._host_vec(lib().mrn_get_table, axis)
+
+    def window(self, axis=0):
+        return self._host_vec(lib().mrn_get_window, axis)
+
+    # ---------------------------------------------------------------- setup
+    def set_traj(self, kx, ky, kz, kw, frame=0):
+        M = kx.numel()
+        for t, nm in ((kx, "kx"), (ky, "ky"), (kz, "kz"), (kw, "kw")):
+            _need(t, torch.float32, (M,), nm)
+        self._c(lib().mrn_set_traj(self._h, frame, M, _ptr(kx), _ptr(ky), _ptr(kz), _ptr(kw), _stream()))
+        self.M[frame] = M
+
+    def set_smaps(self, smaps):
+        _need(smaps, torch.complex64, (self.C,) + self.image_shape, "smaps")
+        self._c(lib().mrn_set_smaps(self._h, _ptr(smaps), _stream()))
+
+    def set_kdata(self, data, frame=0):
+        _need(data, torch.complex64, (self.C, self.M[frame]), "data")
+        self._c(lib().mrn_set_kdata(self._h, frame, _ptr(data), _stream()))
+
+    def autotune(self, frame=0):
+        self._c(lib().mrn_autotune(self._h, frame, _stream()))
+
+    # ------------------------------------------------------------- operator
+    def backward(self, X, data=None, frame=0, sub=None):
+        """A: data[c] = A_c X (minus sub when given); returns data (C, M)."""
+        _need(X, torch.complex64, self.image_shape, "X")
+        if data is None:
+            data = torch.empty((self.C, self.M[frame]), dtype=torch.complex64, device=X.device)
+        _need(data, torch.complex64, (self.C, self.M[frame]), "data")
+        if sub is not None:
+            _need(sub, torch.complex64, (self.C, self.M[frame]), "sub")
+        self._c(lib().mrn_image_to_kspace(self._h, frame, _ptr(X), _ptr(data), _ptr(sub), _stream()))
+        return data
+
+    def backward_residual(self, X, sub, data=None, frame=0):
+        return self.backward(X, data=data, frame=frame, sub=sub)
+
+    def forward(self, data, X=None, frame=0, accumulate=True):
+        """A^H: X += sum_c A_c^H (kw taper data_c); a new X starts at zero."""
+        _need(data, torch.complex64, (self.C, self.M[frame]), "data")
+        if X is None:
+            X = torch.zeros(self.image_shape, dtype=torch.complex64, device=data.device)
+        _need(X, torch.complex64, self.image_shape, "X")
+        self._c(lib().mrn_kspace_to_image(self._h, frame, _ptr(data), _ptr(X), int(bool(accumulate)), _stream()))
+        return X
+
+    image_to_kspace = backward
+    kspace_to_image = forward
+
+    def normal(self, X, out=None, frame=0):
+        """E'E X = sum_c A_c^H W A_c X."""
+        _need(X, torch.complex64, self.image_shape, "X")
+        if out is None:
+            out = torch.empty_like(X)
+        self._c(lib().mrn_normal(self._h, frame, _ptr(X), _ptr(out), _stream()))
+        return out
+
+    def coil_backward(self, X, smap, traj, sub=None, data=None):
+        """Single-coil A with the trajectory given per call (plain form)."""
+        kx, ky, kz, kw = traj
+        M = kx.numel()
+        if data is None:
+            data = torch.empty(M, dtype=torch.complex64, device=X.device)
+        self._c(lib().mrn_coil_backward(self._h, _ptr(X), _ptr(smap), _ptr(data), M, _ptr(kx), _ptr(ky),
+                                        _ptr(kz), _ptr(kw), _ptr(sub), _stream()))
+        return data
+
+    def coil_forward(self, X, smap, data, traj):
+        """Single-coil X += A^H (kw taper data) (plain form)."""
+        kx, ky, kz, kw = traj
+        self._c(lib().mrn_coil_forward(self._h, _ptr(X), _ptr(smap), _ptr(data), kx.numel(), _ptr(kx),
+                                       _ptr(ky), _ptr(kz), _ptr(kw), _stream()))
+        return X
+
+    # -------------------------------------------------------------- solvers
+    def cg(self, iters, X=None, graph=True, trace=True):
+        """SENSE-CG over all frames (recon_lib.cpp:1523-1759).  Returns
+        (X (F, Nz, Ny, Nx), trace (iters, 4) [sum|R|^2, Re a, Im a, sum|Rnew|^2])."""
+        dev = torch.device("cuda")
+        if X is None:
+            X = torch.empty((self.F,) + self.image_shape, dtype=torch.complex64, device=dev)
+        tr = (ctypes.c_double * (4 * max(iters, 1)))() if trace else None
+        self._c(lib().mrn_cg(self._h, _ptr(X), int(iters), int(bool(graph)), tr, _stream()))
+        tt = torch.tensor(list(tr), dtype=torch.float64).reshape(-1, 4)[:iters] if trace else None
+        return X, tt
+
+    def power_method(self, iters=30):
+        step = ctypes.c_double()
+        self._c(lib().mrn_power_method(self._h, int(iters), ctypes.byref(step), _stream()))
+        return step.value
+
+    def fista(self, iters, step, X=None, graph=True):
+        dev = torch.device("cuda")
+        if X is None:
+            X = torch.empty((self.F,) + self.image_shape, dtype=torch.complex64, device=dev)
+        res = (ctypes.c_double * max(iters, 1))()
+        self._c(lib().mrn_fista(self._h, _ptr(X), int(iters), float(step), int(bool(graph)), res, _stream()))
+        return X, torch.tensor(list(res), dtype=torch.float64)[:iters]
+
+    # ------------------------------------------------------------- gradient
+    def traj_grad(self, X, g, frame=0):
+        """grad_k[m, a] = Re sum_c conj(g[c, m]) dA_c X[m]/dk_a  -> (M, 3) float32."""
+        _need(X, torch.complex64, self.image_shape, "X")
+        _need(g, torch.complex64, (self.C, self.M[frame]), "g")
+        out = torch.empty((self.M[frame], 3), dtype=torch.float32, device=X.device)
+        self._c(lib().mrn_traj_grad(self._h, frame, _ptr(X), _ptr(g), _ptr(out), _stream()))
+        return out
+
+    def sync(self):
+        self._c(lib().mrn_sync(self._h, _stream()))

@@ -1,0 +1,263 @@
+"""Parity of the HIP operator with the oracle on seeded small problems.
+
+Bar (FP32 arithmetic, atomics reorder sums): relative L2 error <= 1e-5 against
+the FP32 oracle and <= 1e-4 against the FP64 oracle (the same discretisation
+evaluated in double).  Every test calls the library through its C ABI.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from mri_recon_amd import GridFFT, MrnError
+from tests.helpers import crandn, radial_problem, rel, to_dev, traj_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL32, TOL64 = 1e-5, 1e-4
+
+CASES = [
+    # N, C, overgrid, dwin, nspokes, npts
+    (16, 2, 1.5, 2.5, 200, 16),
+    (32, 4, 1.25, 2.5, 400, 32),
+    (24, 3, 1.5, 2.0, 300, 24),
+    (32, 8, 1.5, 2.0, 500, 32),
+]
+
+
+def make(case, form="batched", seed=0, **kw):
+    N, C, os_, dwin, ns, npts = case
+    traj, X, smaps, d = radial_problem(N, C, ns, npts, seed=seed)
+    op = GridFFT(N, ncoils=C, overgrid=os_, dwin=dwin, form=form, **kw)
+    op.set_traj(*traj_dev(traj))
+    op.set_smaps(to_dev(smaps))
+    o32 = orc.OracleOp(N, overgrid=os_, dwin=dwin, precision="f32", **{k: v for k, v in kw.items() if k == "k_rad"})
+    o64 = orc.OracleOp(N, overgrid=os_, dwin=dwin, precision="f64", **{k: v for k, v in kw.items() if k == "k_rad"})
+    return op, o32, o64, traj, X, smaps, d
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_tables_windows_geometry_match_oracle(case):
+    N, C, os_, dwin, _, _ = case
+    op = GridFFT(N, ncoils=C, overgrid=os_, dwin=dwin)
+    o32 = orc.OracleOp(N, overgrid=os_, dwin=dwin, precision="f32")
+    g = o32.geometry()
+    assert op.S == tuple(g["S"]) and op.og == tuple(g["og"])
+    assert abs(op.beta[0] - g["beta"][0]) < 1e-6 * g["beta"][0]
+    for a in range(3):
+        t = op.table(a).numpy()
+        assert np.max(np.abs(t - o32.table(a))) <= 2e-7 * np.max(np.abs(t))
+        w = op.window(a).numpy()
+        assert np.max(np.abs(w - o32.window(a))) <= 2e-7
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+@pytest.mark.parametrize("case", CASES)
+def test_image_to_kspace(case, form):
+    op, o32, o64, traj, X, smaps, d = make(case, form)
+    y = op.backward(to_dev(X)).cpu().numpy()
+    y32 = o32.image_to_kspace(X, smaps, traj)
+    y64 = o64.image_to_kspace(X.astype(np.complex128), smaps.astype(np.complex128), traj)
+    e32, e64 = rel(y, y32), rel(y, y64)
+    print("A %s %s: vs f32 %.2e vs f64 %.2e" % (form, case, e32, e64))
+    assert e32 <= TOL32 and e64 <= TOL64
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+@pytest.mark.parametrize("case", CASES)
+def test_kspace_to_image_accumulates(case, form):
+    op, o32, o64, traj, X, smaps, d = make(case, form)
+    X0 = crandn(X.shape, 99)
+    Xg = op.forward(to_dev(d), X=to_dev(X0.copy()), accumulate=True).cpu().numpy()
+    X32 = o32.kspace_to_image(d, smaps, traj, X=X0.copy())
+    X64 = o64.kspace_to_image(d.astype(np.complex128), smaps.astype(np.complex128), traj,
+                              X=X0.astype(np.complex128))
+    e32, e64 = rel(Xg, X32), rel(Xg, X64)
+    print("AH %s %s: vs f32 %.2e vs f64 %.2e" % (form, case, e32, e64))
+    assert e32 <= TOL32 and e64 <= TOL64
+    # overwrite mode equals accumulate onto zeros
+    Xz = op.forward(to_dev(d), X=to_dev(X0.copy()), accumulate=False).cpu().numpy()
+    assert rel(Xz, X32 - X0) <= TOL32 * 10
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_residual_and_skipped_samples(form):
+    case = CASES[1]
+    op, o32, _, traj, X, smaps, d = make(case, form)
+    kw = traj[3].copy()
+    kw[::7] = 0.0  # skipped samples give 0, not -d
+    traj2 = (traj[0], traj[1], traj[2], kw)
+    op.set_traj(*traj_dev(traj2))
+    y = op.backward_residual(to_dev(X), to_dev(d)).cpu().numpy()
+    y32 = o32.image_to_kspace(X, smaps, traj2, sub=d)
+    assert rel(y, y32) <= TOL32
+    assert np.all(y[:, ::7] == 0)
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+@pytest.mark.parametrize("case", CASES[:2])
+def test_normal_operator(case, form):
+    op, o32, o64, traj, X, smaps, d = make(case, form)
+    L = op.normal(to_dev(X)).cpu().numpy()
+    L32 = o32.normal(X, smaps, traj)
+    L64 = o64.normal(X.astype(np.complex128), smaps.astype(np.complex128), traj)
+    e32, e64 = rel(L, L32), rel(L, L64)
+    print("EhE %s %s: vs f32 %.2e vs f64 %.2e" % (form, case, e32, e64))
+    assert e32 <= TOL32 and e64 <= TOL64
+
+
+def test_adjoint_dot_product():
+    case = CASES[3]
+    N, C = case[0], case[1]
+    traj, X, smaps, d = radial_problem(N, C, case[4], case[5], seed=5, kw_ones=True)
+    op = GridFFT(N, ncoils=C, overgrid=case[2], dwin=case[3], k_rad=1e30)
+    op.set_traj(*traj_dev(traj))
+    op.set_smaps(to_dev(smaps))
+    Ax = op.backward(to_dev(X)).cpu().numpy().astype(np.complex128)
+    AHy = op.forward(to_dev(d), accumulate=False).cpu().numpy().astype(np.complex128)
+    lhs, rhs = np.vdot(Ax, d), np.vdot(X, AHy)
+    err = abs(lhs - rhs) / abs(lhs)
+    print("adjoint dot rel err %.2e" % err)
+    assert err < 1e-5
+
+
+def test_batched_equals_plain_and_coil_entry_points():
+    case = CASES[1]
+    opb, o32, _, traj, X, smaps, d = make(case, "batched")
+    opp, _, _, _, _, _, _ = make(case, "plain")
+    yb = opb.backward(to_dev(X)).cpu().numpy()
+    yp = opp.backward(to_dev(X)).cpu().numpy()
+    assert rel(yb, yp) <= TOL32
+    # single-coil reference-style calls (trajectory per call)
+    tr = traj_dev(traj)
+    sm = to_dev(smaps)
+    y0 = opp.coil_backward(to_dev(X), sm[0].contiguous(), tr).cpu().numpy()
+    assert rel(y0, yp[0]) <= 1e-6
+    Xa = torch.zeros(X.shape, dtype=torch.complex64, device="cuda")
+    for c in range(case[1]):
+        opp.coil_forward(Xa, sm[c].contiguous(), to_dev(d[c]), tr)
+    Xr = o32.kspace_to_image(d, smaps, traj)
+    assert rel(Xa.cpu().numpy(), Xr) <= TOL32
+
+
+def test_edge_samples_empty_and_all_skipped():
+    N = 16
+    op = GridFFT(N, ncoils=2, overgrid=1.5)
+    o32 = orc.OracleOp(N, overgrid=1.5, precision="f32")
+    sm = crandn((2, N, N, N), 3)
+    op.set_smaps(to_dev(sm))
+    X = crandn((N, N, N), 4)
+    # samples on the faces and corners of |k| <= N/2 and beyond
+    vals = np.array([-N / 2, N / 2, -N / 2 + 0.25, N / 2 - 0.25, 0.0, N / 2 + 3.0], np.float32)
+    g = np.stack(np.meshgrid(vals, vals, vals, indexing="ij")).reshape(3, -1)
+    traj = (g[0].copy(), g[1].copy(), g[2].copy(), np.ones(g.shape[1], np.float32))
+    op.set_traj(*traj_dev(traj))
+    y = op.backward(to_dev(X)).cpu().numpy()
+    assert rel(y, o32.image_to_kspace(X, sm, traj)) <= TOL32
+    d = crandn((2, g.shape[1]), 5)
+    Xg = op.forward(to_dev(d), accumulate=False).cpu().numpy()
+    assert rel(Xg, o32.kspace_to_image(d, sm, traj)) <= TOL32
+    # empty frame
+    e = torch.empty(0, dtype=torch.float32, device="cuda")
+    op.set_traj(e, e, e, e)
+    assert op.backward(to_dev(X)).shape == (2, 0)
+    assert torch.count_nonzero(op.forward(torch.empty((2, 0), dtype=torch.complex64, device="cuda"),
+                                          accumulate=False)) == 0
+    # every sample skipped (kw = 0)
+    z = (traj[0], traj[1], traj[2], np.zeros_like(traj[3]))
+    op.set_traj(*traj_dev(z))
+    assert torch.count_nonzero(op.backward(to_dev(X))) == 0
+    assert torch.count_nonzero(op.forward(to_dev(d), accumulate=False)) == 0
+
+
+def test_bad_arguments_raise():
+    op = GridFFT(16, ncoils=2)
+    with pytest.raises(MrnError):
+        op.backward(torch.zeros((16, 16, 16), dtype=torch.complex64, device="cuda"))  # no trajectory
+    with pytest.raises(ValueError):
+        op.set_smaps(torch.zeros((3, 16, 16, 16), dtype=torch.complex64, device="cuda"))
+    with pytest.raises(MrnError):
+        GridFFT(16, ncoils=2, dwin=5.0)
+
+
+@pytest.mark.parametrize("form,graph", [("batched", True), ("batched", False), ("plain", False)])
+def test_cg_matches_oracle(form, graph):
+    case = CASES[0]
+    N, C = case[0], case[1]
+    traj, X, smaps, _ = radial_problem(N, C, case[4], case[5], seed=11)
+    o32 = orc.OracleOp(N, overgrid=case[2], dwin=case[3], precision="f32")
+    o64 = orc.OracleOp(N, overgrid=case[2], dwin=case[3], precision="f64")
+    d = o64.image_to_kspace(X.astype(np.complex128), smaps.astype(np.complex128), traj).astype(np.complex64)
+    d = d + 0.01 * crandn(d.shape, 12) * np.sqrt(np.mean(np.abs(d) ** 2))
+    op = GridFFT(N, ncoils=C, overgrid=case[2], dwin=case[3], form=form)
+    op.set_traj(*traj_dev(traj))
+    op.set_smaps(to_dev(smaps))
+    op.set_kdata(to_dev(d))
+    iters = 8
+    Xg, tr = op.cg(iters, graph=graph)
+    X32, t32 = o32.cg(smaps, d, traj, iters)
+    X64, t64 = o64.cg(smaps.astype(np.complex128), d.astype(np.complex128), traj, iters)
+    e32, e64 = rel(Xg[0].cpu().numpy(), X32), rel(Xg[0].cpu().numpy(), X64)
+    print("CG %s graph=%s: vs f32 %.2e vs f64 %.2e" % (form, graph, e32, e64))
+    assert e32 <= 1e-5 and e64 <= 1e-4
+    assert np.max(np.abs(tr.numpy() - t32) / np.abs(t32)) < 1e-4
+
+
+def test_power_method_and_fista_match_oracle():
+    case = CASES[0]
+    N, C = case[0], case[1]
+    traj, X, smaps, _ = radial_problem(N, C, case[4], case[5], seed=21)
+    o32 = orc.OracleOp(N, overgrid=case[2], dwin=case[3], precision="f32")
+    d = o32.image_to_kspace(X, smaps, traj)
+    op = GridFFT(N, ncoils=C, overgrid=case[2], dwin=case[3])
+    op.set_traj(*traj_dev(traj))
+    op.set_smaps(to_dev(smaps))
+    op.set_kdata(to_dev(d))
+    step = op.power_method(15)
+    step32, _ = o32.power_method(smaps, traj, iters=15)
+    assert abs(step - step32) / step32 < 1e-5
+    for graph in (True, False):
+        Xg, res = op.fista(6, step32, graph=graph)
+        X32, r32 = o32.fista(smaps, d, traj, 6, step32)
+        e = rel(Xg[0].cpu().numpy(), X32)
+        print("FISTA graph=%s vs f32 %.2e" % (graph, e))
+        assert e <= 1e-5
+        assert np.max(np.abs(res.numpy() - r32) / r32) < 1e-4
+
+
+def test_frames_of_uneven_length():
+    """Frames are independent operators sharing coil maps; FISTA over frames
+    with one step equals per-frame FISTA."""
+    N, C = 16, 2
+    from mri_recon_amd import synth
+    frames = synth.gated_radial_frames(3, 60, 16, N, seed=3)
+    smaps = synth.coil_maps(C, N, seed=4).numpy()
+    op = GridFFT(N, ncoils=C, nframes=3, overgrid=1.5)
+    op.set_smaps(to_dev(smaps))
+    o32 = orc.OracleOp(N, overgrid=1.5, precision="f32")
+    Xs, ds = [], []
+    for f, tr in enumerate(frames):
+        tr = [t.numpy() for t in tr]
+        op.set_traj(*traj_dev(tr), frame=f)
+        X = crandn((N, N, N), 30 + f)
+        d = o32.image_to_kspace(X, smaps, tr)
+        op.set_kdata(to_dev(d), frame=f)
+        y = op.backward(to_dev(X), frame=f).cpu().numpy()
+        assert rel(y, d) <= TOL32
+        Xs.append(tr)
+        ds.append(d)
+    Xg, _ = op.fista(4, 1e-5, graph=True)
+    for f, tr in enumerate(Xs):
+        Xf, _ = o32.fista(smaps, ds[f], tr, 4, 1e-5)
+        assert rel(Xg[f].cpu().numpy(), Xf) <= TOL32
+
+
+def test_autotune_keeps_results():
+    case = CASES[3]
+    op, o32, _, traj, X, smaps, d = make(case)
+    before = op.forward(to_dev(d), accumulate=False).cpu().numpy()
+    op.autotune()
+    after = op.forward(to_dev(d), accumulate=False).cpu().numpy()
+    assert rel(after, before) <= 1e-6
