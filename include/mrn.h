@@ -15,8 +15,10 @@
  *  - every pointer passed to an apply call is a HIP device pointer; the op owns
  *    its grids, bins, tables and workspaces, and apply calls allocate nothing.
  *  - every call returns an mrn_status and never exits the process.
- *  - calls are asynchronous on the given stream (NULL = the op's own stream);
- *    one op must not be used from two streams at once.
+ *  - calls are asynchronous on the given stream (NULL = the null stream);
+ *    one op must not be used from two streams at once.  Solver calls with
+ *    use_graph = 1 replay a captured hipGraph on the op's own stream, fenced
+ *    with events against the caller's stream.
  */
 #ifndef MRN_H
 #define MRN_H
@@ -132,6 +134,12 @@ int mrn_set_kdata(mrn_op *op, int frame, const void *data, void *stream);
  * iteration in a hipGraph and replays it.  trace (host, nullable) receives
  * 4 doubles per iteration: sum|R|^2, Re alpha, Im alpha, sum|R_new|^2. */
 int mrn_cg(mrn_op *op, void *X, int iters, int use_graph, double *trace, void *stream);
+/* The same loop in two parts: mrn_cg_init sets X = 0, R = P = sum E^H W d;
+ * mrn_cg_iterate continues the iteration from the op's state for `iters`
+ * more steps (the hipGraph of one step is captured once and cached until the
+ * trajectory, maps or data change).  trace as in mrn_cg for these steps. */
+int mrn_cg_init(mrn_op *op, void *X, void *stream);
+int mrn_cg_iterate(mrn_op *op, void *X, int iters, int use_graph, double *trace, void *stream);
 
 /* Power method for the FISTA step (recon_lib.cpp:1791-1915): iters
  * applications of E'E starting from X = -0.5-0.5i; *step = 1/max_eig. */
@@ -154,7 +162,14 @@ int mrn_traj_grad(mrn_op *op, int frame, const void *X, const void *g, float *gr
  * the reference. */
 int mrn_autotune(mrn_op *op, int frame, void *stream);
 
-/* Wait for the op's work on `stream` (NULL = its own stream). */
+/* Average device time (ms, HIP events on `stream`) of each batched stage of
+ * one frame over `reps` runs: [0] set_image+z FFT, [1] y FFT, [2] x FFT,
+ * [3] gather, [4] scatter, [5] x IFFT, [6] y IFFT, [7] z IFFT+accumulate.
+ * Diagnostic; no counterpart in the reference (its -time_grid prints,
+ * gridFFT.cpp:727-747, time the same stages on the host). */
+int mrn_stage_times(mrn_op *op, int frame, const void *X, int reps, float ms[8], void *stream);
+
+/* Wait for the work queued on `stream` (NULL = the null stream). */
 int mrn_sync(mrn_op *op, void *stream);
 
 const char *mrn_status_string(int status);

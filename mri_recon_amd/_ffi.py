@@ -20,8 +20,9 @@ EXPORTS = [
     "mrn_params_default", "mrn_parse_args", "mrn_op_create", "mrn_op_destroy",
     "mrn_get_geometry", "mrn_get_table", "mrn_get_window", "mrn_set_traj", "mrn_set_smaps",
     "mrn_image_to_kspace", "mrn_kspace_to_image", "mrn_normal", "mrn_coil_backward",
-    "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_power_method", "mrn_fista",
-    "mrn_traj_grad", "mrn_autotune", "mrn_sync", "mrn_status_string", "mrn_last_error",
+    "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_cg_init", "mrn_cg_iterate",
+    "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_stage_times",
+    "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
 
 
@@ -81,6 +82,9 @@ def lib():
             "mrn_coil_forward": (i32, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
             "mrn_set_kdata": (i32, [vp, i32, vp, vp]),
             "mrn_cg": (i32, [vp, vp, i32, i32, vp, vp]),
+            "mrn_cg_init": (i32, [vp, vp, vp]),
+            "mrn_cg_iterate": (i32, [vp, vp, i32, i32, vp, vp]),
+            "mrn_stage_times": (i32, [vp, i32, vp, i32, vp, vp]),
             "mrn_power_method": (i32, [vp, i32, ctypes.POINTER(dbl), vp]),
             "mrn_fista": (i32, [vp, vp, i32, dbl, i32, vp, vp]),
             "mrn_traj_grad": (i32, [vp, i32, vp, vp, vp, vp]),

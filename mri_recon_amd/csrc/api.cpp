@@ -11,12 +11,14 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cstdio>
 
 #include "mrn_internal.hpp"
 
 namespace mrn {
-int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, hipStream_t);
+int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, bool, hipStream_t);
+int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_gather_plain(mrn_op *, const float2 *, float2 *, int64_t, const float *, const float *,
                         const float *, const float *, const float2 *, hipStream_t);
@@ -64,42 +66,74 @@ static PrePostArgs prepost(const mrn_op *op, int ch, int overwrite, int CC) {
 }
 
 // ---------------------------------------------------- batched transforms
-// image -> full k-space grid of every coil block
-static int fwd_grid(mrn_op *op, const float2 *X, hipStream_t st) {
+// Lines of the y and x passes of every coil block.  Forward passes load the
+// crop range (the rest is zero) and store everything; inverse passes load
+// everything and store the crop range (the rest is never read).
+static LinesArgs ypass_args(const mrn_op *op, bool inverse) {
   const Geom &g = op->g;
   const int CC = g.CC;
-  const int64_t cg = ncell(op) * CC;
-  MRN_TRY(fft_pre_z(op, op->grid, X, op->smaps, prepost(op, 0, 0, CC), st));
   LinesArgs y{};
   y.off0 = (int64_t)g.og[0] * CC;
   y.ystride = (int64_t)g.S[1] * g.S[0] * CC;
-  y.zstride = cg;
+  y.zstride = ncell(op) * CC;
   y.pstride = (int64_t)g.S[0] * CC;
   y.rstride = 0;
   y.run = g.N[0] * CC;
   y.nlines = g.N[0] * CC;
-  y.ld0 = g.og[1];
-  y.ld1 = g.og[1] + g.N[1];
-  y.st0 = 0;
-  y.st1 = g.S[1];
+  y.ld0 = inverse ? 0 : g.og[1];
+  y.ld1 = inverse ? g.S[1] : g.og[1] + g.N[1];
+  y.st0 = inverse ? g.og[1] : 0;
+  y.st1 = inverse ? g.og[1] + g.N[1] : g.S[1];
   y.scale = fft_scale(g.S[1]);
-  MRN_TRY(fft_lines(op, g.S[1], -1, op->grid, y, dim3(cdiv(y.nlines, FFT_LINES), g.S[2], op->nch),
-                    op->tw_d[1], st));
+  return y;
+}
+
+static LinesArgs xpass_args(const mrn_op *op, bool inverse) {
+  const Geom &g = op->g;
+  const int CC = g.CC;
   LinesArgs x{};
   x.off0 = 0;
   x.ystride = (int64_t)g.S[1] * g.S[0] * CC;
-  x.zstride = cg;
+  x.zstride = ncell(op) * CC;
   x.pstride = CC;
   x.rstride = (int64_t)g.S[0] * CC;
   x.run = CC;
   x.nlines = g.S[1] * CC;
-  x.ld0 = g.og[0];
-  x.ld1 = g.og[0] + g.N[0];
-  x.st0 = 0;
-  x.st1 = g.S[0];
+  x.ld0 = inverse ? 0 : g.og[0];
+  x.ld1 = inverse ? g.S[0] : g.og[0] + g.N[0];
+  x.st0 = inverse ? g.og[0] : 0;
+  x.st1 = inverse ? g.og[0] + g.N[0] : g.S[0];
   x.scale = fft_scale(g.S[0]);
-  MRN_TRY(fft_lines(op, g.S[0], -1, op->grid, x, dim3(cdiv(x.nlines, FFT_LINES), g.S[2], op->nch),
-                    op->tw_d[0], st));
+  return x;
+}
+
+// forward stage q of the batched transform: 0 set_image+z, 1 y, 2 x
+static int fwd_stage(mrn_op *op, int q, const float2 *X, hipStream_t st) {
+  const Geom &g = op->g;
+  if (q == 0) return fft_pre_z(op, op->grid, X, op->smaps, prepost(op, 0, 0, g.CC), st);
+  const LinesArgs a = (q == 1) ? ypass_args(op, false) : xpass_args(op, false);
+  const int S = g.S[q == 1 ? 1 : 0];
+  return fft_lines(op, S, -1, op->grid, a, dim3(cdiv(a.nlines, FFT_LINES), g.S[2], op->nch),
+                   op->tw_d[q == 1 ? 1 : 0], st);
+}
+
+// inverse stage q: 0 x, 1 y, 2 z+accumulate over the coil blocks in order
+static int adj_stage(mrn_op *op, int q, float2 *X, bool overwrite, hipStream_t st) {
+  const Geom &g = op->g;
+  if (q == 2) {
+    for (int ch = 0; ch < op->nch; ch++)
+      MRN_TRY(fft_post_z(op, op->grid, X, op->smaps, prepost(op, ch, overwrite && ch == 0, g.CC), st));
+    return MRN_OK;
+  }
+  const LinesArgs a = (q == 1) ? ypass_args(op, true) : xpass_args(op, true);
+  const int S = g.S[q == 1 ? 1 : 0];
+  return fft_lines(op, S, 1, op->grid, a, dim3(cdiv(a.nlines, FFT_LINES), g.S[2], op->nch),
+                   op->tw_d[q == 1 ? 1 : 0], st);
+}
+
+// image -> full k-space grid of every coil block
+static int fwd_grid(mrn_op *op, const float2 *X, hipStream_t st) {
+  for (int q = 0; q < 3; q++) MRN_TRY(fwd_stage(op, q, X, st));
   return MRN_OK;
 }
 
@@ -107,40 +141,7 @@ int fwd_grid_pub(mrn_op *op, const float2 *X, hipStream_t st) { return fwd_grid(
 
 // full grids of every coil block -> image (X = or += sum over coils)
 static int adj_grid(mrn_op *op, float2 *X, bool overwrite, hipStream_t st) {
-  const Geom &g = op->g;
-  const int CC = g.CC;
-  const int64_t cg = ncell(op) * CC;
-  LinesArgs x{};
-  x.off0 = 0;
-  x.ystride = (int64_t)g.S[1] * g.S[0] * CC;
-  x.zstride = cg;
-  x.pstride = CC;
-  x.rstride = (int64_t)g.S[0] * CC;
-  x.run = CC;
-  x.nlines = g.S[1] * CC;
-  x.ld0 = 0;
-  x.ld1 = g.S[0];
-  x.st0 = g.og[0];
-  x.st1 = g.og[0] + g.N[0];
-  x.scale = fft_scale(g.S[0]);
-  MRN_TRY(fft_lines(op, g.S[0], 1, op->grid, x, dim3(cdiv(x.nlines, FFT_LINES), g.S[2], op->nch),
-                    op->tw_d[0], st));
-  LinesArgs y{};
-  y.off0 = (int64_t)g.og[0] * CC;
-  y.ystride = (int64_t)g.S[1] * g.S[0] * CC;
-  y.zstride = cg;
-  y.pstride = (int64_t)g.S[0] * CC;
-  y.run = g.N[0] * CC;
-  y.nlines = g.N[0] * CC;
-  y.ld0 = 0;
-  y.ld1 = g.S[1];
-  y.st0 = g.og[1];
-  y.st1 = g.og[1] + g.N[1];
-  y.scale = fft_scale(g.S[1]);
-  MRN_TRY(fft_lines(op, g.S[1], 1, op->grid, y, dim3(cdiv(y.nlines, FFT_LINES), g.S[2], op->nch),
-                    op->tw_d[1], st));
-  for (int ch = 0; ch < op->nch; ch++)
-    MRN_TRY(fft_post_z(op, op->grid, X, op->smaps, prepost(op, ch, overwrite && ch == 0, CC), st));
+  for (int q = 0; q < 3; q++) MRN_TRY(adj_stage(op, q, X, overwrite, st));
   return MRN_OK;
 }
 
@@ -242,7 +243,7 @@ static int normal_frame(mrn_op *op, int f, const float2 *X, float2 *out, hipStre
   FrameData &fr = op->frames[f];
   if (batched(op)) {
     MRN_TRY(fwd_grid(op, X, st));
-    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, st));
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, true, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
     return adj_grid(op, out, true, st);
   }
@@ -260,7 +261,7 @@ static int grad_frame(mrn_op *op, int f, const float2 *X, float2 *R, hipStream_t
   FrameData &fr = op->frames[f];
   if (batched(op)) {
     MRN_TRY(fwd_grid(op, X, st));
-    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, fr.kdata, st));
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, fr.kdata, true, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
     return adj_grid(op, R, true, st);
   }
@@ -278,7 +279,8 @@ static int grad_frame(mrn_op *op, int f, const float2 *X, float2 *R, hipStream_t
 static int adjoint_data_frame(mrn_op *op, int f, float2 *R, hipStream_t st) {
   FrameData &fr = op->frames[f];
   if (batched(op)) {
-    MRN_TRY(launch_scatter(op, op->grid, fr.kdata, fr, st));
+    MRN_TRY(launch_weight(op, op->kwork, fr.kdata, fr, st));
+    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
     return adj_grid(op, R, true, st);
   }
   MRN_TRY(op_zero(op, R, op->nvox, st));
@@ -288,7 +290,7 @@ static int adjoint_data_frame(mrn_op *op, int f, float2 *R, hipStream_t st) {
   return MRN_OK;
 }
 
-static int ensure_solver(mrn_op *op, int trace_len) {
+static int ensure_solver(mrn_op *op) {
   const int64_t n = (int64_t)op->F * op->nvox;
   if (!op->vR) {
     MRN_HIP(op, hipMalloc(&op->vR, sizeof(float2) * n));
@@ -296,13 +298,32 @@ static int ensure_solver(mrn_op *op, int trace_len) {
     MRN_HIP(op, hipMalloc(&op->vL, sizeof(float2) * n));
     MRN_HIP(op, hipMalloc(&op->vX0, sizeof(float2) * n));
   }
-  if (trace_len > op->dtrace_cap) {
-    hipFree(op->dtrace);
-    op->dtrace = nullptr;
-    MRN_HIP(op, hipMalloc(&op->dtrace, sizeof(double) * trace_len));
-    op->dtrace_cap = trace_len;
+  if (!op->dtrace) {
+    MRN_HIP(op, hipMalloc(&op->dtrace, sizeof(double) * 4 * MRN_TRACE_CAP));
+    op->dtrace_cap = MRN_TRACE_CAP;
   }
   return MRN_OK;
+}
+
+// Copy trace entries [first, first + count) of the ring (width doubles each).
+static int copy_trace(mrn_op *op, double *host, int64_t first, int64_t count, int width, hipStream_t st) {
+  for (int64_t j = 0; j < count;) {
+    const int64_t pos = (first + j) % MRN_TRACE_CAP;
+    const int64_t run = std::min<int64_t>(count - j, MRN_TRACE_CAP - pos);
+    MRN_HIP(op, hipMemcpyAsync(host + j * width, op->dtrace + pos * width, sizeof(double) * width * run,
+                               hipMemcpyDeviceToHost, st));
+    j += run;
+  }
+  MRN_HIP(op, hipStreamSynchronize(st));
+  return MRN_OK;
+}
+
+static void drop_cg_graph(mrn_op *op) {
+  if (op->cg_exec) (void)hipGraphExecDestroy(op->cg_exec);
+  if (op->cg_graph) (void)hipGraphDestroy(op->cg_graph);
+  op->cg_exec = nullptr;
+  op->cg_graph = nullptr;
+  op->cg_X = nullptr;
 }
 
 // Run `body` iters times on the caller's stream, or capture one call of it
@@ -384,6 +405,7 @@ int mrn_set_traj(mrn_op *op, int f, int64_t M, const float *kx, const float *ky,
     op->kplain_cap = M;
   }
   fr.has_traj = true;
+  op->version++;
   MRN_HIP(op, hipStreamSynchronize(st));
   return MRN_OK;
 }
@@ -396,6 +418,7 @@ int mrn_set_smaps(mrn_op *op, const void *smaps, void *stream) {
   } else {
     MRN_HIP(op, hipMemcpyAsync(op->smaps, smaps, sizeof(float2) * op->nvox * op->C, hipMemcpyDeviceToDevice, st));
   }
+  op->version++;
   return MRN_OK;
 }
 
@@ -416,18 +439,21 @@ int mrn_set_kdata(mrn_op *op, int f, const void *data, void *stream) {
     MRN_TRY(op_permute_in(op, fr.kdata, fr.kraw, fr, st));
   }
   fr.has_kdata = true;
+  op->version++;
+  op->cg_ready = false;
   MRN_HIP(op, hipStreamSynchronize(st));
   return MRN_OK;
 }
 
 int mrn_image_to_kspace(mrn_op *op, int f, const void *X, void *data, const void *sub, void *stream) {
-  if (!op || !X || !data) return MRN_EINVAL;
+  if (!op || !X) return MRN_EINVAL;
   MRN_TRY(check_frame(op, f, false));
+  if (!data && op->frames[f].M > 0) return fail(op, MRN_EINVAL, "null k-space pointer");
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
   if (batched(op)) {
     MRN_TRY(fwd_grid(op, (const float2 *)X, st));
-    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, st));
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, false, st));
     return op_permute_out(op, (float2 *)data, op->kwork, fr, (const float2 *)sub, st);
   }
   for (int c = 0; c < op->C; c++)
@@ -438,12 +464,14 @@ int mrn_image_to_kspace(mrn_op *op, int f, const void *X, void *data, const void
 }
 
 int mrn_kspace_to_image(mrn_op *op, int f, const void *data, void *X, int accumulate, void *stream) {
-  if (!op || !X || !data) return MRN_EINVAL;
+  if (!op || !X) return MRN_EINVAL;
   MRN_TRY(check_frame(op, f, false));
+  if (!data && op->frames[f].M > 0) return fail(op, MRN_EINVAL, "null k-space pointer");
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
   if (batched(op)) {
     MRN_TRY(op_permute_in(op, op->kwork, (const float2 *)data, fr, st));
+    MRN_TRY(launch_weight(op, op->kwork, op->kwork, fr, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
     return adj_grid(op, (float2 *)X, !accumulate, st);
   }
@@ -476,47 +504,90 @@ int mrn_coil_forward(mrn_op *op, void *X, const void *smap, const void *data, in
                        pick(op, stream));
 }
 
-int mrn_cg(mrn_op *op, void *Xv, int iters, int use_graph, double *trace, void *stream) {
-  if (!op || !Xv || iters < 0) return MRN_EINVAL;
-  for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, true));
-  MRN_TRY(ensure_solver(op, 4 * (iters > 0 ? iters : 1)));
-  hipStream_t st = pick(op, stream);
-  float2 *X = (float2 *)Xv;
+// one CG step (recon_lib.cpp:1578-1722) on the op's solver vectors
+static int cg_step(mrn_op *op, float2 *X, bool with_trace, hipStream_t s) {
   const int64_t n = (int64_t)op->F * op->nvox;
-  double *tr = trace ? op->dtrace : nullptr;
+  double *tr = with_trace ? op->dtrace : nullptr;
+  for (int f = 0; f < op->F; f++)
+    MRN_TRY(normal_frame(op, f, op->vP + f * op->nvox, op->vL + f * op->nvox, s));
+  MRN_TRY(op_cg_alpha(op, op->vR, op->vP, op->vL, n, tr, s));
+  if (!batched(op)) {  // plain form: the reference's host-side scalar round trip
+    double host_scal[16];
+    MRN_HIP(op, hipMemcpyAsync(host_scal, op->dscal, sizeof(host_scal), hipMemcpyDeviceToHost, s));
+    MRN_HIP(op, hipStreamSynchronize(s));
+  }
+  MRN_TRY(op_cg_update(op, X, op->vR, op->vP, op->vL, n, tr, s));
+  if (!batched(op)) {
+    double host_scal[16];
+    MRN_HIP(op, hipMemcpyAsync(host_scal, op->dscal, sizeof(host_scal), hipMemcpyDeviceToHost, s));
+    MRN_HIP(op, hipStreamSynchronize(s));
+  }
+  return MRN_OK;
+}
+
+int mrn_cg_init(mrn_op *op, void *Xv, void *stream) {
+  if (!op || !Xv) return MRN_EINVAL;
+  for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, true));
+  MRN_TRY(ensure_solver(op));
+  hipStream_t st = pick(op, stream);
+  const int64_t n = (int64_t)op->F * op->nvox;
   MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
   for (int f = 0; f < op->F; f++) MRN_TRY(adjoint_data_frame(op, f, op->vR + f * op->nvox, st));
   MRN_TRY(op_copy(op, op->vP, op->vR, n, st));
-  MRN_TRY(op_zero(op, X, n, st));
-  const bool plain = !batched(op);
-  double host_scal[16];
-  auto body = [&](hipStream_t s) -> int {
-    for (int f = 0; f < op->F; f++)
-      MRN_TRY(normal_frame(op, f, op->vP + f * op->nvox, op->vL + f * op->nvox, s));
-    MRN_TRY(op_cg_alpha(op, op->vR, op->vP, op->vL, n, tr, s));
-    if (plain) {  // the reference's host-side scalar round trip
-      MRN_HIP(op, hipMemcpyAsync(host_scal, op->dscal, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
-      MRN_HIP(op, hipStreamSynchronize(s));
-    }
-    MRN_TRY(op_cg_update(op, X, op->vR, op->vP, op->vL, n, tr, s));
-    if (plain) {
-      MRN_HIP(op, hipMemcpyAsync(host_scal, op->dscal, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
-      MRN_HIP(op, hipStreamSynchronize(s));
-    }
-    return MRN_OK;
-  };
-  MRN_TRY(run_iters(op, iters, use_graph && !plain, st, body));
-  if (trace && iters > 0) {
-    MRN_HIP(op, hipMemcpyAsync(trace, op->dtrace, sizeof(double) * 4 * iters, hipMemcpyDeviceToHost, st));
-    MRN_HIP(op, hipStreamSynchronize(st));
-  }
+  MRN_TRY(op_zero(op, (float2 *)Xv, n, st));
+  op->cg_ready = true;
+  op->cg_count = 0;
   return MRN_OK;
+}
+
+int mrn_cg_iterate(mrn_op *op, void *Xv, int iters, int use_graph, double *trace, void *stream) {
+  if (!op || !Xv || iters < 0) return MRN_EINVAL;
+  if (!op->cg_ready) return fail(op, MRN_ESTATE, "mrn_cg_init not called");
+  hipStream_t st = pick(op, stream);
+  float2 *X = (float2 *)Xv;
+  const int64_t first = op->cg_count;
+  if (use_graph && batched(op) && iters > 0) {
+    if (!op->cg_exec || op->cg_X != Xv || op->cg_version != op->version) {
+      drop_cg_graph(op);
+      MRN_HIP(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+      int rc = cg_step(op, X, true, op->stream);
+      hipError_t ec = hipStreamEndCapture(op->stream, &op->cg_graph);
+      if (rc != MRN_OK) {
+        drop_cg_graph(op);
+        return rc;
+      }
+      MRN_HIP(op, ec);
+      MRN_HIP(op, hipGraphInstantiate(&op->cg_exec, op->cg_graph, nullptr, nullptr, 0));
+      op->cg_X = Xv;
+      op->cg_version = op->version;
+    }
+    hipEvent_t ev0, ev1;
+    MRN_HIP(op, hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+    MRN_HIP(op, hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+    MRN_HIP(op, hipEventRecord(ev0, st));
+    MRN_HIP(op, hipStreamWaitEvent(op->stream, ev0, 0));
+    for (int it = 0; it < iters; it++) MRN_HIP(op, hipGraphLaunch(op->cg_exec, op->stream));
+    MRN_HIP(op, hipEventRecord(ev1, op->stream));
+    MRN_HIP(op, hipStreamWaitEvent(st, ev1, 0));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+  } else {
+    for (int it = 0; it < iters; it++) MRN_TRY(cg_step(op, X, true, st));
+  }
+  op->cg_count += iters;
+  if (trace && iters > 0) MRN_TRY(copy_trace(op, trace, first, iters, 4, st));
+  return MRN_OK;
+}
+
+int mrn_cg(mrn_op *op, void *Xv, int iters, int use_graph, double *trace, void *stream) {
+  MRN_TRY(mrn_cg_init(op, Xv, stream));
+  return mrn_cg_iterate(op, Xv, iters, use_graph, trace, stream);
 }
 
 int mrn_power_method(mrn_op *op, int iters, double *step, void *stream) {
   if (!op || iters <= 0) return MRN_EINVAL;
   for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, false));
-  MRN_TRY(ensure_solver(op, iters));
+  MRN_TRY(ensure_solver(op));
   hipStream_t st = pick(op, stream);
   const int64_t n = (int64_t)op->F * op->nvox;
   MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
@@ -536,7 +607,7 @@ int mrn_power_method(mrn_op *op, int iters, double *step, void *stream) {
 int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, double *resid, void *stream) {
   if (!op || !Xv || iters < 0) return MRN_EINVAL;
   for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, true));
-  MRN_TRY(ensure_solver(op, iters > 0 ? iters : 1));
+  MRN_TRY(ensure_solver(op));
   hipStream_t st = pick(op, stream);
   float2 *X = (float2 *)Xv;
   const int64_t n = (int64_t)op->F * op->nvox;
@@ -551,10 +622,7 @@ int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, doubl
     return op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s);
   };
   MRN_TRY(run_iters(op, iters, use_graph && batched(op), st, body));
-  if (resid && iters > 0) {
-    MRN_HIP(op, hipMemcpyAsync(resid, op->dtrace, sizeof(double) * iters, hipMemcpyDeviceToHost, st));
-    MRN_HIP(op, hipStreamSynchronize(st));
-  }
+  if (resid && iters > 0) MRN_TRY(copy_trace(op, resid, 0, iters, 1, st));
   return MRN_OK;
 }
 
@@ -569,7 +637,45 @@ int mrn_autotune(mrn_op *op, int f, void *stream) {
   if (!op) return MRN_EINVAL;
   MRN_TRY(check_frame(op, f, false));
   if (!batched(op)) return MRN_OK;
+  op->version++;
   return autotune_impl(op, f, pick(op, stream));
+}
+
+int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[8], void *stream) {
+  if (!op || !Xv || !ms || reps <= 0) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  if (!batched(op)) return fail(op, MRN_EUNSUPPORTED, "stage times cover the batched form");
+  hipStream_t st = pick(op, stream);
+  FrameData &fr = op->frames[f];
+  const float2 *X = (const float2 *)Xv;
+  MRN_TRY(ensure_solver(op));
+  hipEvent_t ev[9];
+  for (int q = 0; q < 9; q++) MRN_HIP(op, hipEventCreate(&ev[q]));
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < reps; r++) {
+    MRN_HIP(op, hipEventRecord(ev[0], st));
+    for (int q = 0; q < 3; q++) {
+      MRN_TRY(fwd_stage(op, q, X, st));
+      MRN_HIP(op, hipEventRecord(ev[q + 1], st));
+    }
+    MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, true, st));
+    MRN_HIP(op, hipEventRecord(ev[4], st));
+    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
+    MRN_HIP(op, hipEventRecord(ev[5], st));
+    for (int q = 0; q < 3; q++) {
+      MRN_TRY(adj_stage(op, q, op->vX0, true, st));
+      MRN_HIP(op, hipEventRecord(ev[q + 6], st));
+    }
+    MRN_HIP(op, hipEventSynchronize(ev[8]));
+    for (int q = 0; q < 8; q++) {
+      float t = 0.f;
+      MRN_HIP(op, hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+      acc[q] += t;
+    }
+  }
+  for (int q = 0; q < 9; q++) (void)hipEventDestroy(ev[q]);
+  for (int q = 0; q < 8; q++) ms[q] = (float)(acc[q] / reps);
+  return MRN_OK;
 }
 
 int mrn_sync(mrn_op *op, void *stream) {

@@ -16,7 +16,8 @@
 
 namespace mrn {
 int fwd_grid_pub(mrn_op *op, const float2 *X, hipStream_t st);
-int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, hipStream_t);
+int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, bool, hipStream_t);
+int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int build_bins(mrn_op *, FrameData &, int64_t, const float *, const float *, const float *,
                const float *, hipStream_t);
@@ -135,41 +136,21 @@ int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float 
 }
 
 // ------------------------------------------------------------ autotuner
-// Candidate scatter shapes (tile edge, coils per workgroup, visits per work
-// item) are timed on the frame's own trajectory; the fastest is kept and the
-// bins are rebuilt for it.  Every candidate computes the same result.
+// The scatter's work-item size (samples per wavefront before a tile is split
+// into partial sums) is timed on the frame's own trajectory and the fastest is
+// kept; every candidate computes the same result.
 int autotune_impl(mrn_op *op, int f, hipStream_t st) {
   FrameData &fr = op->frames[f];
-  const Tiles keep = op->t;
-  int ntap = 1;
-  for (int a = 0; a < 3; a++)
-    if (op->g.grid_in[a]) ntap = std::max(ntap, (int)floorf(2 * op->g.dwin[a]) + 1);
-  struct Cand {
-    int T, CCs, maxv;
-  };
-  std::vector<Cand> cands;
-  for (int T : {8, 12, 16})
-    for (int CCs : {1, 2, 4, 8})
-      for (int maxv : {1024, 2048, 4096}) {
-        if (T < ntap - 1 || CCs > op->g.CC) continue;
-        if ((int64_t)T * T * T * CCs * 8 + 4 * (op->g.ntab + 4) > 64 * 1024) continue;
-        cands.push_back({T, CCs, maxv});
-      }
-  // non-zero samples so that no candidate skips work
+  const int keep = op->t.maxv;
+  // non-zero samples so that the timing covers the arithmetic
   MRN_TRY(op_fill(op, op->kwork, fr.Mv * op->g.CC * op->nch, make_float2(1.f, 0.f), st));
   hipEvent_t e0, e1;
   MRN_HIP(op, hipEventCreate(&e0));
   MRN_HIP(op, hipEventCreate(&e1));
   float best = 1e30f;
-  Cand bestc{keep.T[0], keep.CCs, keep.maxv};
-  for (const Cand &cd : cands) {
-    for (int a = 0; a < 3; a++) {
-      op->t.T[a] = cd.T;
-      op->t.nt[a] = (op->g.S[a] + cd.T - 1) / cd.T;
-    }
-    op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
-    op->t.maxv = cd.maxv;
-    op->t.CCs = cd.CCs;
+  int bestv = keep;
+  for (int maxv : {1024, 2048, 4096, 8192, 16384}) {
+    op->t.maxv = maxv;
     MRN_TRY(build_bins(op, fr, fr.M, fr.traw, fr.traw + fr.M, fr.traw + 2 * fr.M, fr.traw + 3 * fr.M, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));  // warm-up
     MRN_HIP(op, hipEventRecord(e0, st));
@@ -180,19 +161,13 @@ int autotune_impl(mrn_op *op, int f, hipStream_t st) {
     MRN_HIP(op, hipEventElapsedTime(&ms, e0, e1));
     if (ms < best) {
       best = ms;
-      bestc = cd;
+      bestv = maxv;
     }
   }
-  for (int a = 0; a < 3; a++) {
-    op->t.T[a] = bestc.T;
-    op->t.nt[a] = (op->g.S[a] + bestc.T - 1) / bestc.T;
-  }
-  op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
-  op->t.maxv = bestc.maxv;
-  op->t.CCs = bestc.CCs;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  // rebuild every frame's bins for the chosen shape (kdata follows the new order)
+  op->t.maxv = bestv;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // rebuild every frame's work list for the chosen size (sample order is unchanged)
   for (int q = 0; q < op->F; q++) {
     FrameData &fq = op->frames[q];
     if (!fq.has_traj) continue;

@@ -6,14 +6,16 @@
 // wrap), table-lerp weights with the (-1)^l fftshift sign, kw/taper on the
 // scatter side only, kw == 0 and out-of-grid samples skipped.
 //
-// Batched form: samples are sorted once per trajectory by the tile of their
-// first tap.  The gather maps one lane to one (sample, coil) pair so that a
-// tap reads CC contiguous coils (CC*8 bytes) of a grid cell.  The scatter is
-// owner-computes: a workgroup owns one tile of the grid, accumulates every
-// tap that lands in it in LDS (from the samples of the tile's own bin and its
-// 7 lower neighbours) and writes the tile once with plain stores.  Tiles whose
-// sample count exceeds one work item are split; only those are pre-zeroed and
-// flushed with global atomics.
+// Batched form: samples are sorted once per trajectory by the 2x2x2-cell bin
+// of their first tap.  The gather maps one lane to one (sample, coil) pair so
+// that a tap reads CC contiguous coils (CC*8 bytes) of a grid cell.  The
+// scatter is grid-driven: one wavefront owns a 4x4x4 tile of cells, one lane
+// per cell with the CC coil sums in registers; the wavefront walks the samples
+// of the bins that can reach the tile (wave-uniform loads) and every lane adds
+// its own tap.  No LDS or global atomics on this path (LDS float atomics run
+// at about 3 cycles per lane on gfx950); a tile is written once with plain
+// stores.  Tiles whose sample list exceeds one work item are split; only those
+// are pre-zeroed and flushed with global atomics.
 #include <hipcub/hipcub.hpp>
 
 #include "mrn_internal.hpp"
@@ -35,12 +37,14 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
+// Sort key of a sample: the bin of its first tap (bins ordered z, y, x);
+// skipped samples (kw == 0 or outside the grid) get the sentinel nbins.
 __global__ void k_traj_keys(const float *__restrict__ kx, const float *__restrict__ ky,
                             const float *__restrict__ kz, const float *__restrict__ kw, int64_t M,
                             Geom g, Tiles t, uint32_t *keys, int32_t *vals, int32_t *counts) {
   int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
-  uint32_t key = (uint32_t)t.ntiles;
+  uint32_t key = (uint32_t)t.nbins;
   if (kw[m] != 0.0f) {
     const float k[3] = {kx[m], ky[m], kz[m]};
     bool ok = true;
@@ -49,13 +53,13 @@ __global__ void k_traj_keys(const float *__restrict__ kx, const float *__restric
     for (int a = 0; a < 3; a++) {
       int s, e;
       ok = ok && axis_range(g, a, grid_coord(g, a, k[a]), s, e);
-      b[a] = ok ? s / t.T[a] : 0;
+      b[a] = ok ? s / t.B : 0;
     }
-    if (ok) key = (uint32_t)((b[2] * t.nt[1] + b[1]) * t.nt[0] + b[0]);
+    if (ok) key = (uint32_t)((b[2] * t.nb[1] + b[1]) * t.nb[0] + b[0]);
   }
   keys[m] = key;
   vals[m] = (int32_t)m;
-  if (key < (uint32_t)t.ntiles) atomicAdd(&counts[key], 1);
+  if (key < (uint32_t)t.nbins) atomicAdd(&counts[key], 1);
 }
 
 __global__ void k_traj_finish(const float *__restrict__ kx, const float *__restrict__ ky,
@@ -76,18 +80,31 @@ __global__ void k_traj_finish(const float *__restrict__ kx, const float *__restr
   inv[m] = (int32_t)i;
 }
 
+// Bins (per axis) whose samples can have a tap in tile `ti`: first taps in
+// [ti*T - NT + 1, ti*T + T - 1].
+__host__ __device__ __forceinline__ void tile_bins(const Tiles &t, int a, int ti, int &b0, int &b1) {
+  int lo = ti * t.T - t.NT + 1;
+  lo = lo < 0 ? 0 : lo;
+  const int hi = ti * t.T + t.T - 1;
+  b0 = lo / t.B;
+  b1 = hi / t.B;
+  if (b1 > t.nb[a] - 1) b1 = t.nb[a] - 1;
+}
+
 __global__ void k_tile_nsub(const int32_t *__restrict__ bin_start, Tiles t, int32_t *nsub,
                             int32_t *visits) {
   int tile = blockIdx.x * blockDim.x + threadIdx.x;
   if (tile >= t.ntiles) return;
   const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
+  int bx0, bx1, by0, by1, bz0, bz1;
+  tile_bins(t, 0, tx, bx0, bx1);
+  tile_bins(t, 1, ty, by0, by1);
+  tile_bins(t, 2, tz, bz0, bz1);
   int v = 0;
-  for (int dz = 0; dz < 2; dz++)
-    for (int dy = 0; dy < 2; dy++)
-      for (int dx = 0; dx < 2; dx++) {
-        const int bx = tx - dx, by = ty - dy, bz = tz - dz;
-        if (bx < 0 || by < 0 || bz < 0) continue;
-        const int b = (bz * t.nt[1] + by) * t.nt[0] + bx;
+  for (int bz = bz0; bz <= bz1; bz++)
+    for (int by = by0; by <= by1; by++)
+      for (int bx = bx0; bx <= bx1; bx++) {
+        const int b = (bz * t.nb[1] + by) * t.nb[0] + bx;
         v += bin_start[b + 1] - bin_start[b];
       }
   visits[tile] = v;
@@ -120,7 +137,8 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
                                                 const float4 *__restrict__ coords,
                                                 const float2 *__restrict__ sub, int64_t Mv, Geom g,
                                                 const float *__restrict__ tables,
-                                                int64_t chunk_grid, int64_t chunk_k) {
+                                                int64_t chunk_grid, int64_t chunk_k,
+                                                const float *__restrict__ taper, int weighted) {
   extern __shared__ float tab_s[];
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
@@ -188,6 +206,11 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
     ar -= d.x;
     ai -= d.y;
   }
+  if (weighted) {  // the scatter's input kw * taper * y (gridFFT.cpp:1040-1108)
+    const float tp = taper[i];
+    ar = (ar * k.w) * tp;
+    ai = (ai * k.w) * tp;
+  }
   out[oi] = make_float2(ar, ai);
 }
 
@@ -196,136 +219,101 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
                              int64_t chunk_grid) {
   const int tile = tiles[blockIdx.x];
   const int ch = blockIdx.y;
-  const int tx0 = (tile % t.nt[0]) * t.T[0], ty0 = ((tile / t.nt[0]) % t.nt[1]) * t.T[1],
-            tz0 = (tile / (t.nt[0] * t.nt[1])) * t.T[2];
-  const int ncell = t.T[0] * t.T[1] * t.T[2] * g.CC;
+  const int tx0 = (tile % t.nt[0]) * t.T, ty0 = ((tile / t.nt[0]) % t.nt[1]) * t.T,
+            tz0 = (tile / (t.nt[0] * t.nt[1])) * t.T;
+  const int ncell = t.T * t.T * t.T * g.CC;
   for (int idx = threadIdx.x; idx < ncell; idx += blockDim.x) {
     const int cell = idx >> g.lgCC, c = idx & (g.CC - 1);
-    const int x = tx0 + cell % t.T[0], y = ty0 + (cell / t.T[0]) % t.T[1], z = tz0 + cell / (t.T[0] * t.T[1]);
+    const int x = tx0 + cell % t.T, y = ty0 + (cell / t.T) % t.T, z = tz0 + cell / (t.T * t.T);
     if (x < g.S[0] && y < g.S[1] && z < g.S[2])
       grid[ch * chunk_grid + (((int64_t)z * g.S[1] + y) * g.S[0] + x) * g.CC + c] = make_float2(0.f, 0.f);
   }
 }
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_scatter_tiles(
+// Grid-driven scatter.  Wavefront w of the workgroup takes work item
+// subs[4*blockIdx.x + w] = {tile, first visit, end visit, heavy}; lane l owns
+// cell (l&3, (l>>2)&3, l>>4) of the 4x4x4 tile and the CCT coil sums of coil
+// block blockIdx.y.  kin holds kw * taper * data (internal sample order).
+template <int CCT>
+__global__ __launch_bounds__(256) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
-    const float *__restrict__ taper, const int32_t *__restrict__ bin_start,
-    const int4 *__restrict__ subs, Geom g, Tiles t, const float *__restrict__ tables,
-    int64_t chunk_grid, int64_t chunk_k, int CCs, int lgCCs) {
-  extern __shared__ float smem[];
-  float *tab_s = smem;
-  float *acc = smem + ((g.ntab + 3) & ~3);
-  const int4 sp = subs[blockIdx.x];
-  const int tile = sp.x;
-  const int nsc = g.CC >> lgCCs;
-  const int ch = blockIdx.y / nsc, cb = (blockIdx.y % nsc) * CCs;
-  const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
-  const int t0[3] = {tx * t.T[0], ty * t.T[1], tz * t.T[2]};
-  const int nacc = t.T[0] * t.T[1] * t.T[2] * CCs;
+    const int32_t *__restrict__ bin_start, const int4 *__restrict__ subs, int nsub, Geom g,
+    Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
+  extern __shared__ float tab_s[];
   load_tables(tab_s, tables, g.ntab);
-  for (int q = threadIdx.x; q < 2 * nacc; q += blockDim.x) acc[q] = 0.f;
-  // the 8 source bins: this tile and its lower neighbours
-  int bs[8], pre[9];
-  pre[0] = 0;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int bx = tx - (q & 1), by = ty - ((q >> 1) & 1), bz = tz - (q >> 2);
-    int cnt = 0, st = 0;
-    if (bx >= 0 && by >= 0 && bz >= 0) {
-      const int b = (bz * t.nt[1] + by) * t.nt[0] + bx;
-      st = bin_start[b];
-      cnt = bin_start[b + 1] - st;
-    }
-    bs[q] = st;
-    pre[q + 1] = pre[q] + cnt;
-  }
   __syncthreads();
-  const int c = threadIdx.x & (CCs - 1);
-  const int lane_s = threadIdx.x >> lgCCs, nls = blockDim.x >> lgCCs;
-  for (int v = sp.y + lane_s; v < sp.z; v += nls) {
-    int q = 0;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int sidx = blockIdx.x * 4 + wave;
+  if (sidx >= nsub) return;
+  const int4 sp = subs[sidx];
+  const int tile = sp.x;
+  const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
+  const int t0x = tix * 4, t0y = tiy * 4, t0z = tiz * 4;
+  const int lx = t0x + (lane & 3), ly = t0y + ((lane >> 2) & 3), lz = t0z + (lane >> 4);
+  const int ch = blockIdx.y;
+  const float2 *kch = kin + ch * chunk_k;
+  float ar[CCT], ai[CCT];
 #pragma unroll
-    for (int r = 1; r < 8; r++) q += (v >= pre[r]);
-    const int64_t i = bs[q] + (v - pre[q]);
-    const float4 k = coords[i];
-    const float2 d = kin[ch * chunk_k + i * g.CC + cb + c];
-    float vr = d.x * k.w, vi = d.y * k.w;
-    if (vr == 0.f && vi == 0.f) continue;
-    const float tp = taper[i];
-    vr *= tp;
-    vi *= tp;
-    const float dk[3] = {k.x, k.y, k.z};
-    int s[3], n[3], lo[3];
-    bool wide = false;
+  for (int c = 0; c < CCT; c++) ar[c] = ai[c] = 0.f;
+  int bx0, bx1, by0, by1, bz0, bz1;
+  tile_bins(t, 0, tix, bx0, bx1);
+  tile_bins(t, 1, tiy, by0, by1);
+  tile_bins(t, 2, tiz, bz0, bz1);
+  int v = 0;
+  for (int bz = bz0; bz <= bz1; bz++)
+    for (int by = by0; by <= by1; by++)
+      for (int bx = bx0; bx <= bx1; bx++) {
+        const int b = (bz * t.nb[1] + by) * t.nb[0] + bx;
+        const int s0 = bin_start[b], n = bin_start[b + 1] - s0;
+        const int j0 = max(sp.y - v, 0), j1 = min(sp.z - v, n);
+        v += n;
+        for (int j = j0; j < j1; j++) {
+          const int64_t i = s0 + j;
+          const float4 k = coords[i];
+          int sx, ex, sy, ey, sz, ez;
+          axis_range(g, 0, k.x, sx, ex);
+          axis_range(g, 1, k.y, sy, ey);
+          axis_range(g, 2, k.z, sz, ez);
+          if (ex < t0x || sx > t0x + 3 || ey < t0y || sy > t0y + 3 || ez < t0z || sz > t0z + 3) continue;
+          float2 d[CCT];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-      int e;
-      axis_range(g, a, dk[a], s[a], e);
-      n[a] = e - s[a] + 1;
-      lo[a] = s[a] - t0[a];
-      wide = wide || n[a] > NT;
-    }
-    if (!wide) {
-      float w[3][NT];
+          for (int c = 0; c < CCT; c++) d[c] = kch[i * CCT + c];
+          if (lx >= sx && lx <= ex && ly >= sy && ly <= ey && lz >= sz && lz <= ez) {
+            const float w = (tap_weight(g, 2, tab_s + g.toff[2], k.z, lz) *
+                             tap_weight(g, 1, tab_s + g.toff[1], k.y, ly)) *
+                            tap_weight(g, 0, tab_s + g.toff[0], k.x, lx);
 #pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int tt = 0; tt < NT; tt++) {
-          const int l = lo[a] + tt;
-          const bool in = tt < n[a] && l >= 0 && l < t.T[a];
-          w[a][tt] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], s[a] + tt) : 0.0f;
-        }
-#pragma unroll
-      for (int zz = 0; zz < NT; zz++) {
-        if (w[2][zz] == 0.f) continue;
-#pragma unroll
-        for (int yy = 0; yy < NT; yy++) {
-          const float wzy = w[2][zz] * w[1][yy];
-          if (wzy == 0.f) continue;
-          const int rowc = ((lo[2] + zz) * t.T[1] + lo[1] + yy) * t.T[0] + lo[0];
-#pragma unroll
-          for (int xx = 0; xx < NT; xx++) {
-            const float ww = wzy * w[0][xx];
-            if (ww == 0.f) continue;
-            float *a2 = acc + 2 * (((rowc + xx) << lgCCs) + c);
-            atomicAdd(a2, ww * vr);
-            atomicAdd(a2 + 1, ww * vi);
+            for (int c = 0; c < CCT; c++) {
+              ar[c] += w * d[c].x;
+              ai[c] += w * d[c].y;
+            }
           }
         }
       }
-    } else {
-      for (int lz = max(s[2], t0[2]); lz < min(s[2] + n[2], t0[2] + t.T[2]); lz++) {
-        const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
-        for (int ly = max(s[1], t0[1]); ly < min(s[1] + n[1], t0[1] + t.T[1]); ly++) {
-          const float wzy = wz * tap_weight(g, 1, tab_s + g.toff[1], dk[1], ly);
-          for (int lx = max(s[0], t0[0]); lx < min(s[0] + n[0], t0[0] + t.T[0]); lx++) {
-            const float ww = wzy * tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
-            const int cell = ((lz - t0[2]) * t.T[1] + (ly - t0[1])) * t.T[0] + (lx - t0[0]);
-            float *a2 = acc + 2 * ((cell << lgCCs) + c);
-            atomicAdd(a2, ww * vr);
-            atomicAdd(a2 + 1, ww * vi);
-          }
-        }
-      }
+  if (lx >= g.S[0] || ly >= g.S[1] || lz >= g.S[2]) return;
+  float2 *dst = grid + ch * chunk_grid + (((int64_t)lz * g.S[1] + ly) * g.S[0] + lx) * CCT;
+  if (sp.w) {
+#pragma unroll
+    for (int c = 0; c < CCT; c++) {
+      if (ar[c] != 0.f) atomicAdd(&dst[c].x, ar[c]);
+      if (ai[c] != 0.f) atomicAdd(&dst[c].y, ai[c]);
     }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CCT; c++) dst[c] = make_float2(ar[c], ai[c]);
   }
-  __syncthreads();
-  const bool heavy = sp.w != 0;
-  float2 *gch = grid + ch * chunk_grid + cb;
-  for (int idx = threadIdx.x; idx < nacc; idx += blockDim.x) {
-    const int cell = idx >> lgCCs, cc = idx & (CCs - 1);
-    const int x = t0[0] + cell % t.T[0], y = t0[1] + (cell / t.T[0]) % t.T[1],
-              z = t0[2] + cell / (t.T[0] * t.T[1]);
-    if (x >= g.S[0] || y >= g.S[1] || z >= g.S[2]) continue;
-    const float2 val = make_float2(acc[2 * idx], acc[2 * idx + 1]);
-    float2 *dst = gch + (((int64_t)z * g.S[1] + y) * g.S[0] + x) * g.CC + cc;
-    if (heavy) {
-      if (val.x != 0.f) atomicAdd(&dst->x, val.x);
-      if (val.y != 0.f) atomicAdd(&dst->y, val.y);
-    } else {
-      *dst = val;
-    }
+}
+
+// kout = (kin * kw) * taper for the valid samples of one frame, every block
+__global__ void k_weight(float2 *kout, const float2 *kin, const float4 *__restrict__ coords,
+                         const float *__restrict__ taper, int64_t Mv, int CC, int lgCC, int64_t total) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = (idx >> lgCC) % Mv;
+    const float w = coords[i].w, tp = taper[i];
+    const float2 d = kin[idx];
+    kout[idx] = make_float2((d.x * w) * tp, (d.y * w) * tp);
   }
 }
 
@@ -435,15 +423,19 @@ int taps_main(const Geom &g) {
 }
 
 int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
-                  const float2 *sub, hipStream_t st) {
+                  const float2 *sub, bool weighted, hipStream_t st) {
   const Geom &g = op->g;
   if (fr.Mv == 0) return MRN_OK;
   dim3 g3(nblk(fr.Mv * g.CC, 256), op->nch);
   size_t sh = sizeof(float) * g.ntab;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
+  const int wflag = weighted ? 1 : 0;
   switch (taps_main(g)) {
-#define MRN_G(n) \
-  case n: hipLaunchKernelGGL((k_gather<n>), g3, dim3(256), sh, st, grid, out, fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck); break;
+#define MRN_G(n)                                                                                  \
+  case n:                                                                                         \
+    hipLaunchKernelGGL((k_gather<n>), g3, dim3(256), sh, st, grid, out, fr.coords, sub, fr.Mv, g, \
+                       op->tables_d, cg, ck, fr.taper, wflag);                                    \
+    break;
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
 #undef MRN_G
     default:
@@ -462,21 +454,31 @@ int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::Frame
     hipLaunchKernelGGL(k_zero_tiles, dim3(fr.nheavy, op->nch), dim3(256), 0, st, grid, fr.heavy, g, t, cg);
     MRN_HIP(op, hipGetLastError());
   }
-  int CCs = t.CCs, lg = 0;
-  while ((1 << lg) < CCs) lg++;
-  dim3 g3(fr.nsub, op->nch * (g.CC / CCs));
-  size_t sh = sizeof(float) * (((g.ntab + 3) & ~3) + 2 * t.T[0] * t.T[1] * t.T[2] * CCs);
-  switch (taps_main(g)) {
-#define MRN_S(n)                                                                                   \
-  case n:                                                                                          \
-    hipLaunchKernelGGL((k_scatter_tiles<n>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.taper, \
-                       fr.bin_start, fr.subs, g, t, op->tables_d, cg, ck, CCs, lg);                \
+  dim3 g3((fr.nsub + 3) / 4, op->nch);
+  size_t sh = sizeof(float) * g.ntab;
+  switch (g.CC) {
+#define MRN_S(n)                                                                                 \
+  case n:                                                                                        \
+    hipLaunchKernelGGL((k_scatter_gd<n>), g3, dim3(256), sh, st, grid, kin, fr.coords,           \
+                       fr.bin_start, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck);              \
     break;
-    MRN_S(1) MRN_S(2) MRN_S(3) MRN_S(4) MRN_S(5) MRN_S(6) MRN_S(7) MRN_S(8)
+    MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)
 #undef MRN_S
     default:
-      return fail(op, MRN_EUNSUPPORTED, "kernel width above 8 taps");
+      return fail(op, MRN_EUNSUPPORTED, "coil block size");
   }
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+int launch_weight(mrn_op *op, float2 *kout, const float2 *kin, const mrn::FrameData &fr,
+                  hipStream_t st) {
+  const int64_t total = (int64_t)op->nch * fr.Mv * op->g.CC;
+  if (total == 0) return MRN_OK;
+  int64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  hipLaunchKernelGGL(k_weight, dim3((unsigned)b), dim3(256), 0, st, kout, kin, fr.coords, fr.taper,
+                     fr.Mv, op->g.CC, op->g.lgCC, total);
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
 }
@@ -511,7 +513,7 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
                const float *kz, const float *kw, hipStream_t st) {
   const Geom &g = op->g;
   const Tiles &t = op->t;
-  const int nt = t.ntiles;
+  const int nt = t.ntiles, nbin = t.nbins;
   uint32_t *keys = nullptr, *keys2 = nullptr;
   int32_t *vals = nullptr, *counts = nullptr, *nsub = nullptr, *visits = nullptr, *off = nullptr,
           *hflag = nullptr, *hoff = nullptr;
@@ -561,30 +563,36 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   MRN_BH(hipMalloc(&vals, sizeof(int32_t) * Mm));
   MRN_BH(hipMalloc(&fr.perm, sizeof(int32_t) * Mm));
   MRN_BH(hipMalloc(&fr.inv, sizeof(int32_t) * Mm));
-  MRN_BH(hipMalloc(&counts, sizeof(int32_t) * (nt + 1)));
-  MRN_BH(hipMalloc(&fr.bin_start, sizeof(int32_t) * (nt + 1)));
-  MRN_BH(hipMemsetAsync(counts, 0, sizeof(int32_t) * (nt + 1), st));
+  MRN_BH(hipMalloc(&counts, sizeof(int32_t) * (nbin + 1)));
+  MRN_BH(hipMalloc(&fr.bin_start, sizeof(int32_t) * (nbin + 1)));
+  MRN_BH(hipMemsetAsync(counts, 0, sizeof(int32_t) * (nbin + 1), st));
   if (M > 0) {
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(M, 256)), dim3(256), 0, st, fr.inv, -1, M);
     hipLaunchKernelGGL(k_traj_keys, dim3(nblk(M, 256)), dim3(256), 0, st, kx, ky, kz, kw, M, g, t,
                        keys, vals, counts);
     MRN_BH(hipGetLastError());
     int bits = 1;
-    while ((1u << bits) <= (uint32_t)nt) bits++;
+    while ((1u << bits) <= (uint32_t)nbin) bits++;
     MRN_BH(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, vals, fr.perm, (int)M, 0, bits, st));
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, counts, fr.bin_start, nt + 1, st));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, counts, fr.bin_start, nbin + 1, st));
+    size_t b3 = 0;
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, counts, fr.bin_start, nt + 1, st));
+    b2 = b2 > b3 ? b2 : b3;
     tmp_bytes = tmp_bytes > b2 ? tmp_bytes : b2;
     MRN_BH(hipMalloc(&tmp, tmp_bytes));
     MRN_BH(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, vals, fr.perm, (int)M, 0, bits, st));
   }
   size_t tb = tmp_bytes;
   if (!tmp) {
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts, fr.bin_start, nt + 1, st));
+    size_t b3 = 0;
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts, fr.bin_start, nbin + 1, st));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, counts, fr.bin_start, nt + 1, st));
+    tb = tb > b3 ? tb : b3;
     MRN_BH(hipMalloc(&tmp, tb));
   }
-  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, fr.bin_start, nt + 1, st));
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, fr.bin_start, nbin + 1, st));
   int32_t Mv32 = 0;
-  MRN_BH(hipMemcpyAsync(&Mv32, fr.bin_start + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MRN_BH(hipMemcpyAsync(&Mv32, fr.bin_start + nbin, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MRN_BH(hipStreamSynchronize(st));
   fr.Mv = Mv32;
   MRN_BH(hipMalloc(&fr.coords, sizeof(float4) * (fr.Mv > 0 ? fr.Mv : 1)));

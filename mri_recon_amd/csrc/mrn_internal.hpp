@@ -21,6 +21,7 @@
 #define FFT_LINES 16         // 1-D lines per FFT workgroup
 #define FFT_THREADS 256
 #define FFT_PAD 1            // LDS line padding (float2) against bank conflicts
+#define MRN_TRACE_CAP 65536  // solver trace ring (iterations)
 
 namespace mrn {
 
@@ -41,14 +42,19 @@ struct Geom {
   float k_rad;
 };
 
-// Scatter tiles (owner-computes): tile t covers cells [t*T, t*T+T) per axis
-// and receives the samples whose first tap lies in tile t or t-1 per axis.
+// Sample bins and scatter tiles.  Samples are sorted by the B^3-cell bin of
+// their first tap; scatter tile t covers cells [t*T, t*T+T) per axis (T = 4:
+// one wavefront of cells) and reads the bins whose first taps lie in
+// [t*T - NT + 1, t*T + T - 1].
 struct Tiles {
-  int T[3];
-  int nt[3];
+  int B;        // bin edge (cells)
+  int nb[3];    // bins per axis
+  int nbins;
+  int T;        // tile edge (cells)
+  int nt[3];    // tiles per axis
   int ntiles;
-  int maxv;   // sample visits per scatter work item
-  int CCs;    // coils per scatter workgroup (LDS accumulator depth)
+  int NT;       // max taps per axis
+  int maxv;     // samples per scatter work item
 };
 
 struct FrameData {
@@ -95,6 +101,13 @@ struct mrn_op {
   double *dtrace = nullptr;    // per-iteration trace
   int dtrace_cap = 0;
   int *dcounter = nullptr;
+  // cached hipGraph of one CG step (valid while version == cg_version)
+  hipGraph_t cg_graph = nullptr;
+  hipGraphExec_t cg_exec = nullptr;
+  void *cg_X = nullptr;
+  int version = 0, cg_version = -1;
+  bool cg_ready = false;
+  int64_t cg_count = 0;
   int64_t nvox = 0;
   std::string err;
   void *scan_tmp = nullptr;

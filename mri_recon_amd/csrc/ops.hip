@@ -174,7 +174,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_alpha(const double *part, in
     f[F_AR] = (float)ar;
     f[F_AI] = (float)ai;
     if (trace) {
-      const int it = *counter;
+      const int it = (*counter) % MRN_TRACE_CAP;
       trace[4 * it + 0] = rr;
       trace[4 * it + 1] = ar;
       trace[4 * it + 2] = ai;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_beta(const double *part, int
     dscal[D_RN] = rn;
     float *f = (float *)(dscal + 8);
     f[F_BETA] = (float)(rn / dscal[D_RR]);
-    if (trace) trace[4 * (*counter) + 3] = rn;
+    if (trace) trace[4 * ((*counter) % MRN_TRACE_CAP) + 3] = rn;
     *counter += 1;
   }
 }
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_eig_finish(const double *part, 
     const double me = sqrt(e);
     dscal[D_EIG] = me;
     ((float *)(dscal + 8))[F_SCALE] = (float)(1. / me);
-    if (trace) trace[*counter] = me;
+    if (trace) trace[(*counter) % MRN_TRACE_CAP] = me;
     *counter += 1;
   }
 }
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_energy_finish(const double *par
   for (int b = threadIdx.x; b < nb; b += RED_THREADS) e += part[b];
   e = block_sum(e, sh);
   if (threadIdx.x == 0) {
-    if (trace) trace[*counter] = e;
+    if (trace) trace[(*counter) % MRN_TRACE_CAP] = e;
     *counter += 1;
   }
 }

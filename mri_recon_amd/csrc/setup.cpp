@@ -277,24 +277,21 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   while ((1 << g.lgCC) < g.CC) g.lgCC++;
   op->nch = (op->C + g.CC - 1) / g.CC;
   op->nvox = (int64_t)g.N[0] * g.N[1] * g.N[2];
-  // scatter tiles
+  // sample bins (B = 2 cells) and scatter tiles (T = 4 cells: one wavefront)
   int ntap = 1;
   for (int a = 0; a < 3; a++)
     if (g.grid_in[a]) ntap = std::max(ntap, (int)floorf(2 * g.dwin[a]) + 1);
+  op->t.NT = ntap;
+  op->t.B = 2;
+  op->t.T = 4;
   for (int a = 0; a < 3; a++) {
-    int T = p.tile[a] > 0 ? p.tile[a] : 8;
-    if (T < ntap - 1) T = ntap - 1;
-    op->t.T[a] = T;
-    op->t.nt[a] = (g.S[a] + T - 1) / T;
+    op->t.nb[a] = (g.S[a] + op->t.B - 1) / op->t.B;
+    op->t.nt[a] = (g.S[a] + op->t.T - 1) / op->t.T;
   }
+  op->t.nbins = op->t.nb[0] * op->t.nb[1] * op->t.nb[2];
   op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
-  op->t.maxv = p.max_visits > 0 ? p.max_visits : 2048;
-  op->t.CCs = g.CC < 8 ? g.CC : 8;
-  // dynamic LDS of the scatter (accumulator + tables) stays within 64 KiB
-  while (op->t.CCs > 1 && (int64_t)op->t.T[0] * op->t.T[1] * op->t.T[2] * op->t.CCs * 8 +
-                                  4 * (g.ntab + 4) > 64 * 1024)
-    op->t.CCs >>= 1;
-  if ((int64_t)op->t.T[0] * op->t.T[1] * op->t.T[2] * op->t.CCs * 8 + 4 * (g.ntab + 4) > 64 * 1024) {
+  op->t.maxv = p.max_visits > 0 ? p.max_visits : 4096;
+  if (4 * (g.ntab + 4) > 64 * 1024) {
     delete op;
     return MRN_EUNSUPPORTED;
   }
@@ -305,7 +302,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
         return MRN_EUNSUPPORTED;  // the plain form covers these geometries
       }
     }
-    if (op->t.ntiles > (1 << 30)) {
+    if (op->t.nbins > (1 << 30)) {
       delete op;
       return MRN_EUNSUPPORTED;
     }
@@ -380,6 +377,8 @@ int mrn_op_destroy(mrn_op *op) {
   hipFree(op->cgrid);
   hipFree(op->kwork);
   hipFree(op->kplain);
+  if (op->cg_exec) (void)hipGraphExecDestroy(op->cg_exec);
+  if (op->cg_graph) (void)hipGraphDestroy(op->cg_graph);
   hipFree(op->vR);
   hipFree(op->vP);
   hipFree(op->vL);

@@ -213,6 +213,27 @@ class GridFFT:
         tt = torch.tensor(list(tr), dtype=torch.float64).reshape(-1, 4)[:iters] if trace else None
         return X, tt
 
+    def cg_init(self, X=None):
+        """X = 0, R = P = E^H W d; returns X (F, Nz, Ny, Nx)."""
+        if X is None:
+            X = torch.empty((self.F,) + self.image_shape, dtype=torch.complex64, device="cuda")
+        self._c(lib().mrn_cg_init(self._h, _ptr(X), _stream()))
+        return X
+
+    def cg_iterate(self, X, iters, graph=True, trace=True):
+        """Continue CG for `iters` steps; returns the trace (iters, 4)."""
+        tr = (ctypes.c_double * (4 * max(iters, 1)))() if trace else None
+        self._c(lib().mrn_cg_iterate(self._h, _ptr(X), int(iters), int(bool(graph)), tr, _stream()))
+        return torch.tensor(list(tr), dtype=torch.float64).reshape(-1, 4)[:iters] if trace else None
+
+    def stage_times(self, X, reps=5, frame=0):
+        """Average device ms per batched stage (HIP events on the current stream):
+        pre_z, fft_y, fft_x, gather, scatter, ifft_x, ifft_y, post_z."""
+        ms = (ctypes.c_float * 8)()
+        self._c(lib().mrn_stage_times(self._h, frame, _ptr(X), int(reps), ms, _stream()))
+        names = ["pre_z", "fft_y", "fft_x", "gather", "scatter", "ifft_x", "ifft_y", "post_z"]
+        return dict(zip(names, list(ms)))
+
     def power_method(self, iters=30):
         step = ctypes.c_double()
         self._c(lib().mrn_power_method(self._h, int(iters), ctypes.byref(step), _stream()))

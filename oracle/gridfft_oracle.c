@@ -134,14 +134,15 @@ static void build_tables(orc_op *op) { /* gridFFT.cpp:276-473 */
       case K_KAISER: {
         real g = op->grid[a] * (real)op->p.grid_scale[a];
         real dw = op->dwin[a];
-        real beta = (real)(ORC_PI * (real)sqrt((dw * dw) / (g * g) * (g - 0.5) * (g - 0.5) - 0.8));
+        /* sqrtf of the argument rounded to `real`, as the reference calls sqrtf */
+        real beta = (real)(ORC_PI * (real)sqrt((real)((dw * dw) / (g * g) * (g - 0.5) * (g - 0.5) - 0.8)));
         real beta_min = (real)sqrt(pow(ORC_PI * 2 * dw / g, 2.0) - 9.6752);
         if (beta_min > beta) beta = beta_min;
         op->beta[a] = beta;
         real i0b = orc_bessi0(beta);
         for (int i = 0; i < L + 1; i++) {
           real pos = (real)i / (real)L;
-          real arg = (real)sqrt(1.0 - pos * pos);
+          real arg = (real)sqrt((real)(1.0 - pos * pos));
           f[i] = orc_bessi0(beta * arg) / i0b;
         }
       } break;
