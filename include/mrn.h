@@ -65,7 +65,8 @@ typedef struct mrn_params {
   int32_t coil_chunk;    /* coils per grid cell block (1,2,4,8,16); 0 = auto */
   int32_t tile[3];       /* scatter tile edge (x,y,z); 0 = auto */
   int32_t max_visits;    /* samples per scatter work item; 0 = auto */
-  int32_t reserved[7];
+  int32_t bin_edge;      /* sample sort bin edge in cells (1 or 2); 0 = auto */
+  int32_t reserved[6];
 } mrn_params;
 
 typedef struct mrn_op mrn_op;

@@ -44,7 +44,8 @@ class MrnParams(ctypes.Structure):
         ("coil_chunk", ctypes.c_int32),
         ("tile", ctypes.c_int32 * 3),
         ("max_visits", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("bin_edge", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

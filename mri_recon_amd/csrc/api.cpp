@@ -57,6 +57,7 @@ static PrePostArgs prepost(const mrn_op *op, int ch, int overwrite, int CC) {
     a.og[q] = op->g.og[q];
   }
   a.CC = CC;
+  a.lgCC = __builtin_ctz(CC);
   a.chunk_grid = ncell(op) * CC;
   a.chunk_smap = op->nvox * CC;
   a.ch = ch;

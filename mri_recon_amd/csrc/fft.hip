@@ -198,7 +198,62 @@ __device__ __forceinline__ void zero_outside(float2 *buf, int lo, int hi) {
   }
 }
 
-// Generic grid -> grid pass over FFT_LINES lines (see LinesArgs).
+// Move points [p0, p0+np) of the workgroup's 16 lines between the grid and LDS
+// (STORE = false: grid -> LDS, true: LDS -> grid, scaled).  Three layouts:
+//  - run >= 16: the 16 lines are contiguous at every point (y/z passes, or an
+//    x pass with 16 coils per cell): lane -> line, 128-byte rows;
+//  - run < 16, a power of two dividing 16 (x pass, CC < 16): 16/run rows, each
+//    a contiguous run*np block: shifts only;
+//  - anything else (tiny plain-form grids): generic index arithmetic.
+template <int S, bool STORE>
+__device__ __forceinline__ void move_lines(float2 *__restrict__ grid, float2 *buf, const LinesArgs &a,
+                                           int64_t base, int g0, int p0, int np) {
+  const int tid = threadIdx.x;
+  if (a.run >= FFT_LINES) {
+    const int l = tid & (FFT_LINES - 1);
+    const int gl = g0 + l;
+    if (gl >= a.nlines) return;
+    const int r = gl / a.run, cc = gl - r * a.run;
+    float2 *src = grid + base + (int64_t)r * a.rstride + cc;
+    for (int p = p0 + (tid >> 4); p < p0 + np; p += FFT_THREADS / FFT_LINES) {
+      if (STORE)
+        src[(int64_t)p * a.pstride] = scl(buf[l * (S + FFT_PAD) + p], a.scale);
+      else
+        buf[l * (S + FFT_PAD) + p] = src[(int64_t)p * a.pstride];
+    }
+  } else if ((a.run & (a.run - 1)) == 0 && a.pstride == a.run) {
+    const int W = a.run;
+    const int lgW = __builtin_ctz(W);
+    for (int rr = 0; rr < FFT_LINES / W; rr++) {
+      if (g0 + rr * W >= a.nlines) break;
+      float2 *row = grid + base + (int64_t)((g0 >> lgW) + rr) * a.rstride;
+      for (int idx = tid; idx < W * np; idx += FFT_THREADS) {
+        const int c = idx & (W - 1), p = p0 + (idx >> lgW);
+        const int l = rr * W + c;
+        if (STORE)
+          row[(int64_t)p * W + c] = scl(buf[l * (S + FFT_PAD) + p], a.scale);
+        else
+          buf[l * (S + FFT_PAD) + p] = row[(int64_t)p * W + c];
+      }
+    }
+  } else {
+    for (int idx = tid; idx < FFT_LINES * np; idx += FFT_THREADS) {
+      const int l = idx / np, p = p0 + idx % np;
+      const int gl = g0 + l;
+      if (gl >= a.nlines) continue;
+      const int r = gl / a.run, cc = gl - r * a.run;
+      float2 *e = grid + base + (int64_t)r * a.rstride + cc + (int64_t)p * a.pstride;
+      if (STORE)
+        *e = scl(buf[l * (S + FFT_PAD) + p], a.scale);
+      else
+        buf[l * (S + FFT_PAD) + p] = *e;
+    }
+  }
+}
+
+// Generic grid -> grid pass over FFT_LINES lines (see LinesArgs).  Lines past
+// nlines are never loaded (their LDS rows hold zeros or stale values, which
+// only feed their own, unstored, transforms).
 template <int S, int DIR>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(float2 *__restrict__ grid, LinesArgs a,
                                                            const float2 *__restrict__ tw) {
@@ -207,33 +262,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(float2 *__restrict__ 
   load_twiddles<S>(tws, tw);
   const int64_t base = a.off0 + (int64_t)blockIdx.y * a.ystride + (int64_t)blockIdx.z * a.zstride;
   const int g0 = blockIdx.x * FFT_LINES;
-  const int W = a.run < FFT_LINES ? a.run : FFT_LINES;
   zero_outside<S>(buf, a.ld0, a.ld1);
-  const int np = a.ld1 - a.ld0;
-  for (int idx = threadIdx.x; idx < FFT_LINES * np; idx += FFT_THREADS) {
-    const int c = idx % W, t = idx / W;
-    const int p = t % np, l = (t / np) * W + c;
-    const int gl = g0 + l;
-    float2 v = make_float2(0.f, 0.f);
-    if (gl < a.nlines) {
-      const int r = gl / a.run, cc = gl - r * a.run;
-      v = grid[base + (int64_t)r * a.rstride + cc + (int64_t)(a.ld0 + p) * a.pstride];
-    }
-    buf[l * (S + FFT_PAD) + a.ld0 + p] = v;
-  }
+  move_lines<S, false>(grid, buf, a, base, g0, a.ld0, a.ld1 - a.ld0);
   __syncthreads();
   fft_stages<S, DIR, 0, 1>(buf, tws);
-  const int ns = a.st1 - a.st0;
-  for (int idx = threadIdx.x; idx < FFT_LINES * ns; idx += FFT_THREADS) {
-    const int c = idx % W, t = idx / W;
-    const int p = t % ns, l = (t / ns) * W + c;
-    const int gl = g0 + l;
-    if (gl < a.nlines) {
-      const int r = gl / a.run, cc = gl - r * a.run;
-      grid[base + (int64_t)r * a.rstride + cc + (int64_t)(a.st0 + p) * a.pstride] =
-          scl(buf[l * (S + FFT_PAD) + a.st0 + p], a.scale);
-    }
-  }
+  move_lines<S, true>(grid, buf, a, base, g0, a.st0, a.st1 - a.st0);
 }
 
 // Fused set_image + forward z pass.  Workgroup: 16 lines (x, coil) of one
@@ -258,7 +291,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z(
     const int gl = g0 + l;
     float2 v = make_float2(0.f, 0.f);
     if (gl < nl) {
-      const int xi = gl / CC;
+      const int xi = gl >> a.lgCC;
       const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];
       const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
       const float2 xv = X[r + xi];
@@ -305,7 +338,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   const float wyv = wy[a.og[1] + y];
   const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
   for (int pr = threadIdx.x; pr < nxl * a.N[2]; pr += FFT_THREADS) {
-    const int xl = pr % nxl, zi = pr / nxl;
+    const int lgx = 4 - a.lgCC;
+    const int xl = pr & (nxl - 1), zi = pr >> lgx;
     const int xi = blockIdx.x * nxl + xl;
     if (xi >= a.N[0]) continue;
     const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];

@@ -148,22 +148,32 @@ int autotune_impl(mrn_op *op, int f, hipStream_t st) {
   MRN_HIP(op, hipEventCreate(&e0));
   MRN_HIP(op, hipEventCreate(&e1));
   float best = 1e30f;
-  int bestv = keep;
-  for (int maxv : {1024, 2048, 4096, 8192, 16384}) {
-    op->t.maxv = maxv;
-    MRN_TRY(build_bins(op, fr, fr.M, fr.traw, fr.traw + fr.M, fr.traw + 2 * fr.M, fr.traw + 3 * fr.M, st));
-    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));  // warm-up
-    MRN_HIP(op, hipEventRecord(e0, st));
-    for (int r = 0; r < 3; r++) MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-    MRN_HIP(op, hipEventRecord(e1, st));
-    MRN_HIP(op, hipEventSynchronize(e1));
-    float ms = 0.f;
-    MRN_HIP(op, hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) {
-      best = ms;
-      bestv = maxv;
+  int bestv = keep, bestB = op->t.B;
+  auto set_bins = [&](int B) {
+    op->t.B = B;
+    for (int a = 0; a < 3; a++) op->t.nb[a] = (op->g.S[a] + B - 1) / B;
+    op->t.nbins = op->t.nb[0] * op->t.nb[1] * op->t.nb[2];
+  };
+  for (int B : {1, 2})
+    for (int maxv : {1024, 4096, 16384}) {
+        set_bins(B);
+        op->t.maxv = maxv;
+        MRN_TRY(build_bins(op, fr, fr.M, fr.traw, fr.traw + fr.M, fr.traw + 2 * fr.M, fr.traw + 3 * fr.M, st));
+        MRN_TRY(op_fill(op, op->kwork, fr.Mv * op->g.CC * op->nch, make_float2(1.f, 0.f), st));
+        MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));  // warm-up
+        MRN_HIP(op, hipEventRecord(e0, st));
+        for (int r = 0; r < 3; r++) MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
+        MRN_HIP(op, hipEventRecord(e1, st));
+        MRN_HIP(op, hipEventSynchronize(e1));
+        float ms = 0.f;
+        MRN_HIP(op, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          bestv = maxv;
+          bestB = B;
+        }
     }
-  }
+  set_bins(bestB);
   op->t.maxv = bestv;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -6,16 +6,16 @@
 // wrap), table-lerp weights with the (-1)^l fftshift sign, kw/taper on the
 // scatter side only, kw == 0 and out-of-grid samples skipped.
 //
-// Batched form: samples are sorted once per trajectory by the 2x2x2-cell bin
+// Batched form: samples are sorted once per trajectory by the bin (B^3 cells)
 // of their first tap.  The gather maps one lane to one (sample, coil) pair so
 // that a tap reads CC contiguous coils (CC*8 bytes) of a grid cell.  The
 // scatter is grid-driven: one wavefront owns a 4x4x4 tile of cells, one lane
 // per cell with the CC coil sums in registers; the wavefront walks the samples
 // of the bins that can reach the tile (wave-uniform loads) and every lane adds
-// its own tap.  No LDS or global atomics on this path (LDS float atomics run
-// at about 3 cycles per lane on gfx950); a tile is written once with plain
-// stores.  Tiles whose sample list exceeds one work item are split; only those
-// are pre-zeroed and flushed with global atomics.
+// its own tap.  No atomics on this path (LDS float atomics run at about 3 cycles
+// per lane on gfx950); a tile is written once with plain stores.  Tiles whose
+// sample list exceeds one work item are split; only those are pre-zeroed and
+// flushed with global atomics.
 #include <hipcub/hipcub.hpp>
 
 #include "mrn_internal.hpp"
@@ -102,11 +102,10 @@ __global__ void k_tile_nsub(const int32_t *__restrict__ bin_start, Tiles t, int3
   tile_bins(t, 2, tz, bz0, bz1);
   int v = 0;
   for (int bz = bz0; bz <= bz1; bz++)
-    for (int by = by0; by <= by1; by++)
-      for (int bx = bx0; bx <= bx1; bx++) {
-        const int b = (bz * t.nb[1] + by) * t.nb[0] + bx;
-        v += bin_start[b + 1] - bin_start[b];
-      }
+    for (int by = by0; by <= by1; by++) {
+      const int b = (bz * t.nb[1] + by) * t.nb[0];
+      v += bin_start[b + bx1 + 1] - bin_start[b + bx0];  // bins of a row are contiguous
+    }
   visits[tile] = v;
   nsub[tile] = v > t.maxv ? (v + t.maxv - 1) / t.maxv : 1;
 }
@@ -234,6 +233,8 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 // subs[4*blockIdx.x + w] = {tile, first visit, end visit, heavy}; lane l owns
 // cell (l&3, (l>>2)&3, l>>4) of the 4x4x4 tile and the CCT coil sums of coil
 // block blockIdx.y.  kin holds kw * taper * data (internal sample order).
+// Sample coordinates and data are wave-uniform loads; every lane adds its own
+// tap.
 template <int CCT>
 __global__ __launch_bounds__(256) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
@@ -262,35 +263,35 @@ __global__ __launch_bounds__(256) void k_scatter_gd(
   tile_bins(t, 2, tiz, bz0, bz1);
   int v = 0;
   for (int bz = bz0; bz <= bz1; bz++)
-    for (int by = by0; by <= by1; by++)
-      for (int bx = bx0; bx <= bx1; bx++) {
-        const int b = (bz * t.nb[1] + by) * t.nb[0] + bx;
-        const int s0 = bin_start[b], n = bin_start[b + 1] - s0;
-        const int j0 = max(sp.y - v, 0), j1 = min(sp.z - v, n);
-        v += n;
-        for (int j = j0; j < j1; j++) {
-          const int64_t i = s0 + j;
-          const float4 k = coords[i];
-          int sx, ex, sy, ey, sz, ez;
-          axis_range(g, 0, k.x, sx, ex);
-          axis_range(g, 1, k.y, sy, ey);
-          axis_range(g, 2, k.z, sz, ez);
-          if (ex < t0x || sx > t0x + 3 || ey < t0y || sy > t0y + 3 || ez < t0z || sz > t0z + 3) continue;
-          float2 d[CCT];
+    for (int by = by0; by <= by1; by++) {
+      // the bins bx0..bx1 of a row hold one contiguous run of samples
+      const int brow = (bz * t.nb[1] + by) * t.nb[0];
+      const int s0 = bin_start[brow + bx0], n = bin_start[brow + bx1 + 1] - s0;
+      const int j0 = max(sp.y - v, 0), j1 = min(sp.z - v, n);
+      v += n;
+      for (int j = j0; j < j1; j++) {
+        const int64_t i = (int64_t)s0 + j;
+        const float4 k = coords[i];
+        int sx, ex, sy, ey, sz, ez;
+        axis_range(g, 0, k.x, sx, ex);
+        axis_range(g, 1, k.y, sy, ey);
+        axis_range(g, 2, k.z, sz, ez);
+        if (ex < t0x || sx > t0x + 3 || ey < t0y || sy > t0y + 3 || ez < t0z || sz > t0z + 3) continue;
+        float2 d[CCT];
 #pragma unroll
-          for (int c = 0; c < CCT; c++) d[c] = kch[i * CCT + c];
-          if (lx >= sx && lx <= ex && ly >= sy && ly <= ey && lz >= sz && lz <= ez) {
-            const float w = (tap_weight(g, 2, tab_s + g.toff[2], k.z, lz) *
-                             tap_weight(g, 1, tab_s + g.toff[1], k.y, ly)) *
-                            tap_weight(g, 0, tab_s + g.toff[0], k.x, lx);
+        for (int c = 0; c < CCT; c++) d[c] = kch[i * CCT + c];
+        if (lx >= sx && lx <= ex && ly >= sy && ly <= ey && lz >= sz && lz <= ez) {
+          const float w = (tap_weight(g, 2, tab_s + g.toff[2], k.z, lz) *
+                           tap_weight(g, 1, tab_s + g.toff[1], k.y, ly)) *
+                          tap_weight(g, 0, tab_s + g.toff[0], k.x, lx);
 #pragma unroll
-            for (int c = 0; c < CCT; c++) {
-              ar[c] += w * d[c].x;
-              ai[c] += w * d[c].y;
-            }
+          for (int c = 0; c < CCT; c++) {
+            ar[c] += w * d[c].x;
+            ai[c] += w * d[c].y;
           }
         }
       }
+    }
   if (lx >= g.S[0] || ly >= g.S[1] || lz >= g.S[2]) return;
   float2 *dst = grid + ch * chunk_grid + (((int64_t)lz * g.S[1] + ly) * g.S[0] + lx) * CCT;
   if (sp.w) {
@@ -427,14 +428,13 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
   const Geom &g = op->g;
   if (fr.Mv == 0) return MRN_OK;
   dim3 g3(nblk(fr.Mv * g.CC, 256), op->nch);
-  size_t sh = sizeof(float) * g.ntab;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   const int wflag = weighted ? 1 : 0;
   switch (taps_main(g)) {
-#define MRN_G(n)                                                                                  \
-  case n:                                                                                         \
-    hipLaunchKernelGGL((k_gather<n>), g3, dim3(256), sh, st, grid, out, fr.coords, sub, fr.Mv, g, \
-                       op->tables_d, cg, ck, fr.taper, wflag);                                    \
+#define MRN_G(n)                                                                                    \
+  case n:                                                                                           \
+    hipLaunchKernelGGL((k_gather<n>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,        \
+                       fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag);            \
     break;
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
 #undef MRN_G
@@ -455,7 +455,7 @@ int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::Frame
     MRN_HIP(op, hipGetLastError());
   }
   dim3 g3((fr.nsub + 3) / 4, op->nch);
-  size_t sh = sizeof(float) * g.ntab;
+  const size_t sh = sizeof(float) * g.ntab;
   switch (g.CC) {
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
@@ -597,9 +597,10 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   fr.Mv = Mv32;
   MRN_BH(hipMalloc(&fr.coords, sizeof(float4) * (fr.Mv > 0 ? fr.Mv : 1)));
   MRN_BH(hipMalloc(&fr.taper, sizeof(float) * (fr.Mv > 0 ? fr.Mv : 1)));
-  if (fr.Mv > 0)
+  if (fr.Mv > 0) {
     hipLaunchKernelGGL(k_traj_finish, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, kx, ky, kz, kw,
                        fr.perm, fr.Mv, g, fr.coords, fr.taper, fr.inv);
+  }
   MRN_BH(hipMalloc(&nsub, sizeof(int32_t) * (nt + 1)));
   MRN_BH(hipMalloc(&visits, sizeof(int32_t) * (nt + 1)));
   MRN_BH(hipMalloc(&off, sizeof(int32_t) * (nt + 1)));

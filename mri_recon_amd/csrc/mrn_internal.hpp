@@ -204,7 +204,7 @@ struct LinesArgs {
 
 struct PrePostArgs {
   int N[3], S[3], og[3];
-  int CC;
+  int CC, lgCC;
   int64_t chunk_grid;   // grid elements per coil block
   int64_t chunk_smap;   // smap elements per coil block
   int ch;               // coil block (post: one launch per block)

@@ -277,12 +277,17 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   while ((1 << g.lgCC) < g.CC) g.lgCC++;
   op->nch = (op->C + g.CC - 1) / g.CC;
   op->nvox = (int64_t)g.N[0] * g.N[1] * g.N[2];
-  // sample bins (B = 2 cells) and scatter tiles (T = 4 cells: one wavefront)
+  // sample bins (B = 1: one cell per bin, so a tile's bin neighbourhood is
+  // exact) and scatter tiles (T = 4 cells: one wavefront)
   int ntap = 1;
   for (int a = 0; a < 3; a++)
     if (g.grid_in[a]) ntap = std::max(ntap, (int)floorf(2 * g.dwin[a]) + 1);
   op->t.NT = ntap;
-  op->t.B = 2;
+  if (p.bin_edge != 0 && p.bin_edge != 1 && p.bin_edge != 2) {
+    delete op;
+    return MRN_EINVAL;
+  }
+  op->t.B = p.bin_edge > 0 ? p.bin_edge : 2;
   op->t.T = 4;
   for (int a = 0; a < 3; a++) {
     op->t.nb[a] = (g.S[a] + op->t.B - 1) / op->t.B;
