@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 METRIC = ("batched-graph vs plain SENSE-CG iter/s, 256³×32ch; gridding GB/s; "
           "rel err≤1e-4")
 HBM_PEAK_GBPS = 8000.0
+# device kernel behind each stage of GridFFT.stage_times (as rocprofv3 names it)
+KERNEL_NAMES = {
+    "pre_z": "k_fft_pre_z", "fft_y": "k_fft_lines (y)", "fft_x": "k_fft_lines (x)",
+    "gather": "k_gather", "scatter_zero": "k_zero_tiles", "scatter": "k_scatter_gd",
+    "ifft_x": "k_fft_lines (x, inverse)", "ifft_y": "k_fft_lines (y, inverse)", "post_z": "k_fft_post_z",
+}
 
 
 def log(*a):
@@ -96,8 +102,17 @@ def main():
     value = a.steps / (t1 - t0)
     log("batched+graph CG: %.2f ms/iter, %.2f iter/s" % (ms_step, value))
     if a.profile:
+        # a plain 1 GiB device copy: the HBM counters of this kernel (FETCH_SIZE,
+        # WRITE_SIZE) calibrate those of the operator's kernels
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        st = op.stage_times(xtrue, reps=3)
+        log("stages ms: " + ", ".join("%s %.2f" % (k, v) for k, v in st.items()))
         print(json.dumps({"metric": METRIC, "value": value, "unit": "iter/s", "ms_per_step": ms_step,
-                          "profile": True}))
+                          "profile": True, "stage_ms": st, "tuning": op.tuning()}))
         return
 
     res = {}
@@ -120,23 +135,27 @@ def main():
         "fft_y": 8 * N * N * S * CC + 8 * N * S * S * CC,
         "fft_x": 8 * N * S * S * CC + 8 * S**3 * CC,
         "gather": 8 * S**3 * CC + 8 * M * CC + 16 * M,
+        "scatter_zero": 0,  # tiles shared by several work items; its bytes count in the scatter's
         "scatter": 8 * M * CC + 16 * M + 8 * S**3 * CC,
         "ifft_x": 8 * S**3 * CC + 8 * N * S * S * CC,
         "ifft_y": 8 * N * S * S * CC + 8 * N * N * S * CC,
         "post_z": 8 * N * N * S * CC + 8 * N**3 * CC + 16 * N**3,
     }
-    stages = {k: {"ms": st[k], "GBps": b[k] / (st[k] * 1e-3) / 1e9} for k in st}
+    stages = {k: {"ms": st[k], "GBps": b[k] / (st[k] * 1e-3) / 1e9 if b[k] else None} for k in st}
     dom = max(st, key=lambda k: st[k])
     fft_ms = sum(st[k] for k in ("pre_z", "fft_y", "fft_x"))
     ifft_ms = sum(st[k] for k in ("ifft_x", "ifft_y", "post_z"))
+    scat_ms = st["scatter_zero"] + st["scatter"]
     res["gridding"] = {
-        "gather_GBps": stages["gather"]["GBps"], "scatter_GBps": stages["scatter"]["GBps"],
+        "gather_GBps": stages["gather"]["GBps"],
+        "scatter_GBps": b["scatter"] / (scat_ms * 1e-3) / 1e9,
         "fft_GBps": (2 * 16 * S**3 * CC) / ((fft_ms + ifft_ms) * 1e-3) / 1e9,
         "gather_taps_per_s": M * C * (2 * DW) ** 3 / (st["gather"] * 1e-3),
-        "scatter_taps_per_s": M * C * (2 * DW) ** 3 / (st["scatter"] * 1e-3),
+        "scatter_taps_per_s": M * C * (2 * DW) ** 3 / (scat_ms * 1e-3),
         "stages": stages,
         "bytes_model": "per launch over all coils: gather/scatter 8*S^3*C + 8*M*C + 16*M; "
                        "FFT 16*S^3*C per 3-D transform; passes = compulsory reads+writes",
+        "tuning": op.tuning(),
     }
     log("stages ms: " + ", ".join("%s %.2f" % (k, v) for k, v in st.items()))
 
@@ -225,7 +244,8 @@ def main():
         "config": {"workload": workload, "N": N, "coils": C, "samples_per_coil": M, "grid": list(op.S),
                    "overgrid": OS, "kb_half_width": DW, "form": "batched, hipGraph-captured",
                    "parallelism": "single GPU"},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+        "roofline": {"kernel": KERNEL_NAMES[dom], "stage": dom, "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "algorithmic_bytes": dom_b, "avg_ms": st[dom]},
         "cpu_baseline": cpu,

@@ -66,7 +66,9 @@ typedef struct mrn_params {
   int32_t tile[3];       /* scatter tile edge (x,y,z); 0 = auto */
   int32_t max_visits;    /* samples per scatter work item; 0 = auto */
   int32_t bin_edge;      /* sample sort bin edge in cells (1 or 2); 0 = auto */
-  int32_t reserved[6];
+  int32_t xcd_run;       /* scatter work items per XCD run (1 = hardware order); 0 = auto */
+  int32_t xcd_run_gather;/* gather workgroups per XCD run; 0 = auto */
+  int32_t reserved[4];
 } mrn_params;
 
 typedef struct mrn_op mrn_op;
@@ -163,12 +165,22 @@ int mrn_traj_grad(mrn_op *op, int frame, const void *X, const void *g, float *gr
  * the reference. */
 int mrn_autotune(mrn_op *op, int frame, void *stream);
 
+/* The scatter launch shape in use (after mrn_autotune, the tuned one): the
+ * bin edge in cells and the samples per work item.  No counterpart in the
+ * reference. */
+int mrn_get_tuning(const mrn_op *op, int32_t *bin_edge, int32_t *max_visits);
+
 /* Average device time (ms, HIP events on `stream`) of each batched stage of
  * one frame over `reps` runs: [0] set_image+z FFT, [1] y FFT, [2] x FFT,
- * [3] gather, [4] scatter, [5] x IFFT, [6] y IFFT, [7] z IFFT+accumulate.
- * Diagnostic; no counterpart in the reference (its -time_grid prints,
- * gridFFT.cpp:727-747, time the same stages on the host). */
-int mrn_stage_times(mrn_op *op, int frame, const void *X, int reps, float ms[8], void *stream);
+ * [3] gather (with the k-space weighting of E'E), [4] zeroing of the tiles
+ * that several scatter work items share, [5] tile scatter, [6] x IFFT,
+ * [7] y IFFT, [8] z IFFT+accumulate.  Each stage is one kernel per coil block
+ * or one kernel over all blocks.  Diagnostic; no counterpart in the reference
+ * (its -time_grid prints, gridFFT.cpp:727-747, time the same stages on the
+ * host). */
+#define MRN_NSTAGES 9
+int mrn_stage_times(mrn_op *op, int frame, const void *X, int reps, float ms[MRN_NSTAGES],
+                    void *stream);
 
 /* Wait for the work queued on `stream` (NULL = the null stream). */
 int mrn_sync(mrn_op *op, void *stream);

@@ -2,10 +2,11 @@
 
 The compute path is libmrn.so (hand-written HIP kernels for gfx950, C ABI in
 include/mrn.h).  ``GridFFT`` mirrors the reference's gridFFT call surface for
-torch tensors; ``NufftSense`` (autograd) differentiates it with respect to the
-image and the trajectory.
+torch tensors; ``SenseEncoding`` (autograd) differentiates it with respect to
+the image and the trajectory.
 """
 from ._ffi import MrnError, lib  # noqa: F401
 from .gridfft import GridFFT  # noqa: F401
+from .autograd import SenseEncoding  # noqa: F401
 
-__all__ = ["GridFFT", "MrnError", "lib"]
+__all__ = ["GridFFT", "MrnError", "SenseEncoding", "lib"]

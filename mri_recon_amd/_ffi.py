@@ -21,7 +21,7 @@ EXPORTS = [
     "mrn_get_geometry", "mrn_get_table", "mrn_get_window", "mrn_set_traj", "mrn_set_smaps",
     "mrn_image_to_kspace", "mrn_kspace_to_image", "mrn_normal", "mrn_coil_backward",
     "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_cg_init", "mrn_cg_iterate",
-    "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_stage_times",
+    "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
 
@@ -45,7 +45,9 @@ class MrnParams(ctypes.Structure):
         ("tile", ctypes.c_int32 * 3),
         ("max_visits", ctypes.c_int32),
         ("bin_edge", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("xcd_run", ctypes.c_int32),
+        ("xcd_run_gather", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -90,6 +92,7 @@ def lib():
             "mrn_fista": (i32, [vp, vp, i32, dbl, i32, vp, vp]),
             "mrn_traj_grad": (i32, [vp, i32, vp, vp, vp, vp]),
             "mrn_autotune": (i32, [vp, i32, vp]),
+            "mrn_get_tuning": (i32, [vp, vp, vp]),
             "mrn_sync": (i32, [vp, vp]),
             "mrn_status_string": (ctypes.c_char_p, [i32]),
             "mrn_last_error": (ctypes.c_char_p, [vp]),

@@ -20,6 +20,8 @@ namespace mrn {
 int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, bool, hipStream_t);
 int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
+int launch_scatter_zero(mrn_op *, float2 *, const FrameData &, hipStream_t);
+int launch_scatter_tiles(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_gather_plain(mrn_op *, const float2 *, float2 *, int64_t, const float *, const float *,
                         const float *, const float *, const float2 *, hipStream_t);
 int launch_scatter_plain(mrn_op *, float2 *, const float2 *, int64_t, const float *, const float *,
@@ -642,7 +644,14 @@ int mrn_autotune(mrn_op *op, int f, void *stream) {
   return autotune_impl(op, f, pick(op, stream));
 }
 
-int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[8], void *stream) {
+int mrn_get_tuning(const mrn_op *op, int32_t *bin_edge, int32_t *max_visits) {
+  if (!op || !bin_edge || !max_visits) return MRN_EINVAL;
+  *bin_edge = op->t.B;
+  *max_visits = op->t.maxv;
+  return MRN_OK;
+}
+
+int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[MRN_NSTAGES], void *stream) {
   if (!op || !Xv || !ms || reps <= 0) return MRN_EINVAL;
   MRN_TRY(check_frame(op, f, false));
   if (!batched(op)) return fail(op, MRN_EUNSUPPORTED, "stage times cover the batched form");
@@ -650,32 +659,35 @@ int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[8], vo
   FrameData &fr = op->frames[f];
   const float2 *X = (const float2 *)Xv;
   MRN_TRY(ensure_solver(op));
-  hipEvent_t ev[9];
-  for (int q = 0; q < 9; q++) MRN_HIP(op, hipEventCreate(&ev[q]));
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[MRN_NSTAGES + 1];
+  for (int q = 0; q <= MRN_NSTAGES; q++) MRN_HIP(op, hipEventCreate(&ev[q]));
+  double acc[MRN_NSTAGES] = {0};
   for (int r = 0; r < reps; r++) {
-    MRN_HIP(op, hipEventRecord(ev[0], st));
+    int e = 0;
+    MRN_HIP(op, hipEventRecord(ev[e++], st));
     for (int q = 0; q < 3; q++) {
       MRN_TRY(fwd_stage(op, q, X, st));
-      MRN_HIP(op, hipEventRecord(ev[q + 1], st));
+      MRN_HIP(op, hipEventRecord(ev[e++], st));
     }
     MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, true, st));
-    MRN_HIP(op, hipEventRecord(ev[4], st));
-    MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-    MRN_HIP(op, hipEventRecord(ev[5], st));
+    MRN_HIP(op, hipEventRecord(ev[e++], st));
+    MRN_TRY(launch_scatter_zero(op, op->grid, fr, st));
+    MRN_HIP(op, hipEventRecord(ev[e++], st));
+    MRN_TRY(launch_scatter_tiles(op, op->grid, op->kwork, fr, st));
+    MRN_HIP(op, hipEventRecord(ev[e++], st));
     for (int q = 0; q < 3; q++) {
       MRN_TRY(adj_stage(op, q, op->vX0, true, st));
-      MRN_HIP(op, hipEventRecord(ev[q + 6], st));
+      MRN_HIP(op, hipEventRecord(ev[e++], st));
     }
-    MRN_HIP(op, hipEventSynchronize(ev[8]));
-    for (int q = 0; q < 8; q++) {
+    MRN_HIP(op, hipEventSynchronize(ev[MRN_NSTAGES]));
+    for (int q = 0; q < MRN_NSTAGES; q++) {
       float t = 0.f;
       MRN_HIP(op, hipEventElapsedTime(&t, ev[q], ev[q + 1]));
       acc[q] += t;
     }
   }
-  for (int q = 0; q < 9; q++) (void)hipEventDestroy(ev[q]);
-  for (int q = 0; q < 8; q++) ms[q] = (float)(acc[q] / reps);
+  for (int q = 0; q <= MRN_NSTAGES; q++) (void)hipEventDestroy(ev[q]);
+  for (int q = 0; q < MRN_NSTAGES; q++) ms[q] = (float)(acc[q] / reps);
   return MRN_OK;
 }
 

@@ -39,7 +39,10 @@ __device__ __forceinline__ float i1_over_z(float z) {
   return i1 / z;
 }
 
-// d w(dk - l) / d dk, with the same fftshift sign as tap_weight
+// d w(dk - l) / d dk, with the same fftshift sign as tap_weight.  A tap is
+// used while |dk - l| <= dwin and the kernel drops to zero beyond, so at
+// |dk - l| = dwin exactly the derivative is the one from inside the support
+// (the limit along which the tap stays in the sum).
 __device__ __forceinline__ float tap_dweight(const Geom &g, int a, const float *tab, float dk, int l) {
   if (a == 2 && !g.grid_in[2]) return 0.0f;
   const float u = dk - (float)l;
@@ -48,14 +51,16 @@ __device__ __forceinline__ float tap_dweight(const Geom &g, int a, const float *
   if (g.kmode == MRN_KMODE_EXACT_KB && g.grid_in[a]) {
     const float aw = g.dwin[a];
     const float x = fabsf(u);
-    if (x >= aw) return 0.0f;
+    if (x > aw) return 0.0f;
     const float s = sqrtf(fmaxf(1.0f - (x / aw) * (x / aw), 0.0f));
     const float z = g.beta[a] * s;
     // dK/dx = knorm/I0(beta) * I1(beta s) * beta * ds/dx, ds/dx = -x/(aw^2 s)
     d = -g.knorm[a] / g.i0beta[a] * g.beta[a] * g.beta[a] * i1_over_z(z) * x / (aw * aw);
   } else {
+    // slope of the table segment [i0, i0 + 1]; the last segment inside the
+    // support is [L - 1, L] (the table holds L + 10 entries, zero beyond L)
     const float del = fabsf(600.0f * u);
-    const int i0 = (int)del;
+    const int i0 = min((int)del, g.tlen[a] - 11);
     const float f0 = (i0 < g.tlen[a]) ? tab[i0] : 0.0f;
     const float f1 = (i0 + 1 < g.tlen[a]) ? tab[i0 + 1] : 0.0f;
     d = 600.0f * (f1 - f0);

@@ -27,6 +27,21 @@ __device__ __forceinline__ float grid_coord(const Geom &g, int a, float k) {
   return __fadd_rn(__fmul_rn(k, g.grid[a]), (float)(g.S[a] / 2));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumber them so that each XCD runs runs of `run` consecutive work items:
+// neighbouring tiles and samples, which read the same grid cells and k-space
+// samples, then share one L2, while the runs still interleave over the XCDs
+// so that dense and sparse regions of k-space spread evenly.  run = 1 keeps
+// the hardware order.  A bijection of [0, nb): full groups of 8*run blocks
+// are permuted, the tail keeps its order.
+#define MRN_XCDS 8
+__device__ __forceinline__ int xcd_block(int b, int nb, int run) {
+  const int G = MRN_XCDS * run;
+  if (run <= 1 || b >= (nb / G) * G) return b;
+  const int w = b % G;
+  return (b - w) + (w % MRN_XCDS) * run + w / MRN_XCDS;
+}
+
 __device__ __forceinline__ void load_tables(float *tab_s, const float *tables, int ntab) {
   for (int q = threadIdx.x; q < ntab; q += blockDim.x) tab_s[q] = tables[q];
 }
@@ -137,11 +152,12 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
                                                 const float2 *__restrict__ sub, int64_t Mv, Geom g,
                                                 const float *__restrict__ tables,
                                                 int64_t chunk_grid, int64_t chunk_k,
-                                                const float *__restrict__ taper, int weighted) {
+                                                const float *__restrict__ taper, int weighted,
+                                                int xrun) {
   extern __shared__ float tab_s[];
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gid = (int64_t)xcd_block(blockIdx.x, gridDim.x, xrun) * blockDim.x + threadIdx.x;
   const int c = (int)(gid & (g.CC - 1));
   const int64_t i = gid >> g.lgCC;
   if (i >= Mv) return;
@@ -230,7 +246,8 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 }
 
 // Grid-driven scatter.  Wavefront w of the workgroup takes work item
-// subs[4*blockIdx.x + w] = {tile, first visit, end visit, heavy}; lane l owns
+// subs[4*b + w] = {tile, first visit, end visit, heavy} (b = the workgroup's
+// XCD-ordered index); lane l owns
 // cell (l&3, (l>>2)&3, l>>4) of the 4x4x4 tile and the CCT coil sums of coil
 // block blockIdx.y.  kin holds kw * taper * data (internal sample order).
 // Sample coordinates and data are wave-uniform loads; every lane adds its own
@@ -245,7 +262,7 @@ __global__ __launch_bounds__(256) void k_scatter_gd(
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int sidx = blockIdx.x * 4 + wave;
+  const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
   if (sidx >= nsub) return;
   const int4 sp = subs[sidx];
   const int tile = sp.x;
@@ -434,7 +451,8 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
 #define MRN_G(n)                                                                                    \
   case n:                                                                                           \
     hipLaunchKernelGGL((k_gather<n>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,        \
-                       fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag);            \
+                       fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag,            \
+                       op->t.xrun_gather);                                                          \
     break;
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
 #undef MRN_G
@@ -445,15 +463,24 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
   return MRN_OK;
 }
 
-int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::FrameData &fr,
-                   hipStream_t st) {
+// Zero the tiles that several work items flush with atomics (part of the scatter).
+int launch_scatter_zero(mrn_op *op, float2 *grid, const mrn::FrameData &fr, hipStream_t st) {
+  const Geom &g = op->g;
+  if (fr.nheavy > 0) {
+    const int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC;
+    hipLaunchKernelGGL(k_zero_tiles, dim3(fr.nheavy, op->nch), dim3(256), 0, st, grid, fr.heavy, g,
+                       op->t, cg);
+    MRN_HIP(op, hipGetLastError());
+  }
+  return MRN_OK;
+}
+
+// The tile scatter proper; launch_scatter_zero must precede it.
+int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn::FrameData &fr,
+                         hipStream_t st) {
   const Geom &g = op->g;
   const Tiles &t = op->t;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
-  if (fr.nheavy > 0) {
-    hipLaunchKernelGGL(k_zero_tiles, dim3(fr.nheavy, op->nch), dim3(256), 0, st, grid, fr.heavy, g, t, cg);
-    MRN_HIP(op, hipGetLastError());
-  }
   dim3 g3((fr.nsub + 3) / 4, op->nch);
   const size_t sh = sizeof(float) * g.ntab;
   switch (g.CC) {
@@ -469,6 +496,12 @@ int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::Frame
   }
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
+}
+
+int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::FrameData &fr,
+                   hipStream_t st) {
+  MRN_TRY(launch_scatter_zero(op, grid, fr, st));
+  return launch_scatter_tiles(op, grid, kin, fr, st);
 }
 
 int launch_weight(mrn_op *op, float2 *kout, const float2 *kin, const mrn::FrameData &fr,

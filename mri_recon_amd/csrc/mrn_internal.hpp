@@ -55,6 +55,8 @@ struct Tiles {
   int ntiles;
   int NT;       // max taps per axis
   int maxv;     // samples per scatter work item
+  int xrun;     // scatter work items per XCD run (1 = hardware order)
+  int xrun_gather;  // gather workgroups per XCD run
 };
 
 struct FrameData {

@@ -296,6 +296,8 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->t.nbins = op->t.nb[0] * op->t.nb[1] * op->t.nb[2];
   op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
   op->t.maxv = p.max_visits > 0 ? p.max_visits : 4096;
+  op->t.xrun = p.xcd_run > 0 ? p.xcd_run : 1;
+  op->t.xrun_gather = p.xcd_run_gather > 0 ? p.xcd_run_gather : 64;
   if (4 * (g.ntab + 4) > 64 * 1024) {
     delete op;
     return MRN_EUNSUPPORTED;

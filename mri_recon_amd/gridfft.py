@@ -21,7 +21,12 @@ from ._ffi import MrnParams, check, lib
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    if t is None:
+        return None
+    if t.is_conj() or t.is_neg():
+        # a lazily conjugated/negated view: its memory does not hold its values
+        raise ValueError("pass a tensor with resolved conjugation (t.resolve_conj().resolve_neg())")
+    return ctypes.c_void_p(t.data_ptr())
 
 
 def _stream():
@@ -53,7 +58,7 @@ class GridFFT:
     def __init__(self, N, ncoils=1, nframes=1, overgrid=1.5, dwin=-1.0, kernel="kaiser",
                  grid_in=(1, 1, 1), fft_in=(1, 1, 1), grid=-1.0, grid_scale=1.0, k_rad=9999.0,
                  kernel_mode="table", form="batched", coil_chunk=0, tile=0, max_visits=0,
-                 bin_edge=0, args=None):
+                 bin_edge=0, xcd_run=0, xcd_run_gather=0, args=None):
         L = lib()
         p = MrnParams()
         L.mrn_params_default(ctypes.byref(p))
@@ -76,6 +81,8 @@ class GridFFT:
         p.coil_chunk = int(coil_chunk)
         p.max_visits = int(max_visits)
         p.bin_edge = int(bin_edge)
+        p.xcd_run = int(xcd_run)
+        p.xcd_run_gather = int(xcd_run_gather)
         if args:  # gridFFT::read_commandline flags
             arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
             check(L.mrn_parse_args(ctypes.byref(p), len(args), arr))
@@ -149,6 +156,12 @@ class GridFFT:
 
     def autotune(self, frame=0):
         self._c(lib().mrn_autotune(self._h, frame, _stream()))
+
+    def tuning(self):
+        """Scatter launch shape in use: {"bin_edge": cells, "max_visits": samples per work item}."""
+        b, v = ctypes.c_int32(), ctypes.c_int32()
+        self._c(lib().mrn_get_tuning(self._h, ctypes.byref(b), ctypes.byref(v)))
+        return {"bin_edge": b.value, "max_visits": v.value}
 
     # ------------------------------------------------------------- operator
     def backward(self, X, data=None, frame=0, sub=None):
@@ -229,10 +242,11 @@ class GridFFT:
 
     def stage_times(self, X, reps=5, frame=0):
         """Average device ms per batched stage (HIP events on the current stream):
-        pre_z, fft_y, fft_x, gather, scatter, ifft_x, ifft_y, post_z."""
-        ms = (ctypes.c_float * 8)()
+        pre_z, fft_y, fft_x, gather, scatter_zero, scatter, ifft_x, ifft_y, post_z."""
+        names = ["pre_z", "fft_y", "fft_x", "gather", "scatter_zero", "scatter", "ifft_x", "ifft_y",
+                 "post_z"]
+        ms = (ctypes.c_float * len(names))()
         self._c(lib().mrn_stage_times(self._h, frame, _ptr(X), int(reps), ms, _stream()))
-        names = ["pre_z", "fft_y", "fft_x", "gather", "scatter", "ifft_x", "ifft_y", "post_z"]
         return dict(zip(names, list(ms)))
 
     def power_method(self, iters=30):
