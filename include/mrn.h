@@ -119,14 +119,25 @@ int mrn_kspace_to_image(mrn_op *op, int frame, const void *data, void *X, int ac
 int mrn_normal(mrn_op *op, int frame, const void *X, void *out, void *stream);
 
 /* One coil, one grid, trajectory per call: gridFFT::backward /
- * backward_residual / forward with a smap (gridFFT.h:118-141).  These run
- * the plain per-coil form (no binning); X accumulates in forward. */
+ * backward_residual / forward with a smap (gridFFT.h:118-141), or without
+ * one when smap is NULL (gridFFT.h:144-156).  These run the plain per-coil
+ * form (no binning); X accumulates in forward. */
 int mrn_coil_backward(mrn_op *op, const void *X, const void *smap, void *data, int64_t M,
                       const float *kx, const float *ky, const float *kz, const float *kw,
                       const void *sub, void *stream);
 int mrn_coil_forward(mrn_op *op, void *X, const void *smap, const void *data, int64_t M,
                      const float *kx, const float *ky, const float *kz, const float *kw,
                      void *stream);
+
+/* X += |w * img|^2 for one coil without a coil map: gridFFT::forward_sos
+ * (gridFFT.cpp:722-748, accumulate_sos 960-970), the sum-of-squares combine. */
+int mrn_coil_forward_sos(mrn_op *op, void *X, const void *data, int64_t M, const float *kx,
+                         const float *ky, const float *kz, const float *kw, void *stream);
+
+/* Change the A^H taper radius (the public member gridFFT::k_rad,
+ * gridFFT.h:102, which the reference's callers change between calls,
+ * recon_lib.cpp:2738-2786); k_rad > 0, INFINITY disables the taper. */
+int mrn_set_k_rad(mrn_op *op, float k_rad, void *stream);
 
 /* Measured k-space of one frame data[C][M] for the solvers below. */
 int mrn_set_kdata(mrn_op *op, int frame, const void *data, void *stream);
@@ -153,10 +164,12 @@ int mrn_power_method(mrn_op *op, int iters, double *step, void *stream);
 int mrn_fista(mrn_op *op, void *X, int iters, double step, int use_graph, double *resid,
               void *stream);
 
-/* Trajectory gradient of 0.5*||A(x;k) - y||^2-type losses: given X and the
- * upstream gradient g[C][M] of the k-space output of one frame, writes
- * grad_k[M][3] (float32, reference k units) = 2 Re sum_c conj(g_cm) dA_cm/dk.
- * No counterpart in the reference (its trajectory is fixed). */
+/* Trajectory gradient: given X and the upstream gradient g[C][M] of the
+ * k-space output y = A(X; k) of one frame (g = dL/dRe y + i dL/dIm y for a
+ * real loss L), writes grad_k[M][3] (float32, reference k units)
+ * = Re sum_c conj(g_cm) dy_cm/dk = dL/dk.  The oversampled grid is
+ * recomputed from X.  No counterpart in the reference (its trajectory is
+ * fixed). */
 int mrn_traj_grad(mrn_op *op, int frame, const void *X, const void *g, float *grad_k,
                   void *stream);
 

@@ -22,6 +22,7 @@ EXPORTS = [
     "mrn_image_to_kspace", "mrn_kspace_to_image", "mrn_normal", "mrn_coil_backward",
     "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_cg_init", "mrn_cg_iterate",
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
+    "mrn_coil_forward_sos", "mrn_set_k_rad",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
 
@@ -83,6 +84,8 @@ def lib():
             "mrn_normal": (i32, [vp, i32, vp, vp, vp]),
             "mrn_coil_backward": (i32, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]),
             "mrn_coil_forward": (i32, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
+            "mrn_coil_forward_sos": (i32, [vp, vp, vp, i64, vp, vp, vp, vp, vp]),
+            "mrn_set_k_rad": (i32, [vp, ctypes.c_float, vp]),
             "mrn_set_kdata": (i32, [vp, i32, vp, vp]),
             "mrn_cg": (i32, [vp, vp, i32, i32, vp, vp]),
             "mrn_cg_init": (i32, [vp, vp, vp]),

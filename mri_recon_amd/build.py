@@ -61,7 +61,25 @@ def build(only=None, jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_facade_test()
     return LIB
+
+
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_cg.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_cg")
+
+
+def build_facade_test():
+    """The C++ program that drives the operator through include/mrn_gridfft.hpp
+    (host code only, linked against libmrn.so)."""
+    if not os.path.exists(FACADE_SRC):
+        return None
+    cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), FACADE_SRC,
+           "-L", PKG, "-lmrn", "-Wl,-rpath,$ORIGIN/../../mri_recon_amd", "-o", FACADE_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade test build failed:\n" + r.stdout + r.stderr)
+    return FACADE_BIN
 
 
 def main(argv=None):

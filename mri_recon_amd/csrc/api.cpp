@@ -21,6 +21,7 @@ int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const f
 int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_scatter_zero(mrn_op *, float2 *, const FrameData &, hipStream_t);
+int recompute_taper(mrn_op *, FrameData &, hipStream_t);
 int launch_scatter_tiles(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_gather_plain(mrn_op *, const float2 *, float2 *, int64_t, const float *, const float *,
                         const float *, const float *, const float2 *, hipStream_t);
@@ -35,7 +36,8 @@ int op_smaps_in(mrn_op *, float2 *, const float2 *, hipStream_t);
 int op_permute_in(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int op_permute_out(mrn_op *, float2 *, const float2 *, const FrameData &, const float2 *, hipStream_t);
 int op_set_image_plain(mrn_op *, float2 *, const float2 *, const float2 *, const PrePostArgs &, hipStream_t);
-int op_accumulate_plain(mrn_op *, const float2 *, float2 *, const float2 *, const PrePostArgs &, hipStream_t);
+int op_accumulate_plain(mrn_op *, const float2 *, float2 *, const float2 *, const PrePostArgs &, hipStream_t,
+                        int sos = 0);
 int op_cg_alpha(mrn_op *, const float2 *, const float2 *, const float2 *, int64_t, double *, hipStream_t);
 int op_cg_update(mrn_op *, float2 *, float2 *, float2 *, const float2 *, int64_t, double *, hipStream_t);
 int op_power_step(mrn_op *, float2 *, const float2 *, int64_t, double *, hipStream_t);
@@ -224,11 +226,11 @@ static int plain_A_coil(mrn_op *op, const float2 *X, const float2 *smap, float2 
 
 static int plain_AH_coil(mrn_op *op, float2 *X, const float2 *smap, const float2 *data, int64_t M,
                          const float *kx, const float *ky, const float *kz, const float *kw,
-                         hipStream_t st) {
+                         hipStream_t st, int sos = 0) {
   MRN_TRY(op_zero(op, op->cgrid, ncell(op), st));
   MRN_TRY(launch_scatter_plain(op, op->cgrid, data, M, kx, ky, kz, kw, st));
   MRN_TRY(plain_passes(op, 1, st));
-  return op_accumulate_plain(op, op->cgrid, X, smap, prepost(op, 0, 0, 1), st);
+  return op_accumulate_plain(op, op->cgrid, X, smap, prepost(op, 0, 0, 1), st, sos);
 }
 
 static const float *traw(const FrameData &fr, int q) { return fr.traw + (int64_t)q * fr.M; }
@@ -505,6 +507,25 @@ int mrn_coil_forward(mrn_op *op, void *X, const void *smap, const void *data, in
   if (!op || !X || !data || (M > 0 && (!kx || !ky || !kz || !kw))) return MRN_EINVAL;
   return plain_AH_coil(op, (float2 *)X, (const float2 *)smap, (const float2 *)data, M, kx, ky, kz, kw,
                        pick(op, stream));
+}
+
+int mrn_set_k_rad(mrn_op *op, float k_rad, void *stream) {
+  if (!op || !(k_rad > 0.0f)) return MRN_EINVAL;
+  hipStream_t st = pick(op, stream);
+  op->p.k_rad = k_rad;
+  op->g.k_rad = k_rad;
+  if (batched(op))
+    for (auto &fr : op->frames)
+      if (fr.has_traj) MRN_TRY(recompute_taper(op, fr, st));
+  op->version++;
+  return MRN_OK;
+}
+
+int mrn_coil_forward_sos(mrn_op *op, void *X, const void *data, int64_t M, const float *kx,
+                         const float *ky, const float *kz, const float *kw, void *stream) {
+  if (!op || !X || !data || (M > 0 && (!kx || !ky || !kz || !kw))) return MRN_EINVAL;
+  return plain_AH_coil(op, (float2 *)X, nullptr, (const float2 *)data, M, kx, ky, kz, kw,
+                       pick(op, stream), 1);
 }
 
 // one CG step (recon_lib.cpp:1578-1722) on the op's solver vectors

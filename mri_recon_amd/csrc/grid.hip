@@ -95,6 +95,21 @@ __global__ void k_traj_finish(const float *__restrict__ kx, const float *__restr
   inv[m] = (int32_t)i;
 }
 
+// taper of the valid samples for a new k_rad (same expression as k_traj_finish)
+__global__ void k_taper(const float *__restrict__ traw, int64_t M, const int32_t *__restrict__ perm,
+                        int64_t Mv, Geom g, float *taper) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Mv) return;
+  const int64_t m = perm[i];
+  float kr = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float k = traw[a * M + m];
+    if (g.fft_in[a]) kr += k * k;
+  }
+  taper[i] = (float)exp(-(double)kr / (2.0 * (double)g.k_rad * (double)g.k_rad));
+}
+
 // Bins (per axis) whose samples can have a tap in tile `ti`: first taps in
 // [ti*T - NT + 1, ti*T + T - 1].
 __host__ __device__ __forceinline__ void tile_bins(const Tiles &t, int a, int ti, int &b0, int &b1) {
@@ -537,6 +552,14 @@ int launch_scatter_plain(mrn_op *op, float2 *grid, const float2 *data, int64_t M
   g.lgCC = 0;
   hipLaunchKernelGGL(k_scatter_plain, dim3(nblk(M, 256)), dim3(256), sizeof(float) * g.ntab, st,
                      grid, data, kx, ky, kz, kw, M, g, op->tables_d);
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+int recompute_taper(mrn_op *op, mrn::FrameData &fr, hipStream_t st) {
+  if (fr.Mv == 0) return MRN_OK;
+  hipLaunchKernelGGL(k_taper, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, fr.traw, fr.M, fr.perm, fr.Mv,
+                     op->g, fr.taper);
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
 }

@@ -100,10 +100,12 @@ __global__ void k_set_image_plain(float2 *grid, const float2 *X, const float2 *s
   }
 }
 
-// plain form: X(r) += (w(r) * img(og + r)) * conj(smap(r))
+// plain form: X(r) += (w(r) * img(og + r)) * conj(smap(r)) (gridFFT::accumulate,
+// gridFFT.cpp:926-958), or with sos != 0 X(r) += |w(r) * img(og + r)|^2
+// (deapp + accumulate_sos, gridFFT.cpp:912-924, 960-970)
 __global__ void k_accumulate_plain(const float2 *grid, float2 *X, const float2 *smap,
                                    const float *wx, const float *wy, const float *wz,
-                                   PrePostArgs a) {
+                                   PrePostArgs a, int sos) {
   const int64_t n = (int64_t)a.N[0] * a.N[1] * a.N[2];
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
@@ -111,7 +113,11 @@ __global__ void k_accumulate_plain(const float2 *grid, float2 *X, const float2 *
     const float wt = (wz[k + a.og[2]] * wy[j + a.og[1]]) * wx[i + a.og[0]];
     const float2 im = grid[((int64_t)(k + a.og[2]) * a.S[1] + j + a.og[1]) * a.S[0] + i + a.og[0]];
     float2 v = make_float2(wt * im.x, wt * im.y);
-    if (smap) v = cmulc(v, smap[r]);
+    if (sos) {
+      v = make_float2(v.x * v.x + v.y * v.y, 0.0f);
+    } else if (smap) {
+      v = cmulc(v, smap[r]);
+    }
     X[r] = make_float2(X[r].x + v.x, X[r].y + v.y);
   }
 }
@@ -370,9 +376,9 @@ int op_set_image_plain(mrn_op *op, float2 *grid, const float2 *X, const float2 *
   return MRN_OK;
 }
 int op_accumulate_plain(mrn_op *op, const float2 *grid, float2 *X, const float2 *smap,
-                        const PrePostArgs &a, hipStream_t st) {
+                        const PrePostArgs &a, hipStream_t st, int sos) {
   hipLaunchKernelGGL(k_accumulate_plain, dim3(gs(op->nvox)), dim3(256), 0, st, grid, X, smap,
-                     op->win_d[0], op->win_d[1], op->win_d[2], a);
+                     op->win_d[0], op->win_d[1], op->win_d[2], a, sos);
   LAUNCH_CHECK(op);
   return MRN_OK;
 }

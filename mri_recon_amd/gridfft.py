@@ -215,6 +215,17 @@ class GridFFT:
                                        _ptr(ky), _ptr(kz), _ptr(kw), _stream()))
         return X
 
+    def coil_forward_sos(self, X, data, traj):
+        """X += |w * img|^2 of one coil's data, no coil map (gridFFT::forward_sos)."""
+        kx, ky, kz, kw = traj
+        self._c(lib().mrn_coil_forward_sos(self._h, _ptr(X), _ptr(data), kx.numel(), _ptr(kx), _ptr(ky),
+                                           _ptr(kz), _ptr(kw), _stream()))
+        return X
+
+    def set_k_rad(self, k_rad):
+        """Change the A^H taper radius (the reference's public member k_rad)."""
+        self._c(lib().mrn_set_k_rad(self._h, float(k_rad), _stream()))
+
     # -------------------------------------------------------------- solvers
     def cg(self, iters, X=None, graph=True, trace=True):
         """SENSE-CG over all frames (recon_lib.cpp:1523-1759).  Returns

@@ -453,6 +453,25 @@ static void accumulate(orc_op *op, const cpx *g, cpx *X, const cpx *smap) {
   }
 }
 
+/* gridFFT.cpp:912-924 then 960-970: image *= wt (deapp), X += image * conj(image) */
+static void accumulate_sos(orc_op *op, const cpx *g, cpx *X) {
+  const int *S = op->S, *N = op->N, *og = op->og;
+#pragma omp parallel for
+  for (int k = 0; k < N[2]; k++) {
+    real wtz = op->win[2][k + og[2]];
+    for (int j = 0; j < N[1]; j++) {
+      real wty = wtz * op->win[1][j + og[1]];
+      for (int i = 0; i < N[0]; i++) {
+        real wt = wty * op->win[0][i + og[0]];
+        long r = i + (long)N[0] * (j + (long)N[1] * k);
+        cpx im = g[(i + og[0]) + (long)S[0] * ((j + og[1]) + (long)S[1] * (k + og[2]))];
+        cpx v = {im.re * wt, im.im * wt};
+        X[r].re += v.re * v.re + v.im * v.im;
+      }
+    }
+  }
+}
+
 /* ----------------------------------------------------- tap weights (a6,a7) */
 /* Per axis: first/last tap and the table-lerp factor of each tap, as in the
  * tap loops of chop_grid_backward/forward (gridFFT.cpp:1211-1271, 1276-1306).
@@ -606,6 +625,17 @@ void orc_forward(orc_op *op, real *X, const real *smap, const real *data, const 
   scatter(op, g, (const cpx *)data, kx, ky, kz, kw, M);
   do_ifft(op, g);
   accumulate(op, g, (cpx *)X, (const cpx *)smap);
+  free(g);
+}
+
+/* gridFFT::forward_sos (gridFFT.cpp:722-748): X += |deapodised A^H (kw taper data)|^2. */
+void orc_forward_sos(orc_op *op, real *X, const real *data, const float *kx, const float *ky,
+                     const float *kz, const float *kw, long M) {
+  cpx *g = grid_alloc(op);
+  memset(g, 0, sizeof(cpx) * (size_t)op->S[0] * op->S[1] * op->S[2]);
+  scatter(op, g, (const cpx *)data, kx, ky, kz, kw, M);
+  do_ifft(op, g);
+  accumulate_sos(op, g, (cpx *)X);
   free(g);
 }
 

@@ -63,6 +63,7 @@ def _lib(precision):
         lib.orc_fft1d.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
         lib.orc_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]
         lib.orc_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i64]
+        lib.orc_forward_sos.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64]
         lib.orc_stage_pre_fft.argtypes = [vp, vp, vp, vp]
         lib.orc_stage_gather.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, vp]
         lib.orc_stage_scatter.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64]
@@ -186,6 +187,15 @@ class OracleOp:
         smap, data = self._c(smap), self._c(data)
         self.lib.orc_forward(self.h, X.ctypes.data, _p(smap), data.ctypes.data, kx.ctypes.data,
                              ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0])
+        return X
+
+    def forward_sos(self, X, data, traj):
+        """X += |deapodised A^H (kw taper data)|^2 (gridFFT::forward_sos); X updated in place."""
+        kx, ky, kz, kw = self._traj(traj)
+        assert X.dtype == self.ctype and X.flags.c_contiguous
+        data = self._c(data)
+        self.lib.orc_forward_sos(self.h, X.ctypes.data, data.ctypes.data, kx.ctypes.data,
+                                 ky.ctypes.data, kz.ctypes.data, kw.ctypes.data, kx.shape[0])
         return X
 
     # --- stages ---------------------------------------------------------

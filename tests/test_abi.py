@@ -61,3 +61,20 @@ def test_invalid_params_rejected_without_device():
     assert L.mrn_op_create(ctypes.byref(p), ctypes.byref(h)) == _ffi.MRN_EINVAL
     assert L.mrn_op_destroy(None) == _ffi.MRN_EINVAL
     assert L.mrn_status_string(4) == b"unsupported configuration"
+
+
+def test_cpp_facade_compiles_against_header(tmp_path):
+    """include/mrn_gridfft.hpp (the C++ gridFFT facade) compiles as plain C++17
+    against include/mrn.h with the host compiler alone."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "mrn_gridfft.hpp"\n'
+                   'int main() { mrn::gridFFT g; g.overgrid = 1.25f; mrn_params p; '
+                   'mrn_params_default(&p); return (int)(g.k_rad != p.k_rad); }\n')
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

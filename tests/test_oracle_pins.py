@@ -395,3 +395,19 @@ def test_power_method_and_fista_vs_dense():
         x, x_old = A * x + B * x_old, x
         x = x - step * (H @ x - b)
     assert np.linalg.norm(X.reshape(-1) - x) / np.linalg.norm(x) < 1e-6
+
+
+def test_forward_sos_is_squared_no_map_forward():
+    """forward_sos (deapp, then X += image * conj(image)) equals the squared
+    magnitude of the no-map forward, which accumulates the deapodised image."""
+    N = 16
+    op = orc.OracleOp(N, overgrid=1.5, precision="f64")
+    rng = np.random.default_rng(11)
+    M = 800
+    traj = [rng.uniform(-N / 2, N / 2, M).astype(np.float32) for _ in range(3)]
+    traj.append(rng.uniform(0.2, 1.0, M).astype(np.float32))
+    d = crandn(M, rng=rng)
+    Xf = op.forward(np.zeros((N, N, N), np.complex128), None, d, traj)
+    X0 = crandn(N, N, N, rng=rng)
+    Xs = op.forward_sos(X0.copy(), d, traj)
+    assert np.allclose(Xs - X0, np.abs(Xf) ** 2, rtol=1e-12, atol=1e-12 * np.max(np.abs(Xf) ** 2))
