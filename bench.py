@@ -38,6 +38,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def rocprof_summary(root):
+    """Condense rocprofv3 CSV output under `root` (kernel stats and counter
+    collections) into a short text table on stdout."""
+    import collections
+    import csv
+    import glob
+    out = []
+    for f in sorted(glob.glob(os.path.join(root, "**", "*kernel_stats.csv"), recursive=True)):
+        rows = sorted(csv.DictReader(open(f)), key=lambda r: -float(r["TotalDurationNs"]))
+        out.append("%-64s %6s %11s %10s %6s" % ("kernel (kernel_stats.csv)", "calls", "avg_us", "total_ms", "pct"))
+        for r in rows[:30]:
+            out.append("%-64s %6s %11.1f %10.2f %6.2f" % (r["Name"][:64], r["Calls"], float(r["AverageNs"]) / 1e3,
+                                                          float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    for f in sorted(glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True)):
+        acc = collections.defaultdict(list)
+        for r in csv.DictReader(open(f)):
+            acc[(r["Kernel_Name"][:64], r["Counter_Name"])].append(float(r["Counter_Value"]))
+        out.append("%-64s %-11s %5s %15s %15s" % ("kernel (counter_collection.csv)", "counter", "n",
+                                                  "mean/dispatch", "max/dispatch"))
+        for (k, c), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:30]:
+            out.append("%-64s %-11s %5d %15.1f %15.1f" % (k, c, len(v), sum(v) / len(v), max(v)))
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,7 +74,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile", action="store_true", help="batched iterations only (for rocprofv3)")
     ap.add_argument("--small", action="store_true", help="128^3 x 8 coils, 4096 x 64 radial (os 1.5, W=4)")
+    ap.add_argument("--rocprof-summary", metavar="DIR", help="summarise rocprofv3 CSV output and exit")
     a = ap.parse_args()
+    if a.rocprof_summary:
+        rocprof_summary(a.rocprof_summary)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
