@@ -262,19 +262,43 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 
 // Grid-driven scatter.  Wavefront w of the workgroup takes work item
 // subs[4*b + w] = {tile, first visit, end visit, heavy} (b = the workgroup's
-// XCD-ordered index); lane l owns
-// cell (l&3, (l>>2)&3, l>>4) of the 4x4x4 tile and the CCT coil sums of coil
-// block blockIdx.y.  kin holds kw * taper * data (internal sample order).
-// Sample coordinates and data are wave-uniform loads; every lane adds its own
-// tap.
+// XCD-ordered index); lane l owns cell (l&3, (l>>2)&3, l>>4) of the 4x4x4
+// tile and the CCT coil sums of coil block blockIdx.y.  kin holds
+// kw * taper * data (internal sample order).
+//
+// The samples that can reach the tile lie in the bin rows (bz, by) x
+// [bx0, bx1] of its neighbourhood, each row one contiguous run; their
+// concatenation in (bz, by) order numbers the tile's visits (the order the
+// work list was counted in).  The wavefront walks the visits in batches of
+// SG_BATCH samples:
+//  A. lane (s, q) = (l>>2, l&3) finds visit s of the batch (binary search of
+//     the row prefix sums in LDS), loads its coordinates and, for each axis,
+//     the kernel weight of tile offset q (0 outside the sample's taps).  The
+//     weights and the sample's CCT coil values go to LDS; a ballot marks the
+//     samples with a tap in the tile.
+//  B. for each marked sample, every lane forms its weight
+//     (wz * wy) * wx from LDS and adds weight * data to its CCT sums.
+// Per sample the wavefront thus evaluates 12 weights instead of 192, and a
+// sample that misses the tile costs only its share of step A.  The sums
+// accumulate in visit order, as a lane-per-sample walk would.
+#define SG_BATCH 16
+#define SG_MAXROWS 144
+// Make this wavefront's LDS writes visible to its other lanes (LDS executes a
+// wavefront's instructions in order; the fence keeps the compiler from moving
+// LDS accesses across this point and waits for the writes).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 template <int CCT>
 __global__ __launch_bounds__(256) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int32_t *__restrict__ bin_start, const int4 *__restrict__ subs, int nsub, Geom g,
     Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
-  extern __shared__ float tab_s[];
-  load_tables(tab_s, tables, g.ntab);
-  __syncthreads();
+  __shared__ int s_rs[4][SG_MAXROWS];        // first sample of each bin row
+  __shared__ int s_rp[4][SG_MAXROWS + 1];    // visits before each row
+  __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
+  __shared__ float2 s_d[4][SG_BATCH * CCT];  // [sample][coil] data
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
@@ -282,50 +306,97 @@ __global__ __launch_bounds__(256) void k_scatter_gd(
   const int4 sp = subs[sidx];
   const int tile = sp.x;
   const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
-  const int t0x = tix * 4, t0y = tiy * 4, t0z = tiz * 4;
-  const int lx = t0x + (lane & 3), ly = t0y + ((lane >> 2) & 3), lz = t0z + (lane >> 4);
+  const int t0[3] = {tix * 4, tiy * 4, tiz * 4};
+  const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
   const int ch = blockIdx.y;
   const float2 *kch = kin + ch * chunk_k;
-  float ar[CCT], ai[CCT];
-#pragma unroll
-  for (int c = 0; c < CCT; c++) ar[c] = ai[c] = 0.f;
   int bx0, bx1, by0, by1, bz0, bz1;
   tile_bins(t, 0, tix, bx0, bx1);
   tile_bins(t, 1, tiy, by0, by1);
   tile_bins(t, 2, tiz, bz0, bz1);
-  int v = 0;
-  for (int bz = bz0; bz <= bz1; bz++)
-    for (int by = by0; by <= by1; by++) {
-      // the bins bx0..bx1 of a row hold one contiguous run of samples
-      const int brow = (bz * t.nb[1] + by) * t.nb[0];
-      const int s0 = bin_start[brow + bx0], n = bin_start[brow + bx1 + 1] - s0;
-      const int j0 = max(sp.y - v, 0), j1 = min(sp.z - v, n);
-      v += n;
-      for (int j = j0; j < j1; j++) {
-        const int64_t i = (int64_t)s0 + j;
-        const float4 k = coords[i];
-        int sx, ex, sy, ey, sz, ez;
-        axis_range(g, 0, k.x, sx, ex);
-        axis_range(g, 1, k.y, sy, ey);
-        axis_range(g, 2, k.z, sz, ez);
-        if (ex < t0x || sx > t0x + 3 || ey < t0y || sy > t0y + 3 || ez < t0z || sz > t0z + 3) continue;
-        float2 d[CCT];
+  const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
+  // row starts and inclusive prefix counts (rows r = (bz - bz0) * nry + by - by0)
+  int carry = 0;
+  for (int r0 = 0; r0 < nrows; r0 += 64) {
+    const int r = r0 + lane;
+    int n = 0;
+    if (r < nrows) {
+      const int brow = ((bz0 + r / nry) * t.nb[1] + by0 + r % nry) * t.nb[0];
+      const int s0 = bin_start[brow + bx0];
+      n = bin_start[brow + bx1 + 1] - s0;
+      s_rs[wave][r] = s0;
+    }
+    int x = n;
 #pragma unroll
-        for (int c = 0; c < CCT; c++) d[c] = kch[i * CCT + c];
-        if (lx >= sx && lx <= ex && ly >= sy && ly <= ey && lz >= sz && lz <= ez) {
-          const float w = (tap_weight(g, 2, tab_s + g.toff[2], k.z, lz) *
-                           tap_weight(g, 1, tab_s + g.toff[1], k.y, ly)) *
-                          tap_weight(g, 0, tab_s + g.toff[0], k.x, lx);
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (r < nrows) s_rp[wave][r + 1] = carry + x;
+    carry += __shfl(x, 63);
+  }
+  if (lane == 0) s_rp[wave][0] = 0;
+  wave_sync();
+  const int vbeg = max(sp.y, 0), vend = min(sp.z, carry);
+  float ar[CCT], ai[CCT];
 #pragma unroll
-          for (int c = 0; c < CCT; c++) {
-            ar[c] += w * d[c].x;
-            ai[c] += w * d[c].y;
-          }
-        }
+  for (int c = 0; c < CCT; c++) ar[c] = ai[c] = 0.f;
+  const int sl = lane >> 2, q = lane & 3;
+  constexpr int CPL = CCT >= 4 ? CCT / 4 : 1;  // coil values a lane stages
+  for (int base = vbeg; base < vend; base += SG_BATCH) {
+    // ---- A: weights and data of the batch
+    const int v = base + sl;
+    const bool ok = v < vend;
+    int64_t i = 0;
+    if (ok) {
+      int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_rp[wave][mid] <= v) lo = mid;
+        else hi = mid - 1;
+      }
+      i = (int64_t)s_rs[wave][lo] + (v - s_rp[wave][lo]);
+    }
+    const float4 k = ok ? coords[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float dk[3] = {k.x, k.y, k.z};
+    unsigned long long hit = ~0ull;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      int st = 0, en = -1;
+      const int l = t0[a] + q;
+      bool in = ok && axis_range(g, a, dk[a], st, en);
+      in = in && l >= st && l <= en;
+      s_w[wave][sl * 12 + a * 4 + q] = in ? tap_weight(g, a, tables + g.toff[a], dk[a], l) : 0.0f;
+      unsigned long long m = __ballot(in);
+      m |= m >> 1;
+      m |= m >> 2;
+      hit &= m;  // bit 4s: sample s has a tap in the tile on every axis so far
+    }
+    hit &= 0x1111111111111111ull;
+    if (ok && q * CPL < CCT) {
+#pragma unroll
+      for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = kch[i * CCT + q * CPL + c];
+    }
+    wave_sync();
+    // ---- B: accumulate the samples that reach the tile
+    while (hit) {
+      const int s = __builtin_ctzll(hit) >> 2;
+      hit &= hit - 1;
+      const float *wp = &s_w[wave][s * 12];
+      const float w = (wp[8 + lz] * wp[4 + ly]) * wp[lx];
+      const float2 *dp = &s_d[wave][s * CCT];
+#pragma unroll
+      for (int c = 0; c < CCT; c++) {
+        const float2 d = dp[c];
+        ar[c] += w * d.x;
+        ai[c] += w * d.y;
       }
     }
-  if (lx >= g.S[0] || ly >= g.S[1] || lz >= g.S[2]) return;
-  float2 *dst = grid + ch * chunk_grid + (((int64_t)lz * g.S[1] + ly) * g.S[0] + lx) * CCT;
+    wave_sync();
+  }
+  const int cx = t0[0] + lx, cy = t0[1] + ly, cz = t0[2] + lz;
+  if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) return;
+  float2 *dst = grid + ch * chunk_grid + (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT;
   if (sp.w) {
 #pragma unroll
     for (int c = 0; c < CCT; c++) {
@@ -497,7 +568,9 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   const Tiles &t = op->t;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   dim3 g3((fr.nsub + 3) / 4, op->nch);
-  const size_t sh = sizeof(float) * g.ntab;
+  const size_t sh = 0;
+  const int rows_axis = (t.T + t.NT - 2) / t.B + 2;  // bins per axis a tile can reach
+  if (rows_axis * rows_axis > SG_MAXROWS) return fail(op, MRN_EUNSUPPORTED, "scatter neighbourhood too large");
   switch (g.CC) {
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \

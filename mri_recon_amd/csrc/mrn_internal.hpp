@@ -66,7 +66,7 @@ struct FrameData {
   float *taper = nullptr;      // [Mv] exp(-|k|^2 / (2 k_rad^2))
   int32_t *perm = nullptr;     // [Mv] tile order -> caller index
   int32_t *inv = nullptr;      // [M] caller index -> tile order, -1 if skipped
-  int32_t *bin_start = nullptr;// [ntiles + 1]
+  int32_t *bin_start = nullptr;// [nbins + 1] first sample of each bin
   int4 *subs = nullptr;        // [nsub] {tile, first visit, end visit, heavy}
   int32_t *heavy = nullptr;    // [nheavy] tiles split over several work items
   int nsub = 0, nheavy = 0;

@@ -287,7 +287,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
     delete op;
     return MRN_EINVAL;
   }
-  op->t.B = p.bin_edge > 0 ? p.bin_edge : 2;
+  op->t.B = p.bin_edge > 0 ? p.bin_edge : 1;
   op->t.T = 4;
   for (int a = 0; a < 3; a++) {
     op->t.nb[a] = (g.S[a] + op->t.B - 1) / op->t.B;
