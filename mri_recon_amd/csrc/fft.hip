@@ -205,9 +205,14 @@ __device__ __forceinline__ void zero_outside(float2 *buf, int lo, int hi) {
 //  - run < 16, a power of two dividing 16 (x pass, CC < 16): 16/run rows, each
 //    a contiguous run*np block: shifts only;
 //  - anything else (tiny plain-form grids): generic index arithmetic.
+// In the first two a thread moves at most ceil(S/16) points; it issues all of
+// its loads before the first LDS write (and all LDS reads before the first
+// store), so that a workgroup keeps up to S*16*8 bytes in flight instead of
+// one load per thread.
 template <int S, bool STORE>
 __device__ __forceinline__ void move_lines(float2 *__restrict__ grid, float2 *buf, const LinesArgs &a,
                                            int64_t base, int g0, int p0, int np) {
+  constexpr int NJ = (S + 15) / 16;
   const int tid = threadIdx.x;
   if (a.run >= FFT_LINES) {
     const int l = tid & (FFT_LINES - 1);
@@ -215,26 +220,63 @@ __device__ __forceinline__ void move_lines(float2 *__restrict__ grid, float2 *bu
     if (gl >= a.nlines) return;
     const int r = gl / a.run, cc = gl - r * a.run;
     float2 *src = grid + base + (int64_t)r * a.rstride + cc;
-    for (int p = p0 + (tid >> 4); p < p0 + np; p += FFT_THREADS / FFT_LINES) {
-      if (STORE)
-        src[(int64_t)p * a.pstride] = scl(buf[l * (S + FFT_PAD) + p], a.scale);
-      else
-        buf[l * (S + FFT_PAD) + p] = src[(int64_t)p * a.pstride];
+    const int pb = p0 + (tid >> 4), pe = p0 + np;
+    float2 v[NJ];
+    if (STORE) {
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int p = pb + 16 * j;
+        if (p < pe) v[j] = scl(buf[l * (S + FFT_PAD) + p], a.scale);
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int p = pb + 16 * j;
+        if (p < pe) src[(int64_t)p * a.pstride] = v[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int p = pb + 16 * j;
+        if (p < pe) v[j] = src[(int64_t)p * a.pstride];
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int p = pb + 16 * j;
+        if (p < pe) buf[l * (S + FFT_PAD) + p] = v[j];
+      }
     }
   } else if ((a.run & (a.run - 1)) == 0 && a.pstride == a.run) {
+    // element e = tid + 256 j of the 16 * np moved: row rr = e / (W np), then
+    // (point, line-in-row) = (e % (W np)) / W, % W
     const int W = a.run;
     const int lgW = __builtin_ctz(W);
-    for (int rr = 0; rr < FFT_LINES / W; rr++) {
-      if (g0 + rr * W >= a.nlines) break;
-      float2 *row = grid + base + (int64_t)((g0 >> lgW) + rr) * a.rstride;
-      for (int idx = tid; idx < W * np; idx += FFT_THREADS) {
-        const int c = idx & (W - 1), p = p0 + (idx >> lgW);
-        const int l = rr * W + c;
-        if (STORE)
-          row[(int64_t)p * W + c] = scl(buf[l * (S + FFT_PAD) + p], a.scale);
-        else
-          buf[l * (S + FFT_PAD) + p] = row[(int64_t)p * W + c];
-      }
+    const int rowlen = W * np;
+    float2 v[NJ];
+    int off[NJ];  // LDS index, -1 when not moved
+    int64_t gidx[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int e = tid + FFT_THREADS * j;
+      const int rr = e / rowlen, q = e - rr * rowlen;
+      const int c = q & (W - 1), p = p0 + (q >> lgW);
+      const bool ok = e < FFT_LINES * np && g0 + rr * W < a.nlines;
+      off[j] = ok ? (rr * W + c) * (S + FFT_PAD) + p : -1;
+      gidx[j] = base + (int64_t)((g0 >> lgW) + rr) * a.rstride + (int64_t)p * W + c;
+    }
+    if (STORE) {
+#pragma unroll
+      for (int j = 0; j < NJ; j++)
+        if (off[j] >= 0) v[j] = scl(buf[off[j]], a.scale);
+#pragma unroll
+      for (int j = 0; j < NJ; j++)
+        if (off[j] >= 0) grid[gidx[j]] = v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < NJ; j++)
+        if (off[j] >= 0) v[j] = grid[gidx[j]];
+#pragma unroll
+      for (int j = 0; j < NJ; j++)
+        if (off[j] >= 0) buf[off[j]] = v[j];
     }
   } else {
     for (int idx = tid; idx < FFT_LINES * np; idx += FFT_THREADS) {
@@ -251,7 +293,7 @@ __device__ __forceinline__ void move_lines(float2 *__restrict__ grid, float2 *bu
   }
 }
 
-// Generic grid -> grid pass over FFT_LINES lines (see LinesArgs).  Lines past
+// Grid -> grid pass over FFT_LINES lines (see LinesArgs).  Lines past
 // nlines are never loaded (their LDS rows hold zeros or stale values, which
 // only feed their own, unstored, transforms).
 template <int S, int DIR>
@@ -281,33 +323,61 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z(
   __shared__ float2 tws[S];
   load_twiddles<S>(tws, tw);
   const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
-  const int nl = a.N[0] * CC, g0 = blockIdx.x * FFT_LINES;
+  // 16 consecutive workgroups (x columns) per XCD: with 16 coils per cell a
+  // workgroup reads one x of the image, and X's 128-byte lines hold 16 x
+  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);
+  const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   const int ogz = a.og[2];
   zero_outside<S>(buf, ogz, ogz + a.N[2]);
   const float wyv = wy[a.og[1] + y];
   const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
-  for (int idx = threadIdx.x; idx < FFT_LINES * a.N[2]; idx += FFT_THREADS) {
-    const int l = idx % FFT_LINES, zi = idx / FFT_LINES;
-    const int gl = g0 + l;
-    float2 v = make_float2(0.f, 0.f);
-    if (gl < nl) {
-      const int xi = gl >> a.lgCC;
-      const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];
+  constexpr int NJ = (S + 15) / 16;
+  // thread: line l = tid % 16, z points z0 + 16 j; all loads first
+  const int l = threadIdx.x & (FFT_LINES - 1), z0 = threadIdx.x >> 4;
+  const int gl = g0 + l;
+  const bool lok = gl < nl;
+  const int xi = gl >> a.lgCC;
+  const float wxv = lok ? wx[a.og[0] + xi] : 0.f;
+  float2 xv[NJ], sv[NJ];
+  float wzv[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int zi = z0 + 16 * j;
+    if (lok && zi < a.N[2]) {
       const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
-      const float2 xv = X[r + xi];
-      v = cmul(make_float2(wt * xv.x, wt * xv.y), sm[r * CC + gl]);
+      xv[j] = X[r + xi];
+      sv[j] = sm[r * CC + gl];
+      wzv[j] = wz[ogz + zi];
     }
-    buf[l * (S + FFT_PAD) + ogz + zi] = v;
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int zi = z0 + 16 * j;
+    if (zi < a.N[2]) {
+      float2 v = make_float2(0.f, 0.f);
+      if (lok) {
+        const float wt = (wzv[j] * wyv) * wxv;
+        v = cmul(make_float2(wt * xv[j].x, wt * xv[j].y), sv[j]);
+      }
+      buf[l * (S + FFT_PAD) + ogz + zi] = v;
+    }
   }
   __syncthreads();
   fft_stages<S, -1, 0, 1>(buf, tws);
-  float2 *gch = grid + (int64_t)ch * a.chunk_grid;
-  for (int idx = threadIdx.x; idx < FFT_LINES * S; idx += FFT_THREADS) {
-    const int l = idx % FFT_LINES, p = idx / FFT_LINES;
-    const int gl = g0 + l;
-    if (gl < nl)
-      gch[(((int64_t)p * a.S[1] + a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl] =
-          scl(buf[l * (S + FFT_PAD) + p], a.scale);
+  if (lok) {
+    float2 *gch = grid + (int64_t)ch * a.chunk_grid + ((int64_t)(a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl;
+    const int64_t pstride = (int64_t)a.S[1] * a.S[0] * CC;
+    float2 v[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int p = z0 + 16 * j;
+      if (p < S) v[j] = scl(buf[l * (S + FFT_PAD) + p], a.scale);
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int p = z0 + 16 * j;
+      if (p < S) gch[(int64_t)p * pstride] = v[j];
+    }
   }
 }
 
@@ -322,14 +392,26 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   __shared__ float2 tws[S];
   load_twiddles<S>(tws, tw);
   const int CC = a.CC, y = blockIdx.y, ch = a.ch;
-  const int nl = a.N[0] * CC, g0 = blockIdx.x * FFT_LINES;
-  const float2 *gch = grid + (int64_t)ch * a.chunk_grid;
-  for (int idx = threadIdx.x; idx < FFT_LINES * S; idx += FFT_THREADS) {
-    const int l = idx % FFT_LINES, p = idx / FFT_LINES;
+  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);  // as in k_fft_pre_z
+  const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
+  constexpr int NJ = (S + 15) / 16;
+  {
+    const int l = threadIdx.x & (FFT_LINES - 1), p0 = threadIdx.x >> 4;
     const int gl = g0 + l;
-    float2 v = make_float2(0.f, 0.f);
-    if (gl < nl) v = gch[(((int64_t)p * a.S[1] + a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl];
-    buf[l * (S + FFT_PAD) + p] = v;
+    const float2 *src = grid + (int64_t)ch * a.chunk_grid + ((int64_t)(a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl;
+    const int64_t pstride = (int64_t)a.S[1] * a.S[0] * CC;
+    float2 v[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int p = p0 + 16 * j;
+      v[j] = make_float2(0.f, 0.f);
+      if (gl < nl && p < S) v[j] = src[(int64_t)p * pstride];
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int p = p0 + 16 * j;
+      if (p < S) buf[l * (S + FFT_PAD) + p] = v[j];
+    }
   }
   __syncthreads();
   fft_stages<S, 1, 0, 1>(buf, tws);
@@ -337,26 +419,30 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   const int ogz = a.og[2];
   const float wyv = wy[a.og[1] + y];
   const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
+  const int lgx = 4 - a.lgCC;
   for (int pr = threadIdx.x; pr < nxl * a.N[2]; pr += FFT_THREADS) {
-    const int lgx = 4 - a.lgCC;
     const int xl = pr & (nxl - 1), zi = pr >> lgx;
-    const int xi = blockIdx.x * nxl + xl;
+    const int xi = bx * nxl + xl;
     if (xi >= a.N[0]) continue;
     const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];
     const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0] + xi;
+    // the CC coil maps of this voxel are contiguous: issue every load first
+    float2 smv[FFT_LINES];
+#pragma unroll
+    for (int c = 0; c < FFT_LINES; c++)
+      if (c < CC) smv[c] = sm[r * CC + c];
+    const float2 o = a.overwrite ? make_float2(0.f, 0.f) : X[r];
     float2 acc = make_float2(0.f, 0.f);
-    for (int c = 0; c < CC; c++) {
-      const float2 im = scl(buf[(xl * CC + c) * (S + FFT_PAD) + ogz + zi], a.scale);
-      const float2 t = cmulc(make_float2(wt * im.x, wt * im.y), sm[r * CC + c]);
-      acc.x += t.x;
-      acc.y += t.y;
+#pragma unroll
+    for (int c = 0; c < FFT_LINES; c++) {
+      if (c < CC) {
+        const float2 im = scl(buf[(xl * CC + c) * (S + FFT_PAD) + ogz + zi], a.scale);
+        const float2 t = cmulc(make_float2(wt * im.x, wt * im.y), smv[c]);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
     }
-    if (a.overwrite) {
-      X[r] = acc;
-    } else {
-      float2 o = X[r];
-      X[r] = make_float2(o.x + acc.x, o.y + acc.y);
-    }
+    X[r] = a.overwrite ? acc : make_float2(o.x + acc.x, o.y + acc.y);
   }
 }
 

@@ -27,21 +27,6 @@ __device__ __forceinline__ float grid_coord(const Geom &g, int a, float k) {
   return __fadd_rn(__fmul_rn(k, g.grid[a]), (float)(g.S[a] / 2));
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Renumber them so that each XCD runs runs of `run` consecutive work items:
-// neighbouring tiles and samples, which read the same grid cells and k-space
-// samples, then share one L2, while the runs still interleave over the XCDs
-// so that dense and sparse regions of k-space spread evenly.  run = 1 keeps
-// the hardware order.  A bijection of [0, nb): full groups of 8*run blocks
-// are permuted, the tail keeps its order.
-#define MRN_XCDS 8
-__device__ __forceinline__ int xcd_block(int b, int nb, int run) {
-  const int G = MRN_XCDS * run;
-  if (run <= 1 || b >= (nb / G) * G) return b;
-  const int w = b % G;
-  return (b - w) + (w % MRN_XCDS) * run + w / MRN_XCDS;
-}
-
 __device__ __forceinline__ void load_tables(float *tab_s, const float *tables, int ntab) {
   for (int q = threadIdx.x; q < ntab; q += blockDim.x) tab_s[q] = tables[q];
 }
