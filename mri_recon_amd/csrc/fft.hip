@@ -395,8 +395,26 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   const int bx = xcd_block(blockIdx.x, gridDim.x, 16);  // as in k_fft_pre_z
   const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   constexpr int NJ = (S + 15) / 16;
+  const int nxl = FFT_LINES / CC;
+  const int ogz = a.og[2];
+  const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
+  // Thread = (line l = x-in-group * CC + coil, z0 = tid / 16), output z = z0 + 16 j:
+  // the 16 lanes of a z read one contiguous 128-byte run of coil maps (as the
+  // forward z pass does), and the CC coil terms of a voxel are summed across
+  // lanes.  Coil maps are loaded before the transform.
+  const int l = threadIdx.x & (FFT_LINES - 1), z0 = threadIdx.x >> 4;
+  const int c = l & (CC - 1), xl = l >> a.lgCC;
+  const int xi = bx * nxl + xl;
+  const bool xok = xi < a.N[0];
+  float2 smv[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int zi = z0 + 16 * j;
+    smv[j] = make_float2(0.f, 0.f);
+    if (xok && zi < a.N[2]) smv[j] = sm[(((int64_t)zi * a.N[1] + y) * a.N[0] + xi) * CC + c];
+  }
   {
-    const int l = threadIdx.x & (FFT_LINES - 1), p0 = threadIdx.x >> 4;
+    const int p0 = z0;
     const int gl = g0 + l;
     const float2 *src = grid + (int64_t)ch * a.chunk_grid + ((int64_t)(a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl;
     const int64_t pstride = (int64_t)a.S[1] * a.S[0] * CC;
@@ -415,34 +433,30 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   }
   __syncthreads();
   fft_stages<S, 1, 0, 1>(buf, tws);
-  const int nxl = FFT_LINES / CC;
-  const int ogz = a.og[2];
-  const float wyv = wy[a.og[1] + y];
-  const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
-  const int lgx = 4 - a.lgCC;
-  for (int pr = threadIdx.x; pr < nxl * a.N[2]; pr += FFT_THREADS) {
-    const int xl = pr & (nxl - 1), zi = pr >> lgx;
-    const int xi = bx * nxl + xl;
-    if (xi >= a.N[0]) continue;
-    const float wt = (wz[ogz + zi] * wyv) * wx[a.og[0] + xi];
-    const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0] + xi;
-    // the CC coil maps of this voxel are contiguous: issue every load first
-    float2 smv[FFT_LINES];
+  const float wyx = wy[a.og[1] + y] * (xok ? wx[a.og[0] + xi] : 0.f);
 #pragma unroll
-    for (int c = 0; c < FFT_LINES; c++)
-      if (c < CC) smv[c] = sm[r * CC + c];
-    const float2 o = a.overwrite ? make_float2(0.f, 0.f) : X[r];
-    float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int c = 0; c < FFT_LINES; c++) {
-      if (c < CC) {
-        const float2 im = scl(buf[(xl * CC + c) * (S + FFT_PAD) + ogz + zi], a.scale);
-        const float2 t = cmulc(make_float2(wt * im.x, wt * im.y), smv[c]);
-        acc.x += t.x;
-        acc.y += t.y;
+  for (int j = 0; j < NJ; j++) {
+    const int zi = z0 + 16 * j;
+    const bool zok = zi < a.N[2];
+    float2 t = make_float2(0.f, 0.f);
+    if (xok && zok) {
+      const float wt = wz[ogz + zi] * wyx;
+      const float2 im = scl(buf[l * (S + FFT_PAD) + ogz + zi], a.scale);
+      t = cmulc(make_float2(wt * im.x, wt * im.y), smv[j]);
+    }
+    for (int o = 1; o < CC; o <<= 1) {  // sum the CC coils (adjacent lanes)
+      t.x += __shfl_xor(t.x, o);
+      t.y += __shfl_xor(t.y, o);
+    }
+    if (xok && zok && c == 0) {
+      const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0] + xi;
+      if (a.overwrite) {
+        X[r] = t;
+      } else {
+        const float2 o = X[r];
+        X[r] = make_float2(o.x + t.x, o.y + t.y);
       }
     }
-    X[r] = a.overwrite ? acc : make_float2(o.x + acc.x, o.y + acc.y);
   }
 }
 
