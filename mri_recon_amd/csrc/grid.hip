@@ -145,7 +145,32 @@ __global__ void k_compact(const int32_t *__restrict__ flag, const int32_t *__res
 // Taps per axis handled by the unrolled path; samples with more taps on an
 // axis (a coordinate exactly an integer away from the window edge) take the
 // loop path.
-template <int NT>
+// NC coils per lane (NC = 2: one 16-byte load per tap covers two coils, which
+// halves the vector-memory instructions the texture path has to address).
+template <int NC>
+struct CoilVec;
+template <>
+struct CoilVec<1> {
+  float2 v[1];
+  __device__ __forceinline__ static CoilVec load(const float2 *p) {
+    CoilVec r;
+    r.v[0] = *p;
+    return r;
+  }
+};
+template <>
+struct CoilVec<2> {
+  float2 v[2];
+  __device__ __forceinline__ static CoilVec load(const float2 *p) {
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    CoilVec r;
+    r.v[0] = make_float2(q.x, q.y);
+    r.v[1] = make_float2(q.z, q.w);
+    return r;
+  }
+};
+
+template <int NT, int NC>
 __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
                                                 float2 *__restrict__ out,
                                                 const float4 *__restrict__ coords,
@@ -158,8 +183,9 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
   const int64_t gid = (int64_t)xcd_block(blockIdx.x, gridDim.x, xrun) * blockDim.x + threadIdx.x;
-  const int c = (int)(gid & (g.CC - 1));
-  const int64_t i = gid >> g.lgCC;
+  const int lgl = g.lgCC - (NC == 2 ? 1 : 0);  // log2 of the lanes per sample
+  const int c = (int)(gid & ((1 << lgl) - 1)) * NC;
+  const int64_t i = gid >> lgl;
   if (i >= Mv) return;
   const int ch = blockIdx.y;
   const float4 k = coords[i];
@@ -174,7 +200,9 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
     wide = wide || n[a] > NT;
   }
   const float2 *G = grid + ch * chunk_grid + c;
-  float ar = 0.f, ai = 0.f;
+  float ar[NC], ai[NC];
+#pragma unroll
+  for (int q = 0; q < NC; q++) ar[q] = ai[q] = 0.f;
   if (!wide) {
     float w[3][NT];
     int o[3][NT];
@@ -195,9 +223,12 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
 #pragma unroll
         for (int tx = 0; tx < NT; tx++) {
           const float ww = wzy * w[0][tx];
-          const float2 v = row[(int64_t)o[0][tx] * g.CC];
-          ar += ww * v.x;
-          ai += ww * v.y;
+          const CoilVec<NC> v = CoilVec<NC>::load(row + (int64_t)o[0][tx] * g.CC);
+#pragma unroll
+          for (int q = 0; q < NC; q++) {
+            ar[q] += ww * v.v[q].x;
+            ai[q] += ww * v.v[q].y;
+          }
         }
       }
   } else {
@@ -208,25 +239,31 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
         const float2 *row = G + ((int64_t)lz * g.S[1] + ly) * g.S[0] * g.CC;
         for (int lx = s[0]; lx < s[0] + n[0]; lx++) {
           const float ww = wzy * tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
-          const float2 v = row[(int64_t)lx * g.CC];
-          ar += ww * v.x;
-          ai += ww * v.y;
+          const CoilVec<NC> v = CoilVec<NC>::load(row + (int64_t)lx * g.CC);
+#pragma unroll
+          for (int q = 0; q < NC; q++) {
+            ar[q] += ww * v.v[q].x;
+            ai[q] += ww * v.v[q].y;
+          }
         }
       }
     }
   }
-  const int64_t oi = ch * chunk_k + i * g.CC + c;
-  if (sub) {
-    const float2 d = sub[oi];
-    ar -= d.x;
-    ai -= d.y;
+#pragma unroll
+  for (int q = 0; q < NC; q++) {
+    const int64_t oi = ch * chunk_k + i * g.CC + c + q;
+    if (sub) {
+      const float2 d = sub[oi];
+      ar[q] -= d.x;
+      ai[q] -= d.y;
+    }
+    if (weighted) {  // the scatter's input kw * taper * y (gridFFT.cpp:1040-1108)
+      const float tp = taper[i];
+      ar[q] = (ar[q] * k.w) * tp;
+      ai[q] = (ai[q] * k.w) * tp;
+    }
+    out[oi] = make_float2(ar[q], ai[q]);
   }
-  if (weighted) {  // the scatter's input kw * taper * y (gridFFT.cpp:1040-1108)
-    const float tp = taper[i];
-    ar = (ar * k.w) * tp;
-    ai = (ai * k.w) * tp;
-  }
-  out[oi] = make_float2(ar, ai);
 }
 
 // ------------------------------------------------------------ scatter
@@ -515,18 +552,21 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
                   const float2 *sub, bool weighted, hipStream_t st) {
   const Geom &g = op->g;
   if (fr.Mv == 0) return MRN_OK;
-  dim3 g3(nblk(fr.Mv * g.CC, 256), op->nch);
+  const int nc = g.CC >= 2 ? 2 : 1;  // coils per lane
+  dim3 g3(nblk(fr.Mv * (g.CC / nc), 256), op->nch);
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   const int wflag = weighted ? 1 : 0;
-  switch (taps_main(g)) {
-#define MRN_G(n)                                                                                    \
-  case n:                                                                                           \
-    hipLaunchKernelGGL((k_gather<n>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,        \
+  switch (taps_main(g) * 2 + (nc - 1)) {
+#define MRN_G1(n, c)                                                                                \
+  case n * 2 + c - 1:                                                                               \
+    hipLaunchKernelGGL((k_gather<n, c>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,     \
                        fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag,            \
                        op->t.xrun_gather);                                                          \
     break;
+#define MRN_G(n) MRN_G1(n, 1) MRN_G1(n, 2)
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
 #undef MRN_G
+#undef MRN_G1
     default:
       return fail(op, MRN_EUNSUPPORTED, "kernel width above 8 taps");
   }
