@@ -29,6 +29,9 @@
  *   src/recon_lib.cpp:1791-1915  power method for the FISTA step size
  *   src/recon_lib.cpp:1925-2105  FISTA/IST gradient step
  *   src/recon_lib.cpp:2177-2212  fista_update momentum
+ *   src/wavelet3D.cpp:99-190,563-791  3-D periodic wavelet (4-tap Daubechies pair)
+ *   src/threshold.cpp:250-380,766-800 fraction threshold (bisection), soft threshold
+ *   src/recon_lib.cpp:2416-2527  L1_threshold (cycle-spun wavelet soft threshold)
  *
  * Pinned by tests/test_oracle_*.py (closed forms, brute-force DFT/NDFT on tiny
  * shapes, adjointness, special-case reductions, dense solves).
@@ -863,4 +866,240 @@ int orc_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ====================================================== L1 wavelet proximal step
+ * WAVELET3D (wavelet3D.cpp) with its default 4-tap Daubechies pair: lowpass
+ * h = ((1+r3), (3+r3), (3-r3), (1-r3)) / (4 r2) in reversed order, highpass its
+ * quadrature mirror, synthesis the time reverses (wavelet3D.cpp:143-190).
+ * The coefficients are computed here from that closed form. */
+#define ORC_WN 4
+static void d4_filters(real lpf[4], real hpf[4], real slpf[4], real shpf[4]) {
+  const double r3 = sqrt(3.0), n = 4.0 * sqrt(2.0);
+  const double h0 = (1 + r3) / n, h1 = (3 + r3) / n, h2 = (3 - r3) / n, h3 = (1 - r3) / n;
+  lpf[0] = (real)h3; lpf[1] = (real)h2; lpf[2] = (real)h1; lpf[3] = (real)h0;
+  hpf[0] = (real)-h0; hpf[1] = (real)h1; hpf[2] = (real)-h2; hpf[3] = (real)h3;
+  slpf[0] = (real)h0; slpf[1] = (real)h1; slpf[2] = (real)h2; slpf[3] = (real)h3;
+  shpf[0] = (real)h3; shpf[1] = (real)-h2; shpf[2] = (real)h1; shpf[3] = (real)-h0;
+}
+
+/* levels per axis: the requested count, lowered until N / 2^L is an integer
+ * (setup_wavelet_directions, wavelet3D.cpp:99-115) */
+void orc_wavelet_levels(const int32_t N[3], const int32_t req[3], int32_t L[3]) {
+  for (int a = 0; a < 3; a++) {
+    int l = req[a];
+    while (l > 0 && (N[a] % (1 << l)) != 0) l--;
+    L[a] = l;
+  }
+}
+
+/* index i modulo pts for any sign (the reference's (i + pts) % pts, which
+ * only stays in range for lines at least as long as the filter) */
+static int wmod(int i, int pts) { return ((i % pts) + pts) % pts; }
+
+/* wave1D / iwave1D (wavelet3D.cpp:754-791), periodic, shifted by sh */
+static void wave1d(const cpx *in, cpx *out, int pts, int sh, const real *lpf, const real *hpf) {
+  for (int pos = 0; pos < pts / 2; pos++) {
+    cpx lo = {0, 0}, hi = {0, 0};
+    for (int fp = 0; fp < ORC_WN; fp++) {
+      const int c = wmod(2 * pos - fp + sh, pts);
+      lo.re += lpf[fp] * in[c].re;
+      lo.im += lpf[fp] * in[c].im;
+      hi.re += hpf[fp] * in[c].re;
+      hi.im += hpf[fp] * in[c].im;
+    }
+    out[pos] = lo;
+    out[pos + pts / 2] = hi;
+  }
+}
+static void iwave1d(const cpx *in, cpx *out, int pts, int sh, const real *slpf, const real *shpf) {
+  for (int p = 0; p < pts; p++) out[p].re = out[p].im = 0;
+  for (int pos = 0; pos < pts / 2; pos++)
+    for (int fp = 0; fp < ORC_WN; fp++) {
+      const int c = wmod(2 * pos + fp - ORC_WN + 1 + sh, pts);
+      out[c].re += in[pos].re * slpf[fp];
+      out[c].im += in[pos].im * slpf[fp];
+      out[c].re += in[pos + pts / 2].re * shpf[fp];
+      out[c].im += in[pos + pts / 2].im * shpf[fp];
+    }
+}
+
+/* lines along axis a of the sub-block [0,sx) x [0,sy) x [0,sz) of X[Nz][Ny][Nx] */
+static void wave_axis(cpx *X, const int32_t N[3], int a, int sx, int sy, int sz, int sh, int inv) {
+  real lpf[4], hpf[4], slpf[4], shpf[4];
+  d4_filters(lpf, hpf, slpf, shpf);
+  const int sz3[3] = {sx, sy, sz};
+  const long st[3] = {1, N[0], (long)N[0] * N[1]};
+  const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+  const int pts = sz3[a];
+#pragma omp parallel for
+  for (int v = 0; v < sz3[c]; v++) {
+    cpx *in = (cpx *)malloc(sizeof(cpx) * (size_t)pts);
+    cpx *out = (cpx *)malloc(sizeof(cpx) * (size_t)pts);
+    for (int u = 0; u < sz3[b]; u++) {
+      cpx *line = X + (long)u * st[b] + (long)v * st[c];
+      for (int p = 0; p < pts; p++) in[p] = line[(long)p * st[a]];
+      if (inv) iwave1d(in, out, pts, sh, slpf, shpf);
+      else wave1d(in, out, pts, sh, lpf, hpf);
+      for (int p = 0; p < pts; p++) line[(long)p * st[a]] = out[p];
+    }
+    free(in);
+    free(out);
+  }
+}
+
+/* forward3D / backward3D (wavelet3D.cpp:563-631), in place; inv = 0 forward */
+void orc_wavelet3d(real *Xr, const int32_t N[3], const int32_t req_levels[3], const int32_t shift[3],
+                   int inv) {
+  cpx *X = (cpx *)Xr;
+  int32_t L[3];
+  orc_wavelet_levels(N, req_levels, L);
+  int maxl = 0;
+  for (int a = 0; a < 3; a++) maxl = L[a] > maxl ? L[a] : maxl;
+  if (!inv) {
+    int s[3] = {N[0], N[1], N[2]};
+    for (int l = 0; l < maxl; l++) {
+      if (l < L[0]) wave_axis(X, N, 0, s[0], s[1], s[2], shift[0], 0);
+      if (l < L[1]) wave_axis(X, N, 1, s[0], s[1], s[2], shift[1], 0);
+      if (l < L[2]) wave_axis(X, N, 2, s[0], s[1], s[2], shift[2], 0);
+      for (int a = 0; a < 3; a++)
+        if (l < L[a]) s[a] = N[a] >> (l + 1);
+    }
+  } else {
+    int s[3];
+    for (int a = 0; a < 3; a++) s[a] = N[a] >> L[a];
+    for (int l = maxl - 1; l >= 0; l--) {
+      for (int a = 0; a < 3; a++)
+        if (l < L[a]) s[a] = N[a] >> l;
+      if (l < L[2]) wave_axis(X, N, 2, s[0], s[1], s[2], shift[2], 1);
+      if (l < L[1]) wave_axis(X, N, 1, s[0], s[1], s[2], shift[1], 1);
+      if (l < L[0]) wave_axis(X, N, 0, s[0], s[1], s[2], shift[0], 1);
+    }
+  }
+}
+
+static real cabs_(cpx v) { return (real)sqrt((double)v.re * v.re + (double)v.im * v.im); }
+
+/* THRESHOLD::get_threshold for complex-grouped coefficients
+ * (threshold.cpp:250-380): bisection on the value for which
+ * count(|c| < value) is within 0.1% of fraction * total, at most 64 steps,
+ * starting at 0.5*(max-min) with the bracket [min, fraction*max]. */
+double orc_fraction_threshold(const real *Xr, long n, double fraction_d) {
+  const cpx *X = (const cpx *)Xr;
+  const real fraction = (real)fraction_d;
+  real mx = 0, mn = cabs_(X[0]);
+  for (long q = 0; q < n; q++) {
+    const real v = cabs_(X[q]);
+    mx = v > mx ? v : mx;
+    mn = v < mn ? v : mn;
+  }
+  const long total = n, target = (long)(fraction * (double)total);
+  const long accuracy = (long)(0.001 * (double)total);
+  if (target < 2) return 0.0;
+  real value = (real)0.5 * (mx - mn), max_t = fraction * mx, min_t = mn;
+  long found = total;
+  int iter = 0;
+  while (labs(found - target) > accuracy && iter < 64) {
+    if (iter > 0) {
+      if (found > target) max_t = value;
+      else min_t = value;
+      value = (real)0.5 * (max_t + min_t);
+    }
+    iter++;
+    found = 0;
+    for (long q = 0; q < n; q++)
+      if (cabs_(X[q]) < value) found++;
+  }
+  return value;
+}
+
+/* THRESHOLD::thresholding, complex grouped (threshold.cpp:766-800) */
+void orc_soft_threshold(real *Xr, long n, double value_d) {
+  cpx *X = (cpx *)Xr;
+  const real value = (real)value_d;
+  for (long q = 0; q < n; q++) {
+    const real cc = cabs_(X[q]);
+    if (cc <= value) {
+      X[q].re = X[q].im = 0;
+    } else {
+      const real f = (real)1 - value / cc;
+      X[q].re *= f;
+      X[q].im *= f;
+    }
+  }
+}
+
+/* RECON::L1_threshold with a wavelet transform, no phase rotation and no
+ * temporal transform (recon_lib.cpp:2416-2527): per cycle spin, forward
+ * wavelet of every frame, the threshold taken once (first spin) over all
+ * frames with scale 1/spins, soft threshold, backward wavelet.
+ * shifts[spin][3] are the per-axis wavelet shifts. Returns the threshold. */
+double orc_l1_wavelet(real *X, int frames, const int32_t N[3], const int32_t levels[3], double fraction,
+                      int spins, const int32_t *shifts) {
+  const long nv = (long)N[0] * N[1] * N[2];
+  real thr = 0;
+  for (int sp = 0; sp < spins; sp++) {
+    for (int f = 0; f < frames; f++) orc_wavelet3d(X + 2 * nv * f, N, levels, shifts + 3 * sp, 0);
+    if (sp == 0) thr = ((real)1 / (real)spins) * (real)orc_fraction_threshold(X, nv * frames, fraction);
+    orc_soft_threshold(X, nv * frames, thr);
+    for (int f = 0; f < frames; f++) orc_wavelet3d(X + 2 * nv * f, N, levels, shifts + 3 * sp, 1);
+  }
+  return thr;
+}
+
+/* Cycle-spin shift stream shared with the HIP library: splitmix64 from `seed`,
+ * shift = (next >> 33) % 4 per axis, three per spin (the reference draws
+ * rand() % wN, wavelet3D.cpp:443-449). */
+void orc_spin_shifts(uint64_t seed, long count, int32_t *out) {
+  uint64_t st = seed;
+  for (long q = 0; q < count; q++) {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[q] = (int32_t)((z >> 33) % ORC_WN);
+  }
+}
+
+/* FISTA with the L1 wavelet proximal step after each gradient step
+ * (recon_lib.cpp:1925-2150): shifts of iteration `it` are the 3*spins values
+ * starting at shifts + 3*spins*it. */
+void orc_fista_l1(orc_op *op, real *Xo, const real *smaps, int C, const real *data, const float *kx,
+                  const float *ky, const float *kz, const float *kw, long M, int iters, double step,
+                  double fraction, const int32_t levels[3], int spins, const int32_t *shifts,
+                  double *resid, double *thresh) {
+  long nv = nvox(op);
+  cpx *X = (cpx *)Xo;
+  cpx *Xold = (cpx *)calloc((size_t)nv, sizeof(cpx));
+  cpx *R = (cpx *)malloc(sizeof(cpx) * (size_t)nv);
+  real *diff = (real *)malloc(sizeof(cpx) * (size_t)M);
+  real st = (real)step;
+  memset(X, 0, sizeof(cpx) * (size_t)nv);
+  for (int it = 0; it < iters; it++) {
+    memset(R, 0, sizeof(cpx) * (size_t)nv);
+    real A = (real)(1.0 + (it - 1.0) / (it + 1.0));
+    real B = (real)(-(it - 1.0) / (it + 1.0));
+    for (long q = 0; q < nv; q++) {
+      cpx x0 = Xold[q], x1 = X[q];
+      X[q].re = A * x1.re + B * x0.re;
+      X[q].im = A * x1.im + B * x0.im;
+      Xold[q] = x1;
+    }
+    for (int c = 0; c < C; c++) {
+      orc_backward(op, (const real *)X, smaps ? smaps + 2 * nv * c : NULL, diff, kx, ky, kz, kw,
+                   M, data + 2 * M * c);
+      orc_forward(op, (real *)R, smaps ? smaps + 2 * nv * c : NULL, diff, kx, ky, kz, kw, M);
+    }
+    if (resid) resid[it] = energy(R, nv);
+    for (long q = 0; q < nv; q++) {
+      real rr = R[q].re * st, ri = R[q].im * st;
+      X[q].re -= rr;
+      X[q].im -= ri;
+    }
+    const double t = orc_l1_wavelet((real *)X, 1, op->N, levels, fraction, spins, shifts + 3 * spins * it);
+    if (thresh) thresh[it] = t;
+  }
+  free(Xold);
+  free(R);
+  free(diff);
 }

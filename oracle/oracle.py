@@ -64,6 +64,16 @@ def _lib(precision):
         lib.orc_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]
         lib.orc_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i64]
         lib.orc_forward_sos.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64]
+        lib.orc_wavelet_levels.argtypes = [vp, vp, vp]
+        lib.orc_wavelet3d.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+        lib.orc_fraction_threshold.argtypes = [vp, i64, dbl]
+        lib.orc_fraction_threshold.restype = dbl
+        lib.orc_soft_threshold.argtypes = [vp, i64, dbl]
+        lib.orc_l1_wavelet.argtypes = [vp, ctypes.c_int, vp, vp, dbl, ctypes.c_int, vp]
+        lib.orc_l1_wavelet.restype = dbl
+        lib.orc_spin_shifts.argtypes = [ctypes.c_uint64, i64, vp]
+        lib.orc_fista_l1.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int,
+                                     dbl, dbl, vp, ctypes.c_int, vp, vp, vp]
         lib.orc_stage_pre_fft.argtypes = [vp, vp, vp, vp]
         lib.orc_stage_gather.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, vp]
         lib.orc_stage_scatter.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64]
@@ -101,6 +111,65 @@ def fft1d(x, sign=-1, precision="f64"):
     n = a.shape[-1]
     _lib(precision).orc_fft1d(n, sign, a.ctypes.data, a.size // n)
     return a
+
+
+def _i3(v):
+    return np.ascontiguousarray(v, dtype=np.int32)
+
+
+def wavelet_levels(shape, levels=(4, 4, 4)):
+    """Levels per axis (x, y, z) the transform uses for an image of `shape`
+    (Nz, Ny, Nx)."""
+    N = _i3([shape[2], shape[1], shape[0]])
+    L = np.zeros(3, np.int32)
+    req = _i3(levels)  # arrays held in locals while C reads them
+    _lib("f64").orc_wavelet_levels(N.ctypes.data, req.ctypes.data, L.ctypes.data)
+    return tuple(int(v) for v in L)
+
+
+def wavelet3d(X, levels=(4, 4, 4), shift=(0, 0, 0), inverse=False, precision="f64"):
+    """WAVELET3D forward / backward of one image (Nz, Ny, Nx); returns a copy."""
+    ct = np.complex64 if precision == "f32" else np.complex128
+    Y = np.array(X, dtype=ct, order="C", copy=True)
+    N = _i3([Y.shape[2], Y.shape[1], Y.shape[0]])
+    lv, sh = _i3(levels), _i3(shift)
+    _lib(precision).orc_wavelet3d(Y.ctypes.data, N.ctypes.data, lv.ctypes.data, sh.ctypes.data,
+                                  int(bool(inverse)))
+    return Y
+
+
+def fraction_threshold(C, fraction, precision="f64"):
+    ct = np.complex64 if precision == "f32" else np.complex128
+    C = np.ascontiguousarray(C, dtype=ct)
+    return float(_lib(precision).orc_fraction_threshold(C.ctypes.data, C.size, float(fraction)))
+
+
+def soft_threshold(C, value, precision="f64"):
+    ct = np.complex64 if precision == "f32" else np.complex128
+    Y = np.array(C, dtype=ct, order="C", copy=True)
+    _lib(precision).orc_soft_threshold(Y.ctypes.data, Y.size, float(value))
+    return Y
+
+
+def l1_wavelet(X, fraction, levels=(4, 4, 4), shifts=((0, 0, 0),), precision="f64"):
+    """L1_threshold of frames X (F, Nz, Ny, Nx) with the given per-spin shifts;
+    returns (X', threshold)."""
+    ct = np.complex64 if precision == "f32" else np.complex128
+    Y = np.array(X, dtype=ct, order="C", copy=True)
+    if Y.ndim == 3:
+        Y = Y[None]
+    N = _i3([Y.shape[3], Y.shape[2], Y.shape[1]])
+    sh = _i3(np.asarray(shifts).reshape(-1, 3))
+    lv = _i3(levels)
+    t = _lib(precision).orc_l1_wavelet(Y.ctypes.data, Y.shape[0], N.ctypes.data, lv.ctypes.data,
+                                       float(fraction), sh.shape[0], sh.ctypes.data)
+    return Y, float(t)
+
+
+def spin_shifts(seed, count):
+    out = np.zeros(count, np.int32)
+    _lib("f64").orc_spin_shifts(int(seed), int(count), out.ctypes.data)
+    return out
 
 
 class OracleOp:
@@ -286,6 +355,22 @@ class OracleOp:
                                          kz.ctypes.data, kw.ctypes.data, kx.shape[0], iters,
                                          eig.ctypes.data)
         return step, eig
+
+    def fista_l1(self, smaps, data, traj, iters, step, fraction, levels=(4, 4, 4), spins=4, seed=0):
+        """FISTA with the L1 wavelet proximal step; cycle-spin shifts from
+        spin_shifts(seed, 3 * spins * iters).  Returns (X, resid, thresholds)."""
+        kx, ky, kz, kw = self._traj(traj)
+        data, smaps = self._c(data), self._c(smaps)
+        X = np.zeros((self.N[2], self.N[1], self.N[0]), self.ctype)
+        resid = np.zeros(iters)
+        thr = np.zeros(iters)
+        sh = spin_shifts(seed, 3 * spins * iters)
+        lv = _i3(levels)
+        self.lib.orc_fista_l1(self.h, X.ctypes.data, _p(smaps), data.shape[0], data.ctypes.data,
+                              kx.ctypes.data, ky.ctypes.data, kz.ctypes.data, kw.ctypes.data,
+                              kx.shape[0], iters, float(step), float(fraction), lv.ctypes.data,
+                              int(spins), sh.ctypes.data, resid.ctypes.data, thr.ctypes.data)
+        return X, resid, thr
 
     def fista(self, smaps, data, traj, iters, step):
         kx, ky, kz, kw = self._traj(traj)
