@@ -164,6 +164,34 @@ int mrn_power_method(mrn_op *op, int iters, double *step, void *stream);
 int mrn_fista(mrn_op *op, void *X, int iters, double step, int use_graph, double *resid,
               void *stream);
 
+/* 3-D periodic wavelet of `frames` images X[frames][Nz][Ny][Nx] in place
+ * (WAVELET3D::forward3D / backward3D, wavelet3D.cpp:563-631, with the
+ * reference's default 4-tap Daubechies pair, 143-190): levels per axis
+ * (x, y, z) are lowered until N / 2^L is an integer (99-115); shift[a] in
+ * 0..3 is the axis' circular shift (the cycle-spin shift, 443-449). */
+int mrn_wavelet(mrn_op *op, void *X, int frames, const int32_t levels[3], const int32_t shift[3],
+                int inverse, void *stream);
+
+/* The L1 wavelet proximal step RECON::L1_threshold (recon_lib.cpp:2416-2527)
+ * with the fraction threshold (THRESHOLD, threshold.cpp:193-380, 766-800) on
+ * X[frames][Nz][Ny][Nx]: per cycle spin, forward wavelet, the threshold taken
+ * on the first spin over all frames (scaled by 1/spins), complex soft
+ * threshold, backward wavelet.  shifts (host, [spins][3], nullable = drawn
+ * from the op's seeded stream); *threshold (host, nullable) receives the
+ * applied threshold. */
+int mrn_l1_wavelet(mrn_op *op, void *X, int frames, double fraction, const int32_t levels[3], int spins,
+                   const int32_t *shifts, double *threshold, void *stream);
+
+/* Make mrn_fista apply the L1 wavelet step after every gradient step
+ * (recon_lib.cpp:2145-2150); fraction 0 turns it off.  Cycle-spin shifts of
+ * iteration i are entries 3*spins*i .. 3*spins*(i+1)-1 of the splitmix64
+ * stream from `seed`, each (value >> 33) % 4 (the reference uses rand() % 4,
+ * whose sequence depends on every other rand() call of the program). */
+int mrn_set_fista_l1(mrn_op *op, double fraction, const int32_t levels[3], int spins, uint64_t seed);
+/* Host copy of the threshold the last mrn_fista applied at each of its first
+ * n iterations. */
+int mrn_get_fista_thresholds(mrn_op *op, double *out, int n, void *stream);
+
 /* Trajectory gradient: given X and the upstream gradient g[C][M] of the
  * k-space output y = A(X; k) of one frame (g = dL/dRe y + i dL/dIm y for a
  * real loss L), writes grad_k[M][3] (float32, reference k units)

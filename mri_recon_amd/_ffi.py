@@ -22,7 +22,8 @@ EXPORTS = [
     "mrn_image_to_kspace", "mrn_kspace_to_image", "mrn_normal", "mrn_coil_backward",
     "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_cg_init", "mrn_cg_iterate",
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
-    "mrn_coil_forward_sos", "mrn_set_k_rad",
+    "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
+    "mrn_get_fista_thresholds",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
 
@@ -86,6 +87,10 @@ def lib():
             "mrn_coil_forward": (i32, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
             "mrn_coil_forward_sos": (i32, [vp, vp, vp, i64, vp, vp, vp, vp, vp]),
             "mrn_set_k_rad": (i32, [vp, ctypes.c_float, vp]),
+            "mrn_wavelet": (i32, [vp, vp, i32, vp, vp, i32, vp]),
+            "mrn_l1_wavelet": (i32, [vp, vp, i32, ctypes.c_double, vp, i32, vp, vp, vp]),
+            "mrn_set_fista_l1": (i32, [vp, ctypes.c_double, vp, i32, ctypes.c_uint64]),
+            "mrn_get_fista_thresholds": (i32, [vp, vp, i32, vp]),
             "mrn_set_kdata": (i32, [vp, i32, vp, vp]),
             "mrn_cg": (i32, [vp, vp, i32, i32, vp, vp]),
             "mrn_cg_init": (i32, [vp, vp, vp]),

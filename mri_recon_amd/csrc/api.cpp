@@ -639,14 +639,65 @@ int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, doubl
   MRN_TRY(op_zero(op, X, n, st));
   MRN_TRY(op_zero(op, op->vP, n, st));  // X_old
   const float stf = (float)step;
+  const bool l1 = op->l1_fraction > 0.0f;
+  if (l1) MRN_TRY(ensure_l1_state(op));  // before any capture
   auto body = [&](hipStream_t s) -> int {
     MRN_TRY(op_fista_momentum(op, X, op->vP, n, s));
     for (int f = 0; f < op->F; f++)
       MRN_TRY(grad_frame(op, f, X + f * op->nvox, op->vR + f * op->nvox, s));
-    return op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s);
+    MRN_TRY(op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s));
+    // L1 wavelet soft threshold after the gradient step (recon_lib.cpp:2145-2150)
+    if (l1) MRN_TRY(l1_fista_step(op, X, s));
+    return MRN_OK;
   };
   MRN_TRY(run_iters(op, iters, use_graph && batched(op), st, body));
   if (resid && iters > 0) MRN_TRY(copy_trace(op, resid, 0, iters, 1, st));
+  return MRN_OK;
+}
+
+int mrn_wavelet(mrn_op *op, void *X, int frames, const int32_t levels[3], const int32_t shift[3], int inverse,
+                void *stream) {
+  if (!op || !X || !levels || !shift || frames < 1) return MRN_EINVAL;
+  for (int a = 0; a < 3; a++)
+    if (levels[a] < 0 || shift[a] < 0 || shift[a] > 3) return fail(op, MRN_EINVAL, "wavelet levels or shift");
+  return wavelet_host(op, (float2 *)X, frames, levels, shift, inverse != 0, pick(op, stream));
+}
+
+int mrn_l1_wavelet(mrn_op *op, void *X, int frames, double fraction, const int32_t levels[3], int spins,
+                   const int32_t *shifts, double *threshold, void *stream) {
+  if (!op || !X || !levels || frames < 1) return MRN_EINVAL;
+  if (!(fraction >= 0.0 && fraction <= 1.0) || spins < 1 || spins > MRN_MAX_SPINS)
+    return fail(op, MRN_EINVAL, "fraction in [0,1] and 1 <= spins <= 64");
+  for (int a = 0; a < 3; a++)
+    if (levels[a] < 0) return fail(op, MRN_EINVAL, "wavelet levels");
+  if (shifts)
+    for (int q = 0; q < 3 * spins; q++)
+      if (shifts[q] < 0 || shifts[q] > 3) return fail(op, MRN_EINVAL, "wavelet shifts are 0..3");
+  return l1_wavelet_host(op, (float2 *)X, frames, (float)fraction, levels, spins, shifts, threshold,
+                         pick(op, stream));
+}
+
+int mrn_set_fista_l1(mrn_op *op, double fraction, const int32_t levels[3], int spins, uint64_t seed) {
+  if (!op || !levels) return MRN_EINVAL;
+  if (!(fraction >= 0.0 && fraction <= 1.0) || spins < 1 || spins > MRN_MAX_SPINS)
+    return fail(op, MRN_EINVAL, "fraction in [0,1] and 1 <= spins <= 64");
+  for (int a = 0; a < 3; a++) {
+    if (levels[a] < 0) return fail(op, MRN_EINVAL, "wavelet levels");
+    op->l1_levels[a] = levels[a];
+  }
+  op->l1_fraction = (float)fraction;
+  op->l1_spins = spins;
+  op->l1_seed = seed;
+  op->version++;
+  return MRN_OK;
+}
+
+int mrn_get_fista_thresholds(mrn_op *op, double *out, int n, void *stream) {
+  if (!op || !out || n < 0 || n > MRN_TRACE_CAP) return MRN_EINVAL;
+  if (!op->dthr_log) return fail(op, MRN_ESTATE, "no L1 FISTA run yet");
+  hipStream_t st = pick(op, stream);
+  MRN_HIP(op, hipMemcpyAsync(out, op->dthr_log, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  MRN_HIP(op, hipStreamSynchronize(st));
   return MRN_OK;
 }
 

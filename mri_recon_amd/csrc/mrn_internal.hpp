@@ -22,6 +22,7 @@
 #define FFT_THREADS 256
 #define FFT_PAD 1            // LDS line padding (float2) against bank conflicts
 #define MRN_TRACE_CAP 65536  // solver trace ring (iterations)
+#define MRN_MAX_SPINS 64     // wavelet cycle spins per L1 step
 
 namespace mrn {
 
@@ -114,6 +115,14 @@ struct mrn_op {
   std::string err;
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
+  // L1 wavelet proximal step of FISTA (fraction 0 = none)
+  float l1_fraction = 0.0f;
+  int l1_levels[3] = {4, 4, 4};
+  int l1_spins = 4;
+  uint64_t l1_seed = 0;
+  void *dthr = nullptr;        // threshold search state
+  int *dsh = nullptr;          // cycle-spin shifts [MRN_MAX_SPINS][3]
+  double *dthr_log = nullptr;  // threshold per FISTA iteration
 };
 
 namespace mrn {
@@ -237,5 +246,11 @@ int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
 int fft_post_z(mrn_op *op, const float2 *grid, float2 *X, const float2 *smaps,
                const PrePostArgs &a, hipStream_t st);
 bool fft_size_supported(int S);
+int wavelet_host(mrn_op *op, float2 *X, int frames, const int levels[3], const int32_t shift[3], bool inv,
+                 hipStream_t st);
+int l1_wavelet_host(mrn_op *op, float2 *X, int frames, float fraction, const int levels[3], int spins,
+                    const int32_t *shifts, double *thr, hipStream_t st);
+int l1_fista_step(mrn_op *op, float2 *X, hipStream_t st);
+int ensure_l1_state(mrn_op *op);
 
 }  // namespace mrn

@@ -394,6 +394,9 @@ int mrn_op_destroy(mrn_op *op) {
   hipFree(op->dscal);
   hipFree(op->dtrace);
   hipFree(op->dcounter);
+  hipFree(op->dthr);
+  hipFree(op->dsh);
+  hipFree(op->dthr_log);
   if (op->stream) hipStreamDestroy(op->stream);
   delete op;
   return MRN_OK;

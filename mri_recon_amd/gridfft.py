@@ -273,6 +273,41 @@ class GridFFT:
         self._c(lib().mrn_fista(self._h, _ptr(X), int(iters), float(step), int(bool(graph)), res, _stream()))
         return X, torch.tensor(list(res), dtype=torch.float64)[:iters]
 
+    # ------------------------------------------------------ L1 wavelet step
+    def wavelet(self, X, levels=(4, 4, 4), shift=(0, 0, 0), inverse=False):
+        """In-place 3-D Daubechies-4 wavelet of X (F, Nz, Ny, Nx) or (Nz, Ny, Nx)."""
+        frames = X.shape[0] if X.dim() == 4 else 1
+        _need(X, torch.complex64, (frames,) + self.image_shape if X.dim() == 4 else self.image_shape, "X")
+        lv = (ctypes.c_int32 * 3)(*levels)
+        sh = (ctypes.c_int32 * 3)(*shift)
+        self._c(lib().mrn_wavelet(self._h, _ptr(X), frames, lv, sh, int(bool(inverse)), _stream()))
+        return X
+
+    def l1_wavelet(self, X, fraction, levels=(4, 4, 4), spins=4, shifts=None):
+        """The L1 wavelet proximal step on X (F, Nz, Ny, Nx), in place; returns
+        the applied threshold.  shifts: (spins, 3) ints in 0..3 or None."""
+        frames = X.shape[0] if X.dim() == 4 else 1
+        lv = (ctypes.c_int32 * 3)(*levels)
+        sh = None
+        if shifts is not None:
+            flat = [int(v) for row in shifts for v in row]
+            sh = (ctypes.c_int32 * len(flat))(*flat)
+            spins = len(flat) // 3
+        thr = ctypes.c_double()
+        self._c(lib().mrn_l1_wavelet(self._h, _ptr(X), frames, float(fraction), lv, int(spins), sh,
+                                     ctypes.byref(thr), _stream()))
+        return thr.value
+
+    def set_fista_l1(self, fraction, levels=(4, 4, 4), spins=4, seed=0):
+        """Apply the L1 wavelet step after each FISTA gradient step (0 = off)."""
+        lv = (ctypes.c_int32 * 3)(*levels)
+        self._c(lib().mrn_set_fista_l1(self._h, float(fraction), lv, int(spins), ctypes.c_uint64(seed)))
+
+    def fista_thresholds(self, n):
+        out = (ctypes.c_double * max(n, 1))()
+        self._c(lib().mrn_get_fista_thresholds(self._h, out, int(n), _stream()))
+        return torch.tensor(list(out)[:n], dtype=torch.float64)
+
     # ------------------------------------------------------------- gradient
     def traj_grad(self, X, g, frame=0):
         """grad_k[m, a] = Re sum_c conj(g[c, m]) dA_c X[m]/dk_a  -> (M, 3) float32."""
