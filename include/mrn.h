@@ -68,7 +68,8 @@ typedef struct mrn_params {
   int32_t bin_edge;      /* sample sort bin edge in cells (1 or 2); 0 = auto */
   int32_t xcd_run;       /* scatter work items per XCD run (1 = hardware order); 0 = auto */
   int32_t xcd_run_gather;/* gather workgroups per XCD run; 0 = auto */
-  int32_t reserved[4];
+  int32_t gather_tiles;  /* 1 = stage dense tiles of the grid in LDS for the gather (default 0) */
+  int32_t reserved[3];
 } mrn_params;
 
 typedef struct mrn_op mrn_op;

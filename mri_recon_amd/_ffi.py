@@ -49,7 +49,8 @@ class MrnParams(ctypes.Structure):
         ("bin_edge", ctypes.c_int32),
         ("xcd_run", ctypes.c_int32),
         ("xcd_run_gather", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("gather_tiles", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 

@@ -135,6 +135,63 @@ __global__ void k_fill_subs(const int32_t *__restrict__ nsub, const int32_t *__r
   heavy_flag[tile] = n > 1;
 }
 
+// Gather work list: the samples whose first tap lies in tile t (its "home"
+// samples: bins [t*T/B, (t*T+T-1)/B] per axis, one contiguous run per (bz, by)
+// row), in items of at most `per` samples.  Tiles without samples get none.
+__host__ __device__ __forceinline__ void home_bins(const Tiles &t, int a, int ti, int &b0, int &b1) {
+  b0 = ti * t.T / t.B;
+  b1 = (ti * t.T + t.T - 1) / t.B;
+  if (b1 > t.nb[a] - 1) b1 = t.nb[a] - 1;
+}
+
+__global__ void k_gtile_count(const int32_t *__restrict__ bin_start, Tiles t, int per, int min_dense,
+                              int32_t *nsub, int32_t *count, int32_t *nsparse) {
+  int tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= t.ntiles) return;
+  const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
+  int bx0, bx1, by0, by1, bz0, bz1;
+  home_bins(t, 0, tx, bx0, bx1);
+  home_bins(t, 1, ty, by0, by1);
+  home_bins(t, 2, tz, bz0, bz1);
+  int v = 0;
+  for (int bz = bz0; bz <= bz1; bz++)
+    for (int by = by0; by <= by1; by++) {
+      const int b = (bz * t.nb[1] + by) * t.nb[0];
+      v += bin_start[b + bx1 + 1] - bin_start[b + bx0];
+    }
+  count[tile] = v;
+  const bool dense = v >= min_dense;
+  nsub[tile] = dense ? (v + per - 1) / per : 0;
+  nsparse[tile] = dense ? 0 : v;
+}
+
+// internal indices of the home samples of the sparse tiles, tile by tile
+__global__ void k_gtile_sparse(const int32_t *__restrict__ bin_start, Tiles t,
+                               const int32_t *__restrict__ nsparse, const int32_t *__restrict__ soff,
+                               int32_t *idx) {
+  int tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= t.ntiles || nsparse[tile] == 0) return;
+  const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
+  int bx0, bx1, by0, by1, bz0, bz1;
+  home_bins(t, 0, tx, bx0, bx1);
+  home_bins(t, 1, ty, by0, by1);
+  home_bins(t, 2, tz, bz0, bz1);
+  int o = soff[tile];
+  for (int bz = bz0; bz <= bz1; bz++)
+    for (int by = by0; by <= by1; by++) {
+      const int b = (bz * t.nb[1] + by) * t.nb[0];
+      for (int q = bin_start[b + bx0]; q < bin_start[b + bx1 + 1]; q++) idx[o++] = q;
+    }
+}
+
+__global__ void k_gtile_fill(const int32_t *__restrict__ nsub, const int32_t *__restrict__ off,
+                             const int32_t *__restrict__ count, Tiles t, int per, int4 *gsubs) {
+  int tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= t.ntiles) return;
+  const int n = nsub[tile], o = off[tile], v = count[tile];
+  for (int q = 0; q < n; q++) gsubs[o + q] = make_int4(tile, q * per, min((q + 1) * per, v), 0);
+}
+
 __global__ void k_compact(const int32_t *__restrict__ flag, const int32_t *__restrict__ off, int n,
                           int32_t *out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,15 +235,21 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
                                                 const float *__restrict__ tables,
                                                 int64_t chunk_grid, int64_t chunk_k,
                                                 const float *__restrict__ taper, int weighted,
-                                                int xrun) {
+                                                int xrun, const int32_t *__restrict__ list,
+                                                int64_t nlist) {
   extern __shared__ float tab_s[];
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
   const int64_t gid = (int64_t)xcd_block(blockIdx.x, gridDim.x, xrun) * blockDim.x + threadIdx.x;
   const int lgl = g.lgCC - (NC == 2 ? 1 : 0);  // log2 of the lanes per sample
   const int c = (int)(gid & ((1 << lgl) - 1)) * NC;
-  const int64_t i = gid >> lgl;
-  if (i >= Mv) return;
+  int64_t i = gid >> lgl;  // sample (or position in `list` when given)
+  if (list) {
+    if (i >= nlist) return;
+    i = list[i];
+  } else if (i >= Mv) {
+    return;
+  }
   const int ch = blockIdx.y;
   const float4 k = coords[i];
   const float dk[3] = {k.x, k.y, k.z};
@@ -263,6 +326,166 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
       ai[q] = (ai[q] * k.w) * tp;
     }
     out[oi] = make_float2(ar[q], ai[q]);
+  }
+}
+
+// Tile-staged gather.  Work item gsubs[blockIdx.x] = {tile, first, end} over
+// the tile's home samples (first tap inside the 4x4x4 tile).  The workgroup
+// loads the R^3 cells (R = 4 + NT - 1) that those samples' taps can reach, for
+// coil block blockIdx.y, into LDS with coalesced 16-byte loads, then every
+// home sample reads its NT^3 taps from LDS: lane = (sample, coil pair), so a
+// wavefront serves 64 / (CCT/2) samples per pass.  A sample with more than NT
+// taps on an axis (it sits exactly on the window edge) reads its taps from the
+// grid instead.  Same arithmetic and order per coil as k_gather.
+#define GT_PER 2048  // home samples per work item
+#define GT_MIN 64    // tiles with fewer home samples are gathered directly
+template <int NT, int CCT>
+__global__ __launch_bounds__(256) void k_gather_tile(const float2 *__restrict__ grid, float2 *__restrict__ out,
+                                                     const float4 *__restrict__ coords,
+                                                     const float2 *__restrict__ sub,
+                                                     const int32_t *__restrict__ bin_start,
+                                                     const int4 *__restrict__ gsubs, Geom g, Tiles t,
+                                                     const float *__restrict__ tables, int64_t chunk_grid,
+                                                     int64_t chunk_k, const float *__restrict__ taper,
+                                                     int weighted) {
+  constexpr int R = NT + 3, HC = CCT / 2;  // region edge, float4 per cell
+  constexpr int LPS = HC;                  // lanes per sample
+  constexpr int SPP = 256 / LPS;           // samples per pass
+  extern __shared__ float4 reg[];          // [R][R][R][HC]
+  __shared__ int s_rs[32], s_rp[33];
+  const int4 sp = gsubs[blockIdx.x];
+  const int tile = sp.x;
+  const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
+  const int t0[3] = {tix * 4, tiy * 4, tiz * 4};
+  const int ch = blockIdx.y;
+  const float4 *G4 = reinterpret_cast<const float4 *>(grid + ch * chunk_grid);
+  // ---- stage the region: every load of a thread issued before its LDS writes
+  constexpr int NE = R * R * R * HC, RN = (NE + 255) / 256;
+  {
+    float4 v[RN];
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+      const int e = threadIdx.x + 256 * j;
+      const int q = e % HC, cell = e / HC;
+      const int rx = cell % R, ry = (cell / R) % R, rz = cell / (R * R);
+      const int gx = t0[0] + rx, gy = t0[1] + ry, gz = t0[2] + rz;
+      v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e < NE && gx < g.S[0] && gy < g.S[1] && gz < g.S[2])
+        v[j] = G4[(((int64_t)gz * g.S[1] + gy) * g.S[0] + gx) * HC + q];
+    }
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+      const int e = threadIdx.x + 256 * j;
+      if (e < NE) reg[e] = v[j];
+    }
+  }
+  // ---- home rows of the tile and their prefix counts
+  int bx0, bx1, by0, by1, bz0, bz1;
+  home_bins(t, 0, tix, bx0, bx1);
+  home_bins(t, 1, tiy, by0, by1);
+  home_bins(t, 2, tiz, bz0, bz1);
+  const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
+  if (threadIdx.x < 64) {
+    const int r = threadIdx.x;
+    int n = 0;
+    if (r < nrows) {
+      const int brow = ((bz0 + r / nry) * t.nb[1] + by0 + r % nry) * t.nb[0];
+      const int s0 = bin_start[brow + bx0];
+      n = bin_start[brow + bx1 + 1] - s0;
+      s_rs[r] = s0;
+    }
+    int x = n;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (r >= o) x += y;
+    }
+    if (r < nrows) s_rp[r + 1] = x;
+    if (r == 0) s_rp[0] = 0;
+  }
+  __syncthreads();
+  const int cq = threadIdx.x % LPS, c = cq * 2;
+  for (int base = sp.y; base < sp.z; base += SPP) {
+    const int v = base + threadIdx.x / LPS;
+    if (v >= sp.z) continue;
+    int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_rp[mid] <= v) lo = mid;
+      else hi = mid - 1;
+    }
+    const int64_t i = (int64_t)s_rs[lo] + (v - s_rp[lo]);
+    const float4 k = coords[i];
+    const float dk[3] = {k.x, k.y, k.z};
+    int s3[3], n3[3];
+    bool local = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      int e;
+      axis_range(g, a, dk[a], s3[a], e);
+      n3[a] = e - s3[a] + 1;
+      local = local && n3[a] <= NT && s3[a] - t0[a] >= 0 && s3[a] - t0[a] + n3[a] <= R;
+    }
+    float ar[2] = {0.f, 0.f}, ai[2] = {0.f, 0.f};
+    if (local) {
+      float w[3][NT];
+      int o[3][NT];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int tt = 0; tt < NT; tt++) {
+          const bool in = tt < n3[a];
+          w[a][tt] = in ? tap_weight(g, a, tables + g.toff[a], dk[a], s3[a] + tt) : 0.0f;
+          o[a][tt] = (s3[a] - t0[a]) + (in ? tt : 0);
+        }
+#pragma unroll
+      for (int tz = 0; tz < NT; tz++)
+#pragma unroll
+        for (int ty = 0; ty < NT; ty++) {
+          const float wzy = w[2][tz] * w[1][ty];
+          const float4 *row = reg + ((o[2][tz] * R + o[1][ty]) * R) * HC + cq;
+#pragma unroll
+          for (int tx = 0; tx < NT; tx++) {
+            const float ww = wzy * w[0][tx];
+            const float4 q = row[o[0][tx] * HC];
+            ar[0] += ww * q.x;
+            ai[0] += ww * q.y;
+            ar[1] += ww * q.z;
+            ai[1] += ww * q.w;
+          }
+        }
+    } else {  // taps outside the staged region: read the grid
+      const float2 *Gc = grid + ch * chunk_grid + c;
+      for (int lz = s3[2]; lz < s3[2] + n3[2]; lz++) {
+        const float wz = tap_weight(g, 2, tables + g.toff[2], dk[2], lz);
+        for (int ly = s3[1]; ly < s3[1] + n3[1]; ly++) {
+          const float wzy = wz * tap_weight(g, 1, tables + g.toff[1], dk[1], ly);
+          const float2 *row = Gc + ((int64_t)lz * g.S[1] + ly) * g.S[0] * CCT;
+          for (int lx = s3[0]; lx < s3[0] + n3[0]; lx++) {
+            const float ww = wzy * tap_weight(g, 0, tables + g.toff[0], dk[0], lx);
+            const float4 q = *reinterpret_cast<const float4 *>(row + (int64_t)lx * CCT);
+            ar[0] += ww * q.x;
+            ai[0] += ww * q.y;
+            ar[1] += ww * q.z;
+            ai[1] += ww * q.w;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int qq = 0; qq < 2; qq++) {
+      const int64_t oi = ch * chunk_k + i * CCT + c + qq;
+      if (sub) {
+        const float2 d = sub[oi];
+        ar[qq] -= d.x;
+        ai[qq] -= d.y;
+      }
+      if (weighted) {
+        const float tp = taper[i];
+        ar[qq] = (ar[qq] * k.w) * tp;
+        ai[qq] = (ai[qq] * k.w) * tp;
+      }
+      out[oi] = make_float2(ar[qq], ai[qq]);
+    }
   }
 }
 
@@ -548,20 +771,48 @@ int taps_main(const Geom &g) {
   return nt;
 }
 
-int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
-                  const float2 *sub, bool weighted, hipStream_t st) {
+// LDS bytes of the tile-staged gather (0 when it does not apply)
+static size_t gather_tile_lds(const mrn_op *op, int nt) {
   const Geom &g = op->g;
-  if (fr.Mv == 0) return MRN_OK;
-  const int nc = g.CC >= 2 ? 2 : 1;  // coils per lane
-  dim3 g3(nblk(fr.Mv * (g.CC / nc), 256), op->nch);
+  if ((g.CC != 8 && g.CC != 16) || nt < 3 || nt > 6 || op->t.T != 4 || (op->t.T % op->t.B) != 0) return 0;
+  const size_t R = (size_t)nt + 3;
+  const size_t bytes = R * R * R * (size_t)g.CC * sizeof(float2);
+  return bytes <= 96 * 1024 ? bytes : 0;
+}
+
+template <int NT, int CCT>
+static int launch_gather_tile(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
+                              const float2 *sub, int wflag, size_t lds, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    MRN_HIP(op, hipFuncSetAttribute((const void *)k_gather_tile<NT, CCT>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+    attr = true;
+  }
+  const Geom &g = op->g;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
-  const int wflag = weighted ? 1 : 0;
+  hipLaunchKernelGGL((k_gather_tile<NT, CCT>), dim3(fr.ngsub, op->nch), dim3(256), lds, st, grid, out, fr.coords,
+                     sub, fr.bin_start, fr.gsubs, g, op->t, op->tables_d, cg, ck, fr.taper, wflag);
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
+// Direct gather over all samples, or over the samples listed in `list`.
+static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
+                                const float2 *sub, int wflag, const int32_t *list, int64_t nlist,
+                                hipStream_t st) {
+  const Geom &g = op->g;
+  const int nc = g.CC >= 2 ? 2 : 1;  // coils per lane
+  const int64_t n = list ? nlist : fr.Mv;
+  if (n == 0) return MRN_OK;
+  dim3 g3(nblk(n * (g.CC / nc), 256), op->nch);
+  int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   switch (taps_main(g) * 2 + (nc - 1)) {
 #define MRN_G1(n, c)                                                                                \
   case n * 2 + c - 1:                                                                               \
     hipLaunchKernelGGL((k_gather<n, c>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,     \
                        fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag,            \
-                       op->t.xrun_gather);                                                          \
+                       op->t.xrun_gather, list, nlist);                                             \
     break;
 #define MRN_G(n) MRN_G1(n, 1) MRN_G1(n, 2)
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
@@ -572,6 +823,37 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
   }
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
+}
+
+// Gather: the tile-staged kernel over the dense tiles and the direct kernel
+// over the samples of the sparse ones, or the direct kernel over everything.
+int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
+                  const float2 *sub, bool weighted, hipStream_t st) {
+  const Geom &g = op->g;
+  if (fr.Mv == 0) return MRN_OK;
+  const int wf = weighted ? 1 : 0;
+  const int nt = taps_main(g);
+  const size_t lds = gather_tile_lds(op, nt);
+  if (lds && fr.ngsub > 0 && op->gather_tiles) {
+    int rc = MRN_EUNSUPPORTED;
+    switch (nt * 100 + g.CC) {
+      case 308: rc = launch_gather_tile<3, 8>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 316: rc = launch_gather_tile<3, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 408: rc = launch_gather_tile<4, 8>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 416: rc = launch_gather_tile<4, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 508: rc = launch_gather_tile<5, 8>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 516: rc = launch_gather_tile<5, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 608: rc = launch_gather_tile<6, 8>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 616: rc = launch_gather_tile<6, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      default: break;
+    }
+    if (rc != MRN_EUNSUPPORTED) {
+      if (rc != MRN_OK) return rc;
+      return fr.nsparse > 0 ? launch_gather_direct(op, grid, out, fr, sub, wf, fr.sparse, fr.nsparse, st)
+                            : MRN_OK;
+    }
+  }
+  return launch_gather_direct(op, grid, out, fr, sub, wf, nullptr, 0, st);
 }
 
 // Zero the tiles that several work items flush with atomics (part of the scatter).
@@ -702,6 +984,12 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipFree(fr.bin_start);
   hipFree(fr.subs);
   hipFree(fr.heavy);
+  hipFree(fr.gsubs);
+  hipFree(fr.sparse);
+  fr.gsubs = nullptr;
+  fr.sparse = nullptr;
+  fr.ngsub = 0;
+  fr.nsparse = 0;
   fr.coords = nullptr;
   fr.taper = nullptr;
   fr.perm = nullptr;
@@ -781,6 +1069,33 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   MRN_BH(hipMalloc(&fr.heavy, sizeof(int32_t) * (nh > 0 ? nh : 1)));
   if (nh > 0) {
     hipLaunchKernelGGL(k_compact, dim3(nblk(nt, 256)), dim3(256), 0, st, hflag, hoff, nt, fr.heavy);
+    MRN_BH(hipGetLastError());
+  }
+  // gather work list (home samples per tile), reusing the scratch arrays
+  MRN_BH(hipStreamSynchronize(st));
+  hipLaunchKernelGGL(k_gtile_count, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, GT_PER, GT_MIN,
+                     nsub, visits, hflag);
+  MRN_BH(hipGetLastError());
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nsub, off, nt + 1, st));
+  int32_t ng = 0;
+  MRN_BH(hipMemcpyAsync(&ng, off + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MRN_BH(hipStreamSynchronize(st));
+  fr.ngsub = ng;
+  MRN_BH(hipMalloc(&fr.gsubs, sizeof(int4) * (ng > 0 ? ng : 1)));
+  if (ng > 0) {
+    hipLaunchKernelGGL(k_gtile_fill, dim3(nblk(nt, 256)), dim3(256), 0, st, nsub, off, visits, t, GT_PER, fr.gsubs);
+    MRN_BH(hipGetLastError());
+  }
+  // samples of the sparse tiles, for the direct gather
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, hflag, hoff, nt + 1, st));
+  int32_t nsp = 0;
+  MRN_BH(hipMemcpyAsync(&nsp, hoff + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MRN_BH(hipStreamSynchronize(st));
+  fr.nsparse = nsp;
+  MRN_BH(hipMalloc(&fr.sparse, sizeof(int32_t) * (nsp > 0 ? nsp : 1)));
+  if (nsp > 0) {
+    hipLaunchKernelGGL(k_gtile_sparse, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, hflag, hoff,
+                       fr.sparse);
     MRN_BH(hipGetLastError());
   }
   MRN_BH(hipStreamSynchronize(st));

@@ -71,6 +71,10 @@ struct FrameData {
   int4 *subs = nullptr;        // [nsub] {tile, first visit, end visit, heavy}
   int32_t *heavy = nullptr;    // [nheavy] tiles split over several work items
   int nsub = 0, nheavy = 0;
+  int4 *gsubs = nullptr;       // [ngsub] gather work items {tile, first, end, 0} (home samples)
+  int ngsub = 0;
+  int32_t *sparse = nullptr;   // [nsparse] samples of tiles below the dense limit (direct gather)
+  int64_t nsparse = 0;
   float2 *kdata = nullptr;     // measured k-space, internal layout [nch][Mv][CC]
   float2 *kraw = nullptr;      // measured k-space, caller layout [C][M] (plain form)
   float *traw = nullptr;       // trajectory kx|ky|kz|kw, caller order [4][M]
@@ -116,6 +120,7 @@ struct mrn_op {
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   // L1 wavelet proximal step of FISTA (fraction 0 = none)
+  bool gather_tiles = false;   // tile-staged gather over the dense tiles
   float l1_fraction = 0.0f;
   int l1_levels[3] = {4, 4, 4};
   int l1_spins = 4;

@@ -298,6 +298,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->t.maxv = p.max_visits > 0 ? p.max_visits : 4096;
   op->t.xrun = p.xcd_run > 0 ? p.xcd_run : 1;
   op->t.xrun_gather = p.xcd_run_gather > 0 ? p.xcd_run_gather : 64;
+  op->gather_tiles = p.gather_tiles != 0;
   if (4 * (g.ntab + 4) > 64 * 1024) {
     delete op;
     return MRN_EUNSUPPORTED;
@@ -370,6 +371,7 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.bin_start);
     hipFree(fr.subs);
     hipFree(fr.heavy);
+    hipFree(fr.gsubs);
     hipFree(fr.kdata);
     hipFree(fr.kraw);
     hipFree(fr.traw);
