@@ -50,7 +50,8 @@ class MrnParams(ctypes.Structure):
         ("xcd_run", ctypes.c_int32),
         ("xcd_run_gather", ctypes.c_int32),
         ("gather_tiles", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("scatter_valu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

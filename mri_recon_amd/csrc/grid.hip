@@ -526,6 +526,17 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 // Per sample the wavefront thus evaluates 12 weights instead of 192, and a
 // sample that misses the tile costs only its share of step A.  The sums
 // accumulate in visit order, as a lane-per-sample walk would.
+//
+// MM (CCT = 16): step B is the product T[64 cells][32] += W[64][16] D[16][32]
+// of the batch, W the cell weights of its samples and D their 16 coil values
+// as 32 reals, on the matrix cores: per pair of samples two
+// v_mfma_f32_32x32x2f32 (cells 0-31 and 32-63), A = W column pair (lane
+// i + 32k holds W[cell i][sample k]), B = D row pair (lane j + 32k holds
+// D[sample k][j]).  The VALU form spends 32 FMAs per sample on all 64 lanes
+// while only about a quarter of the cells are taps of the sample, and is
+// bound by that issue rate; the matrix cores do the same 2048 products at
+// twice the FP32 rate, and the VALU is left with the weights.  Sums are
+// FP32 throughout; only the order of the additions differs.
 #define SG_BATCH 16
 #define SG_MAXROWS 144
 // Make this wavefront's LDS writes visible to its other lanes (LDS executes a
@@ -535,8 +546,9 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
-template <int CCT>
-__global__ __launch_bounds__(256) void k_scatter_gd(
+typedef float mrn_v16f __attribute__((ext_vector_type(16)));
+template <int CCT, bool MM>
+__global__ __launch_bounds__(256, 5) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int32_t *__restrict__ bin_start, const int4 *__restrict__ subs, int nsub, Geom g,
     Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
@@ -583,9 +595,11 @@ __global__ __launch_bounds__(256) void k_scatter_gd(
   if (lane == 0) s_rp[wave][0] = 0;
   wave_sync();
   const int vbeg = max(sp.y, 0), vend = min(sp.z, carry);
-  float ar[CCT], ai[CCT];
+  static_assert(!MM || CCT == 16, "matrix-core scatter needs 16 coils per cell");
+  float ar[MM ? 1 : CCT], ai[MM ? 1 : CCT];
 #pragma unroll
-  for (int c = 0; c < CCT; c++) ar[c] = ai[c] = 0.f;
+  for (int c = 0; c < (MM ? 1 : CCT); c++) ar[c] = ai[c] = 0.f;
+  mrn_v16f acc0 = {}, acc1 = {};  // MM: cells 0-31 and 32-63
   const int sl = lane >> 2, q = lane & 3;
   constexpr int CPL = CCT >= 4 ? CCT / 4 : 1;  // coil values a lane stages
   for (int base = vbeg; base < vend; base += SG_BATCH) {
@@ -618,12 +632,29 @@ __global__ __launch_bounds__(256) void k_scatter_gd(
       hit &= m;  // bit 4s: sample s has a tap in the tile on every axis so far
     }
     hit &= 0x1111111111111111ull;
-    if (ok && q * CPL < CCT) {
+    if (q * CPL < CCT && (ok || MM)) {  // MM reads every sample of the batch: zeros past the end
 #pragma unroll
-      for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = kch[i * CCT + q * CPL + c];
+      for (int c = 0; c < CPL; c++)
+        s_d[wave][sl * CCT + q * CPL + c] = ok ? kch[i * CCT + q * CPL + c] : make_float2(0.f, 0.f);
     }
     wave_sync();
     // ---- B: accumulate the samples that reach the tile
+    if constexpr (MM) {
+      const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
+      const int c0 = lane & 31, c1 = c0 + 32;
+#pragma unroll
+      for (int kt = 0; kt < SG_BATCH / 2; kt++) {
+        if (((hit >> (8 * kt)) & 0x11ull) == 0) continue;  // neither sample reaches the tile
+        const int smp = 2 * kt + (lane >> 5);
+        const float *wp = &s_w[wave][smp * 12];
+        const float b = dflat[smp * 2 * CCT + c0];
+        const float a0 = (wp[8 + (c0 >> 4)] * wp[4 + ((c0 >> 2) & 3)]) * wp[c0 & 3];
+        const float a1 = (wp[8 + (c1 >> 4)] * wp[4 + ((c1 >> 2) & 3)]) * wp[c1 & 3];
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b, acc1, 0, 0, 0);
+      }
+      hit = 0;
+    }
     while (hit) {
       const int s = __builtin_ctzll(hit) >> 2;
       hit &= hit - 1;
@@ -638,6 +669,28 @@ __global__ __launch_bounds__(256) void k_scatter_gd(
       }
     }
     wave_sync();
+  }
+  if constexpr (MM) {
+    // accumulator register r of lane l: cell 8*(r/4) + 4*(l/32) + r%4 (+32 for
+    // acc1), real number l%32 of the cell's 16 coils
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int cell = 32 * h + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int cx = t0[0] + (cell & 3), cy = t0[1] + ((cell >> 2) & 3), cz = t0[2] + (cell >> 4);
+        if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) continue;
+        float *dst = reinterpret_cast<float *>(grid + ch * chunk_grid +
+                                               (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT) +
+                     (lane & 31);
+        const float v = h ? acc1[r] : acc0[r];
+        if (sp.w) {
+          if (v != 0.f) atomicAdd(dst, v);
+        } else {
+          *dst = v;
+        }
+      }
+    return;
   }
   const int cx = t0[0] + lx, cy = t0[1] + ly, cz = t0[2] + lz;
   if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) return;
@@ -878,10 +931,16 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   const size_t sh = 0;
   const int rows_axis = (t.T + t.NT - 2) / t.B + 2;  // bins per axis a tile can reach
   if (rows_axis * rows_axis > SG_MAXROWS) return fail(op, MRN_EUNSUPPORTED, "scatter neighbourhood too large");
+  if (g.CC == 16 && !op->scatter_valu) {
+    hipLaunchKernelGGL((k_scatter_gd<16, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
+                       fr.subs, fr.nsub, g, t, op->tables_d, cg, ck);
+    MRN_HIP(op, hipGetLastError());
+    return MRN_OK;
+  }
   switch (g.CC) {
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
-    hipLaunchKernelGGL((k_scatter_gd<n>), g3, dim3(256), sh, st, grid, kin, fr.coords,           \
+    hipLaunchKernelGGL((k_scatter_gd<n, false>), g3, dim3(256), sh, st, grid, kin, fr.coords,    \
                        fr.bin_start, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck);              \
     break;
     MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)

@@ -121,6 +121,7 @@ struct mrn_op {
   size_t scan_tmp_bytes = 0;
   // L1 wavelet proximal step of FISTA (fraction 0 = none)
   bool gather_tiles = false;   // tile-staged gather over the dense tiles
+  bool scatter_valu = false;   // VALU scatter accumulate (else matrix cores for 16-coil blocks)
   float l1_fraction = 0.0f;
   int l1_levels[3] = {4, 4, 4};
   int l1_spins = 4;

@@ -292,3 +292,28 @@ def test_tile_staged_gather_equals_direct_and_oracle(C, dwin):
     o = orc.OracleOp(N, overgrid=1.25, dwin=dwin, precision="f32")
     ref = o.image_to_kspace(X, smaps, traj)
     assert rel(a_tile, ref) < 1e-5
+
+
+@pytest.mark.parametrize("C,dwin,max_visits", [(16, 2.5, 0), (32, 2.0, 0), (16, 1.5, 64), (32, 2.5, 64),
+                                               (16, 3.0, 0)])
+def test_matrix_core_scatter_matches_valu_and_oracle(C, dwin, max_visits):
+    """The matrix-core scatter (16-coil blocks) sums the same products as the
+    VALU scatter in another order: Aᴴ agrees with it to FP32 rounding and with
+    the oracle to the usual bar.  max_visits=64 splits the dense tiles over
+    several work items, whose partial sums are added with atomics."""
+    N = 32
+    traj, X, smaps, d = radial_problem(N, C, 900, 40, seed=43)
+    out = []
+    for valu in (False, True):
+        op = GridFFT(N, ncoils=C, overgrid=1.25, dwin=dwin, scatter_valu=valu, max_visits=max_visits)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        out.append(op.forward(to_dev(d), accumulate=False).cpu().numpy())
+    assert rel(out[0], out[1]) <= 1e-6
+    o32 = orc.OracleOp(N, overgrid=1.25, dwin=dwin, precision="f32")
+    o64 = orc.OracleOp(N, overgrid=1.25, dwin=dwin, precision="f64")
+    X32 = o32.kspace_to_image(d, smaps, traj)
+    X64 = o64.kspace_to_image(d.astype(np.complex128), smaps.astype(np.complex128), traj)
+    e32, e64 = rel(out[0], X32), rel(out[0], X64)
+    print("matrix-core AH C=%d dwin=%.1f: vs f32 %.2e vs f64 %.2e" % (C, dwin, e32, e64))
+    assert e32 <= TOL32 and e64 <= TOL64
