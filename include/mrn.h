@@ -228,8 +228,42 @@ int mrn_stage_times(mrn_op *op, int frame, const void *X, int reps, float ms[MRN
 /* Wait for the work queued on `stream` (NULL = the null stream). */
 int mrn_sync(mrn_op *op, void *stream);
 
+/* ---- Pipe density compensation (next to the operator) ------------------
+ * RECON::dcf_calc (recon_lib.cpp:854-917) with DCFgridFFT's polynomial
+ * kernel (DCFgridFFT.cpp:56-440): kw = 1, then `iters` times
+ * kw <- kw / (A^T A kw) (samples with |A^T A kw| < 1e-3 get 0), A the real
+ * gridding onto a trunc(os*N + 16)-cell grid; finally kw = |kw * s^3| with
+ * s = |k|/128*(acc - 1) + 1 (scale_kw, DCFgridFFT.cpp:291-312). */
+enum { MRN_DCF_POLY = 0, MRN_DCF_POLY2 = 1 }; /* DCFgridFFT::POLY_KERNEL, POLY_KERNEL2 */
+typedef struct mrn_dcf_params {
+  int32_t N[3];    /* rcxres, rcyres, rczres (recon_lib.cpp:873-875) */
+  int32_t iters;   /* -dcf_iter, default 20 (recon_lib.cpp:126, 417) */
+  float dwin;      /* -dcf_dwin, default 2.1: kernel half width in cells */
+  float scale;     /* -dcf_scale, default 1: k is multiplied by it for the DCF */
+  float overgrid;  /* -dcf_overgrid, default 2.1 */
+  float acc;       /* -dcf_acc, default 1: kernel widening with |k| */
+  int32_t kernel;  /* MRN_DCF_POLY (what dcf_calc uses) or MRN_DCF_POLY2 */
+  int32_t reserved[3];
+} mrn_dcf_params;
+void mrn_dcf_params_default(mrn_dcf_params *p);
+/* -dcf_iter -dcf_dwin -dcf_scale -dcf_overgrid -dcf_acc (recon_lib.cpp:417-421);
+ * other arguments are ignored. */
+int mrn_dcf_parse_args(mrn_dcf_params *p, int argc, const char **argv);
+/* Grid extents (Sx, Sy, Sz) = trunc(os*(Nz, Ny, Nx) + 16): the reference
+ * declares the grid with the z size first (recon_lib.cpp:873). */
+int mrn_dcf_grid_size(const mrn_dcf_params *p, int32_t S[3]);
+/* One encoding's DCF (the body of recon_lib.cpp:879-913).  kx, ky, kz: device
+ * [M] in the trajectory's units; kw: device [M], overwritten with the weights.
+ * sums: optional host [iters], the printed sum of |kw| after each iteration.
+ * The caller's trajectory is not modified (the reference scales it in place by
+ * dcf_scale and back).  Needs 4*Sx*Sy*Sz bytes of device scratch for the call,
+ * and returns when the weights are on the device.  Errors: mrn_last_error(NULL). */
+int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const float *ky, const float *kz,
+                 float *kw, double *sums, void *stream);
+
 const char *mrn_status_string(int status);
-/* Message of the op's last failure ("" if none). */
+/* Message of the op's last failure ("" if none); for op = NULL, the message of
+ * this thread's last failed call that takes no op (mrn_dcf_*). */
 const char *mrn_last_error(const mrn_op *op);
 
 #ifdef __cplusplus

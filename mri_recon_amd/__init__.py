@@ -8,5 +8,6 @@ the image and the trajectory.
 from ._ffi import MrnError, lib  # noqa: F401
 from .gridfft import GridFFT  # noqa: F401
 from .autograd import SenseEncoding  # noqa: F401
+from .dcf import dcf_calc, dcf_params  # noqa: F401
 
-__all__ = ["GridFFT", "MrnError", "SenseEncoding", "lib"]
+__all__ = ["GridFFT", "MrnError", "SenseEncoding", "dcf_calc", "dcf_params", "lib"]

@@ -23,9 +23,24 @@ EXPORTS = [
     "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_cg_init", "mrn_cg_iterate",
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
     "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
-    "mrn_get_fista_thresholds",
+    "mrn_get_fista_thresholds", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
+    "mrn_dcf_calc",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
+
+
+class DcfParams(ctypes.Structure):
+    """mrn_dcf_params (include/mrn.h)."""
+    _fields_ = [
+        ("N", ctypes.c_int32 * 3),
+        ("iters", ctypes.c_int32),
+        ("dwin", ctypes.c_float),
+        ("scale", ctypes.c_float),
+        ("overgrid", ctypes.c_float),
+        ("acc", ctypes.c_float),
+        ("kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
 
 
 class MrnParams(ctypes.Structure):
@@ -103,6 +118,10 @@ def lib():
             "mrn_traj_grad": (i32, [vp, i32, vp, vp, vp, vp]),
             "mrn_autotune": (i32, [vp, i32, vp]),
             "mrn_get_tuning": (i32, [vp, vp, vp]),
+            "mrn_dcf_params_default": (None, [ctypes.POINTER(DcfParams)]),
+            "mrn_dcf_parse_args": (i32, [ctypes.POINTER(DcfParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
+            "mrn_dcf_grid_size": (i32, [ctypes.POINTER(DcfParams), vp]),
+            "mrn_dcf_calc": (i32, [ctypes.POINTER(DcfParams), i64, vp, vp, vp, vp, vp, vp]),
             "mrn_sync": (i32, [vp, vp]),
             "mrn_status_string": (ctypes.c_char_p, [i32]),
             "mrn_last_error": (ctypes.c_char_p, [vp]),
@@ -124,7 +143,5 @@ def status_string(s):
 
 def check(status, handle=None):
     if status != MRN_OK:
-        msg = ""
-        if handle:
-            msg = (lib().mrn_last_error(handle) or b"").decode()
+        msg = (lib().mrn_last_error(handle) or b"").decode()
         raise MrnError(status, msg)

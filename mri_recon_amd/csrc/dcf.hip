@@ -1,0 +1,350 @@
+// Pipe density compensation on the device: RECON::dcf_calc
+// (recon_lib.cpp:854-917) over DCFgridFFT's real-valued polynomial-kernel
+// gridder (DCFgridFFT.cpp:96-440).
+//
+// The trajectory stays resident; one call runs every iteration on the stream:
+//   kw = 1; repeat iters: X = 0; X += A^T kw (scatter); kw2 = A X (gather);
+//   kw = |kw2| < 1e-3 ? 0 : kw / kw2;   then kw = |kw * scale^3|.
+// The gather and the ratio update are one kernel; no per-iteration host sync.
+//
+// HBM layout: X is the real grid [Sz][Sy][Sx] (x fastest) with the reference's
+// extents trunc(os*N + 16), the kx extent taken from Nz (recon_lib.cpp:873).
+// The sample order is the caller's (radial spokes run outward, so neighbours
+// in a wavefront touch neighbouring cells).
+//
+// Arithmetic follows the reference expression by expression: float where it
+// is float and double where it promotes (the table interpolation and the
+// forward scale), with the *_rn intrinsics so that no multiply-add is fused
+// where the reference rounds twice.  The scatter adds with float atomics, as
+// the reference's OpenMP atomics do, so the order of the additions into a
+// cell differs from run to run; the gather sums in the reference's tap order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "mrn.h"
+
+namespace mrn {
+
+thread_local std::string g_free_err;  // message of the last failed call without an op
+
+const char *free_error() { return g_free_err.c_str(); }
+
+namespace {
+
+constexpr int DCF_MOD = 600;          // table points per cell (DCFgridFFT.cpp:101-103)
+constexpr int DCF_MAX_TABLE = 12288;  // table floats held in LDS (48 KB)
+
+struct DcfGeom {
+  int S[3];
+  float c[3];   // grid centre Sx/2 (integer division), as float
+  float gx;     // grid_x = overgrid
+  float dwin;
+  float scale;  // dcf_scale applied to k
+  float acc;
+  int len;
+};
+
+int dcf_fail(int status, const char *msg) {
+  g_free_err = msg;
+  return status;
+}
+
+#define DCF_HIP(call)                                                       \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      g_free_err = std::string(#call ": ") + hipGetErrorString(e_);         \
+      return MRN_EHIP;                                                      \
+    }                                                                       \
+  } while (0)
+
+// Polynomial table (DCFgridFFT.cpp:56-90, 160-164), on the host in double.
+int dcf_table(float dwin, int kernel, float *tab) {
+  static const double P1[6] = {1., 0.04522831, -3.36020304, 1.12417012, 2.82448025, -1.63447764};
+  static const double P2[6] = {1., 0.03056504, -3.01961845, 0.6679865, 2.77924058, -1.45923643};
+  const double *c = kernel == 1 ? P2 : P1;
+  const int len = (int)(dwin * (float)DCF_MOD) + 100;
+  if (!tab) return len;
+  double sum = 0.0;
+  for (int i = 0; i < len; i++) {
+    const double x = (double)i / (double)len;
+    const double x2 = x * x, x4 = x2 * x2, x3 = x * x2, x5 = x * x4;
+    tab[i] = (float)(c[0] + c[1] * x + c[2] * x2 + c[3] * x3 + c[4] * x4 + c[5] * x5);
+  }
+  for (int i = 0; i < len; i++) sum += tab[i];  // float entries summed in double
+  const double f = 0.5 * (double)DCF_MOD / sum;
+  for (int i = 0; i < len; i++) tab[i] = (float)(tab[i] * f);
+  return len;
+}
+
+void grid_size(const mrn_dcf_params *p, int S[3]) {
+  S[0] = (int)(p->overgrid * (float)p->N[2] + 16.0f);
+  S[1] = (int)(p->overgrid * (float)p->N[1] + 16.0f);
+  S[2] = (int)(p->overgrid * (float)p->N[0] + 16.0f);
+}
+
+__device__ __forceinline__ float kr_of(const DcfGeom &g, float kx, float ky, float kz, float k[3]) {
+  k[0] = __fmul_rn(kx, g.scale);
+  k[1] = __fmul_rn(ky, g.scale);
+  k[2] = __fmul_rn(kz, g.scale);
+  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(k[0], k[0]), __fmul_rn(k[1], k[1])), __fmul_rn(k[2], k[2])));
+}
+
+// forward / scale_kw: kr / 128.0 * (acc - 1) + 1 in double
+__device__ __forceinline__ float scale_fwd(const DcfGeom &g, float kr) {
+  return (float)__dadd_rn(__dmul_rn(__ddiv_rn((double)kr, 128.0), (double)(g.acc - 1.0f)), 1.0);
+}
+// backward: kr / 128 * (acc - 1) + 1 in float
+__device__ __forceinline__ float scale_bwd(const DcfGeom &g, float kr) {
+  return __fadd_rn(__fmul_rn(__fdiv_rn(kr, 128.0f), __fsub_rn(g.acc, 1.0f)), 1.0f);
+}
+
+__device__ __forceinline__ bool axis(const DcfGeom &g, int a, float k, float scale, float &dk, int &s, int &e) {
+  dk = __fadd_rn(__fmul_rn(k, g.gx), g.c[a]);
+  const float r = __fmul_rn(scale, g.dwin);
+  s = max((int)ceilf(__fsub_rn(dk, r)), 0);
+  e = min((int)floorf(__fadd_rn(dk, r)), g.S[a] - 1);
+  return !(s >= g.S[a] || e < 0);
+}
+
+// t[(int)del] * (1.0 - d) + t[(int)del + 1] * d, the first product in double
+// and the second in float, as the reference's expression promotes
+__device__ __forceinline__ double lerp(const float *tab, float dk, int l, float scale) {
+  const float del = __fdiv_rn(fabsf(__fmul_rn((float)DCF_MOD, __fsub_rn(dk, (float)l))), scale);
+  const int i = (int)del;
+  const float d = __fsub_rn(del, (float)i);
+  return __dadd_rn(__dmul_rn((double)tab[i], __dsub_rn(1.0, (double)d)), (double)__fmul_rn(tab[i + 1], d));
+}
+
+__device__ __forceinline__ void load_table(float *s_tab, const float *tab, int len) {
+  for (int i = threadIdx.x; i < len; i += blockDim.x) s_tab[i] = tab[i];
+  __syncthreads();
+}
+
+__global__ void k_dcf_fill(float *v, int64_t n, float value) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = value;
+}
+
+// DCFgridFFT::forward (DCFgridFFT.cpp:171-289): one thread per sample.
+__global__ __launch_bounds__(256) void k_dcf_scatter(float *__restrict__ X, const float *__restrict__ kw,
+                                                     const float *__restrict__ kx, const float *__restrict__ ky,
+                                                     const float *__restrict__ kz, int64_t M, DcfGeom g,
+                                                     const float *__restrict__ tab) {
+  extern __shared__ float s_tab[];
+  load_table(s_tab, tab, g.len);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  float k[3];
+  const float kr = kr_of(g, kx[i], ky[i], kz[i], k);
+  const float scale = scale_fwd(g, kr);
+  const float temp = kw[i];
+  if (temp == 0.0f || !(scale > 0.0f)) return;  // a scale <= 0 (acc < 1, large |k|) has no taps
+  float dk[3];
+  int s[3], e[3];
+  if (!axis(g, 0, k[0], scale, dk[0], s[0], e[0])) return;
+  if (!axis(g, 1, k[1], scale, dk[1], s[1], e[1])) return;
+  if (!axis(g, 2, k[2], scale, dk[2], s[2], e[2])) return;
+  for (int lz = s[2]; lz <= e[2]; lz++) {
+    const float wtz = (float)lerp(s_tab, dk[2], lz, scale);
+    for (int ly = s[1]; ly <= e[1]; ly++) {
+      const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk[1], ly, scale));
+      float *row = X + ((int64_t)lz * g.S[1] + ly) * g.S[0];
+      for (int lx = s[0]; lx <= e[0]; lx++) {
+        const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
+        atomicAdd(row + lx, __fmul_rn(wtx, temp));
+      }
+    }
+  }
+}
+
+// DCFgridFFT::backward (DCFgridFFT.cpp:314-440) and the ratio update of
+// dcf_calc (recon_lib.cpp:895-905), one thread per sample.  sum (optional)
+// collects the iteration's sum of |kw| (the reference prints it).
+__global__ __launch_bounds__(256) void k_dcf_gather_update(const float *__restrict__ X, float *__restrict__ kw,
+                                                           const float *__restrict__ kx,
+                                                           const float *__restrict__ ky,
+                                                           const float *__restrict__ kz, int64_t M, DcfGeom g,
+                                                           const float *__restrict__ tab, double *sum) {
+  extern __shared__ float s_tab[];
+  __shared__ double s_part[256 / 64];
+  load_table(s_tab, tab, g.len);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double mine = 0.0;
+  if (i < M) {
+    float k[3];
+    const float kr = kr_of(g, kx[i], ky[i], kz[i], k);
+    const float scale = scale_bwd(g, kr);
+    float temp = 0.0f;
+    float dk[3];
+    int s[3], e[3];
+    if (scale > 0.0f && axis(g, 0, k[0], scale, dk[0], s[0], e[0]) && axis(g, 1, k[1], scale, dk[1], s[1], e[1]) &&
+        axis(g, 2, k[2], scale, dk[2], s[2], e[2])) {
+      for (int lz = s[2]; lz <= e[2]; lz++) {
+        const float wtz = (float)lerp(s_tab, dk[2], lz, scale);
+        for (int ly = s[1]; ly <= e[1]; ly++) {
+          const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk[1], ly, scale));
+          const float *row = X + ((int64_t)lz * g.S[1] + ly) * g.S[0];
+          for (int lx = s[0]; lx <= e[0]; lx++) {
+            const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
+            temp = __fadd_rn(temp, __fmul_rn(wtx, row[lx]));
+          }
+        }
+      }
+    }
+    float w = kw[i];
+    if ((double)fabsf(temp) < 1e-3) {
+      w = 0.0f;
+    } else {
+      w = __fdiv_rn(w, temp);
+      mine = (double)fabsf(w);
+    }
+    kw[i] = w;
+  }
+  if (sum) {
+    for (int o = 32; o > 0; o >>= 1) mine += __shfl_down(mine, o);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += s_part[w];
+      atomicAdd(sum, t);
+    }
+  }
+}
+
+// scale_kw (DCFgridFFT.cpp:291-312) and kw = |kw| (recon_lib.cpp:910)
+__global__ void k_dcf_finish(float *__restrict__ kw, const float *__restrict__ kx, const float *__restrict__ ky,
+                             const float *__restrict__ kz, int64_t M, DcfGeom g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  float k[3];
+  const double s = (double)scale_fwd(g, kr_of(g, kx[i], ky[i], kz[i], k));
+  kw[i] = fabsf((float)__dmul_rn((double)kw[i], __dmul_rn(__dmul_rn(s, s), s)));
+}
+
+int64_t nblocks(int64_t n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+}  // namespace mrn
+
+using namespace mrn;
+
+extern "C" {
+
+void mrn_dcf_params_default(mrn_dcf_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->iters = 20;
+  p->dwin = 2.1f;
+  p->scale = 1.0f;
+  p->overgrid = 2.1f;
+  p->acc = 1.0f;
+  p->kernel = MRN_DCF_POLY;
+}
+
+int mrn_dcf_parse_args(mrn_dcf_params *p, int argc, const char **argv) {
+  if (!p || (argc > 0 && !argv)) return MRN_EINVAL;
+  for (int i = 0; i < argc; i++) {
+    const char *s = argv[i];
+    const bool has = i + 1 < argc;
+    if (!strcmp(s, "-dcf_iter")) {
+      if (!has) return MRN_EINVAL;
+      p->iters = atoi(argv[++i]);
+    } else if (!strcmp(s, "-dcf_dwin") || !strcmp(s, "-dcf_scale") || !strcmp(s, "-dcf_overgrid") ||
+               !strcmp(s, "-dcf_acc")) {
+      if (!has) return MRN_EINVAL;
+      const float v = (float)atof(argv[++i]);
+      if (!strcmp(s, "-dcf_dwin")) p->dwin = v;
+      else if (!strcmp(s, "-dcf_scale")) p->scale = v;
+      else if (!strcmp(s, "-dcf_overgrid")) p->overgrid = v;
+      else p->acc = v;
+    }
+  }
+  return MRN_OK;
+}
+
+int mrn_dcf_grid_size(const mrn_dcf_params *p, int32_t S[3]) {
+  if (!p || !S) return MRN_EINVAL;
+  int s[3];
+  grid_size(p, s);
+  for (int a = 0; a < 3; a++) S[a] = s[a];
+  return MRN_OK;
+}
+
+int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const float *ky, const float *kz,
+                 float *kw, double *sums, void *stream) {
+  g_free_err.clear();
+  if (!p) return dcf_fail(MRN_EINVAL, "null parameters");
+  if (M < 0 || (M > 0 && (!kx || !ky || !kz || !kw))) return dcf_fail(MRN_EINVAL, "sample arrays");
+  if (p->N[0] < 1 || p->N[1] < 1 || p->N[2] < 1) return dcf_fail(MRN_EINVAL, "image size");
+  if (p->iters < 0) return dcf_fail(MRN_EINVAL, "dcf_iter < 0");
+  if (!(p->dwin > 0.0f) || !(p->overgrid > 0.0f) || !std::isfinite(p->scale) || !std::isfinite(p->acc))
+    return dcf_fail(MRN_EINVAL, "dcf_dwin and dcf_overgrid must be positive, dcf_scale and dcf_acc finite");
+  if (p->kernel != MRN_DCF_POLY && p->kernel != MRN_DCF_POLY2) return dcf_fail(MRN_EINVAL, "dcf kernel");
+  const int len = dcf_table(p->dwin, p->kernel, nullptr);
+  if (len > DCF_MAX_TABLE) return dcf_fail(MRN_EUNSUPPORTED, "dcf_dwin too large for the kernel table");
+  DcfGeom g;
+  grid_size(p, g.S);
+  for (int a = 0; a < 3; a++) {
+    if (g.S[a] < 1) return dcf_fail(MRN_EINVAL, "grid size");
+    g.c[a] = (float)(g.S[a] / 2);
+  }
+  g.gx = p->overgrid;
+  g.dwin = p->dwin;
+  g.scale = p->scale;
+  g.acc = p->acc;
+  g.len = len;
+  if (M == 0) return MRN_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nS = (int64_t)g.S[0] * g.S[1] * g.S[2];
+  float *tab_h = new float[len];
+  dcf_table(p->dwin, p->kernel, tab_h);
+  float *X = nullptr, *tab = nullptr;
+  double *dsum = nullptr;
+  int rc = MRN_OK;
+  auto run = [&]() -> int {
+    DCF_HIP(hipMalloc(&X, sizeof(float) * nS));
+    DCF_HIP(hipMalloc(&tab, sizeof(float) * len));
+    DCF_HIP(hipMalloc(&dsum, sizeof(double) * (p->iters > 0 ? p->iters : 1)));
+    DCF_HIP(hipMemcpyAsync(tab, tab_h, sizeof(float) * len, hipMemcpyHostToDevice, st));
+    const size_t lds = sizeof(float) * len;
+    const int fillb = 4096;
+    hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, kw, M, 1.0f);
+    DCF_HIP(hipGetLastError());
+    if (p->iters > 0) {
+      // zero the per-iteration sums (a double 0 is two float 0s)
+      hipLaunchKernelGGL(k_dcf_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(dsum),
+                         (int64_t)2 * p->iters, 0.0f);
+      DCF_HIP(hipGetLastError());
+    }
+    const dim3 gs((unsigned)nblocks(M, 256));
+    for (int it = 0; it < p->iters; it++) {
+      hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, X, nS, 0.0f);
+      hipLaunchKernelGGL(k_dcf_scatter, gs, dim3(256), lds, st, X, kw, kx, ky, kz, M, g, tab);
+      hipLaunchKernelGGL(k_dcf_gather_update, gs, dim3(256), lds, st, X, kw, kx, ky, kz, M, g, tab,
+                         sums ? dsum + it : nullptr);
+      DCF_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_dcf_finish, gs, dim3(256), 0, st, kw, kx, ky, kz, M, g);
+    DCF_HIP(hipGetLastError());
+    if (sums && p->iters > 0)
+      DCF_HIP(hipMemcpyAsync(sums, dsum, sizeof(double) * p->iters, hipMemcpyDeviceToHost, st));
+    DCF_HIP(hipStreamSynchronize(st));  // the scratch grid is freed below
+    return MRN_OK;
+  };
+  rc = run();
+  if (rc != MRN_OK) hipStreamSynchronize(st);
+  hipFree(X);
+  hipFree(tab);
+  hipFree(dsum);
+  delete[] tab_h;
+  return rc;
+}
+
+}  // extern "C"

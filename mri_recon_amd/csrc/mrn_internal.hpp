@@ -209,6 +209,9 @@ __device__ __forceinline__ float tap_weight(const Geom &g, int a, const float *t
   return w;
 }
 
+// message of the last failed call without an op (dcf.hip)
+const char *free_error();
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Renumber them so that each XCD runs runs of `run` consecutive work items:
 // neighbouring tiles and samples, which read the same grid cells and k-space

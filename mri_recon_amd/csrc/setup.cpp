@@ -445,6 +445,6 @@ const char *mrn_status_string(int s) {
   }
 }
 
-const char *mrn_last_error(const mrn_op *op) { return op ? op->err.c_str() : ""; }
+const char *mrn_last_error(const mrn_op *op) { return op ? op->err.c_str() : mrn::free_error(); }
 
 }  // extern "C"

@@ -35,6 +35,28 @@ class OrcParams(ctypes.Structure):
     ]
 
 
+class DcfParams(ctypes.Structure):
+    """Pipe DCF settings (recon_lib.cpp:124-130 defaults)."""
+    _fields_ = [
+        ("N", ctypes.c_int32 * 3),
+        ("iters", ctypes.c_int32),
+        ("dwin", ctypes.c_float),
+        ("scale", ctypes.c_float),
+        ("overgrid", ctypes.c_float),
+        ("acc", ctypes.c_float),
+        ("kernel", ctypes.c_int32),
+    ]
+
+
+def dcf_params(N, iters=20, dwin=2.1, scale=1.0, overgrid=2.1, acc=1.0, kernel=0):
+    p = DcfParams()
+    n = (N, N, N) if np.isscalar(N) else tuple(N)
+    for a in range(3):
+        p.N[a] = int(n[a])
+    p.iters, p.dwin, p.scale, p.overgrid, p.acc, p.kernel = int(iters), dwin, scale, overgrid, acc, int(kernel)
+    return p
+
+
 _LIBS = {}
 
 
@@ -88,6 +110,15 @@ def _lib(precision):
         lib.orc_fista.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int,
                                   dbl, vp]
         lib.orc_threads.restype = ctypes.c_int
+        lib.orc_dcf_table_len.argtypes = [ctypes.c_float]
+        lib.orc_dcf_table_len.restype = ctypes.c_int
+        lib.orc_dcf_table.argtypes = [ctypes.c_float, ctypes.c_int, vp]
+        lib.orc_dcf_table.restype = ctypes.c_int
+        lib.orc_dcf_grid_size.argtypes = [ctypes.POINTER(DcfParams), vp]
+        lib.orc_dcf_forward.argtypes = [ctypes.POINTER(DcfParams), vp, vp, vp, i64, vp, vp, vp]
+        lib.orc_dcf_backward.argtypes = [ctypes.POINTER(DcfParams), vp, vp, vp, i64, vp, vp, vp]
+        lib.orc_dcf_calc.argtypes = [ctypes.POINTER(DcfParams), i64, vp, vp, vp, vp, vp]
+        lib.orc_dcf_calc.restype = ctypes.c_int
         _LIBS[precision] = lib
     return _LIBS[precision]
 
@@ -170,6 +201,66 @@ def spin_shifts(seed, count):
     out = np.zeros(count, np.int32)
     _lib("f64").orc_spin_shifts(int(seed), int(count), out.ctypes.data)
     return out
+
+
+def _rtype(precision):
+    return np.float32 if precision == "f32" else np.float64
+
+
+def dcf_table(dwin=2.1, kernel=0, precision="f32"):
+    """DCFgridFFT's normalised polynomial table (DCFgridFFT.cpp:56-165)."""
+    lib = _lib(precision)
+    tab = np.zeros(lib.orc_dcf_table_len(dwin), _rtype(precision))
+    lib.orc_dcf_table(dwin, int(kernel), tab.ctypes.data)
+    return tab
+
+
+def dcf_grid_size(p):
+    """(Sx, Sy, Sz) of dcf_calc's grid (recon_lib.cpp:873-875)."""
+    S = np.zeros(3, np.int32)
+    _lib("f64").orc_dcf_grid_size(ctypes.byref(p), S.ctypes.data)
+    return tuple(int(v) for v in S)
+
+
+def _traj3(kx, ky, kz):
+    return [np.ascontiguousarray(np.asarray(k, np.float32).ravel()) for k in (kx, ky, kz)]
+
+
+def dcf_forward(p, kw, kx, ky, kz, precision="f32"):
+    """DCFgridFFT::forward onto a zero grid: returns X of shape (Sz, Sy, Sx)."""
+    rt = _rtype(precision)
+    tab = dcf_table(p.dwin, p.kernel, precision)
+    Sx, Sy, Sz = dcf_grid_size(p)
+    X = np.zeros((Sz, Sy, Sx), rt)
+    kw = np.ascontiguousarray(kw, rt)
+    t = _traj3(kx, ky, kz)
+    _lib(precision).orc_dcf_forward(ctypes.byref(p), tab.ctypes.data, X.ctypes.data, kw.ctypes.data, kw.size,
+                                    *[a.ctypes.data for a in t])
+    return X
+
+
+def dcf_backward(p, X, kx, ky, kz, precision="f32"):
+    """DCFgridFFT::backward: the grid X interpolated at every sample."""
+    rt = _rtype(precision)
+    tab = dcf_table(p.dwin, p.kernel, precision)
+    X = np.ascontiguousarray(X, rt)
+    t = _traj3(kx, ky, kz)
+    out = np.zeros(t[0].size, rt)
+    _lib(precision).orc_dcf_backward(ctypes.byref(p), tab.ctypes.data, X.ctypes.data, out.ctypes.data,
+                                     out.size, *[a.ctypes.data for a in t])
+    return out
+
+
+def dcf_calc(p, kx, ky, kz, precision="f32"):
+    """RECON::dcf_calc for one encoding: returns (kw, per-iteration sums)."""
+    t = _traj3(kx, ky, kz)
+    kw = np.zeros(t[0].size, _rtype(precision))
+    sums = np.zeros(max(p.iters, 1), np.float64)
+    rc = _lib(precision).orc_dcf_calc(ctypes.byref(p), kw.size, *[a.ctypes.data for a in t], kw.ctypes.data,
+                                      sums.ctypes.data)
+    if rc != 0:
+        raise MemoryError("oracle dcf_calc: allocation failed")
+    return kw, sums[: p.iters]
 
 
 class OracleOp:
