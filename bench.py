@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the DCF / L1-FISTA timings")
     ap.add_argument("--profile", action="store_true", help="batched iterations only (for rocprofv3)")
     ap.add_argument("--small", action="store_true", help="128^3 x 8 coils, 4096 x 64 radial (os 1.5, W=4)")
     ap.add_argument("--rocprof-summary", metavar="DIR", help="summarise rocprofv3 CSV output and exit")
@@ -263,8 +264,13 @@ def main():
         log("CPU oracle: %.2f s/coil over %d coils -> %.4f iter/s; GPU vs oracle E'E %.2e"
             % (t_coil, k, 1.0 / t_iter, err))
 
+    # ---- rows beside the operator: Pipe DCF and L1-wavelet FISTA ----------
+    if not a.no_next:
+        res["next_rows"] = next_rows(a, op, N, kx, ky, kz, M, log)
+
     dom_b = b[dom]
     achieved = dom_b / (st[dom] * 1e-3) / 1e9
+    traffic = profiled_traffic(KERNEL_NAMES[dom])
     out = {
         "metric": METRIC, "value": value, "unit": "iter/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -274,12 +280,101 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"kernel": KERNEL_NAMES[dom], "stage": dom, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": traffic["bytes"] if traffic else None, "traffic_source": traffic,
                      "algorithmic_bytes": dom_b, "avg_ms": st[dom]},
         "cpu_baseline": cpu,
     }
     out.update(res)
     print(json.dumps(out))
+
+
+def next_rows(a, op, N, kx, ky, kz, M, log):
+    """Device time of the Pipe DCF (default settings, 20 iterations) on the
+    bench trajectory with the oracle timed on a bounded sample beside it, and
+    of one L1-wavelet FISTA iteration beside one plain FISTA iteration."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import dcf_calc, dcf_params
+    from mri_recon_amd.dcf import dcf_grid_size
+    out = {}
+    p = dcf_params(N)
+    dcf_calc(kx, ky, kz, params=p)  # first call pays the allocations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w = dcf_calc(kx, ky, kz, params=p)
+    torch.cuda.synchronize()
+    t_dcf = time.perf_counter() - t0
+    dcf = {"ms": t_dcf * 1e3, "iters": p.iters, "samples": M, "grid": list(dcf_grid_size(p)),
+           "ms_per_iter": t_dcf * 1e3 / p.iters, "nonzero_weights": int(torch.count_nonzero(w).item())}
+    log("Pipe DCF (%d iterations, grid %s): %.1f ms" % (p.iters, dcf["grid"], t_dcf * 1e3))
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        h = [t.cpu().numpy() for t in (kx, ky, kz)]
+        # one oracle iteration on m and 2m samples: t = clear + m * per-sample
+        m = max(1, M // 32)
+        ts = []
+        for mm in (m, 2 * m):
+            t0 = time.perf_counter()
+            orc.dcf_calc(orc.dcf_params(N, iters=1), h[0][:mm], h[1][:mm], h[2][:mm], "f32")
+            ts.append(time.perf_counter() - t0)
+        per = max(ts[1] - ts[0], 1e-9) / m
+        fixed = max(ts[0] - per * m, 0.0)
+        t_full = p.iters * (fixed + per * M)
+        dcf["cpu_baseline"] = {"ms": t_full * 1e3, "cores": orc.threads(), "kind": "port",
+                               "sample": "one FP32-oracle iteration on the first %d and %d samples, "
+                                         "fitted as grid clearing + per-sample cost and scaled to %d "
+                                         "samples x %d iterations" % (m, 2 * m, M, p.iters)}
+        log("Pipe DCF oracle estimate: %.1f s" % t_full)
+    out["dcf"] = dcf
+    # L1-wavelet FISTA: per-iteration time from calls of 1 and 3 iterations
+    step = op.power_method(iters=4)
+    per = {}
+    for frac in (0.0, 0.5):
+        op.set_fista_l1(frac, spins=4, seed=1)
+        op.fista(1, step)
+        torch.cuda.synchronize()
+        tt = []
+        for k in (1, 3):
+            t0 = time.perf_counter()
+            op.fista(k, step)
+            torch.cuda.synchronize()
+            tt.append(time.perf_counter() - t0)
+        per[frac] = (tt[1] - tt[0]) / 2
+    op.set_fista_l1(0.0)
+    out["fista"] = {"plain_ms_per_iter": per[0.0] * 1e3, "l1_wavelet_ms_per_iter": per[0.5] * 1e3,
+                    "l1": "fraction 0.5, 4 cycle spins, Daubechies-4 levels (4,4,4)"}
+    log("FISTA: %.1f ms/iter plain, %.1f ms/iter with the L1 wavelet step" % (per[0.0] * 1e3, per[0.5] * 1e3))
+    return out
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summaries (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by the
+    same counter's reading for a 1 GiB device copy; None if the summaries do
+    not cover the kernel."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    base = os.path.join(here, "profiles", "sense_cg_256cube_32coil_")
+    vals, cal = {}, {}
+    name = kernel.split(" ")[0]
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        path = base + ctr.lower() + ".txt"
+        if not os.path.exists(path):
+            return None
+        for line in open(path):
+            if line.startswith("# calibration") and ctr in line:
+                cal[ctr] = float(line.split("factor")[1].split()[0])
+            f = line.split()
+            if ctr in f and name in line and "(" in line:
+                i = f.index(ctr)
+                vals[ctr] = float(f[i + 2])  # mean per dispatch, KiB
+    if len(vals) < 2 or len(cal) < 2:
+        return None
+    by = 1024.0 * (vals["FETCH_SIZE"] * cal["FETCH_SIZE"] + vals["WRITE_SIZE"] * cal["WRITE_SIZE"])
+    return {"bytes": by, "fetch_kib": vals["FETCH_SIZE"], "write_kib": vals["WRITE_SIZE"],
+            "calibration": cal, "files": ["profiles/sense_cg_256cube_32coil_fetch_size.txt",
+                                          "profiles/sense_cg_256cube_32coil_write_size.txt"]}
 
 
 if __name__ == "__main__":
