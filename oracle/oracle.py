@@ -57,6 +57,28 @@ def dcf_params(N, iters=20, dwin=2.1, scale=1.0, overgrid=2.1, acc=1.0, kernel=0
     return p
 
 
+class LlrParams(ctypes.Structure):
+    """Locally low-rank settings (clear.cpp:40-96 defaults)."""
+    _fields_ = [
+        ("block", ctypes.c_int32 * 3),
+        ("block_iter", ctypes.c_int32),
+        ("alpha_time", ctypes.c_float),
+        ("alpha_encode", ctypes.c_float),
+        ("amp", ctypes.c_float),
+        ("t2", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+def llr_params(alpha_time=0.0, alpha_encode=0.0, block=(4, 4, 4), block_iter=4, amp=4.0, t2=6.0, seed=0):
+    p = LlrParams()
+    for a in range(3):
+        p.block[a] = int(block[a])
+    p.block_iter, p.alpha_time, p.alpha_encode, p.amp, p.t2 = int(block_iter), alpha_time, alpha_encode, amp, t2
+    p.seed = int(seed)
+    return p
+
+
 _LIBS = {}
 
 
@@ -119,6 +141,18 @@ def _lib(precision):
         lib.orc_dcf_backward.argtypes = [ctypes.POINTER(DcfParams), vp, vp, vp, i64, vp, vp, vp]
         lib.orc_dcf_calc.argtypes = [ctypes.POINTER(DcfParams), i64, vp, vp, vp, vp, vp]
         lib.orc_dcf_calc.restype = ctypes.c_int
+        LP = ctypes.POINTER(LlrParams)
+        lib.orc_llr_shift.argtypes = [LP, vp, i64, ctypes.c_int, vp]
+        lib.orc_svd_values.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+        lib.orc_llr_block_smax.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+        lib.orc_llr_block_smax.restype = i64
+        lib.orc_median_nonzero.argtypes = [vp, i64]
+        lib.orc_median_nonzero.restype = dbl
+        lib.orc_llr_update_threshold.argtypes = [LP, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
+        lib.orc_llr_update_threshold.restype = ctypes.c_float
+        lib.orc_llr_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, vp, dbl]
+        lib.orc_llr.argtypes = [LP, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64]
+        lib.orc_llr.restype = ctypes.c_float
         _LIBS[precision] = lib
     return _LIBS[precision]
 
@@ -261,6 +295,73 @@ def dcf_calc(p, kx, ky, kz, precision="f32"):
     if rc != 0:
         raise MemoryError("oracle dcf_calc: allocation failed")
     return kw, sums[: p.iters]
+
+
+def _img5(X, precision):
+    ct = np.complex64 if precision == "f32" else np.complex128
+    X = np.ascontiguousarray(X, ct)
+    if X.ndim != 5:
+        raise ValueError("X must be (Nt, Ne, Nz, Ny, Nx)")
+    Nt, Ne, Nz, Ny, Nx = X.shape
+    return X, Nt, Ne, np.array([Nx, Ny, Nz], np.int32)
+
+
+def svd_values(A):
+    """Singular values (descending) of a complex matrix by the oracle's
+    one-sided Jacobi SVD."""
+    A = np.asarray(A, np.complex128)
+    Np, N = A.shape
+    buf = np.ascontiguousarray(A.T).view(np.float64)  # column-major, re/im interleaved
+    s = np.zeros(N)
+    _lib("f64").orc_svd_values(buf.ctypes.data, Np, N, s.ctypes.data)
+    return s
+
+
+def median_nonzero(v):
+    v = np.ascontiguousarray(v, np.float64)
+    return _lib("f64").orc_median_nonzero(v.ctypes.data, v.size)
+
+
+def llr_shift(p, bs, call, pass_):
+    b = np.ascontiguousarray(bs, np.int32)
+    out = np.zeros(3, np.int32)
+    _lib("f64").orc_llr_shift(ctypes.byref(p), b.ctypes.data, int(call), int(pass_), out.ctypes.data)
+    return out
+
+
+def llr_block_smax(X, dim, block=(4, 4, 4), precision="f64"):
+    X, Nt, Ne, Nv = _img5(X, precision)
+    bs = np.minimum(np.array(block, np.int32), Nv)
+    nb = int(np.prod(Nv // bs))
+    out = np.zeros(nb)
+    b = np.array(block, np.int32)
+    _lib(precision).orc_llr_block_smax(X.ctypes.data, Nt, Ne, Nv.ctypes.data, b.ctypes.data, int(dim),
+                                       out.ctypes.data)
+    return out
+
+
+def llr_update_threshold(p, X, dim, it, precision="f64"):
+    X, Nt, Ne, Nv = _img5(X, precision)
+    return _lib(precision).orc_llr_update_threshold(ctypes.byref(p), X.ctypes.data, Nt, Ne, Nv.ctypes.data,
+                                                    int(dim), int(it))
+
+
+def llr_pass(X, dim, shift, tau, block=(4, 4, 4), precision="f64"):
+    """One thresh pass (given shift and tau) on a copy of X."""
+    X, Nt, Ne, Nv = _img5(np.array(X), precision)
+    b = np.array(block, np.int32)
+    sh = np.array(shift, np.int32)
+    _lib(precision).orc_llr_pass(X.ctypes.data, Nt, Ne, Nv.ctypes.data, b.ctypes.data, int(dim), sh.ctypes.data,
+                                 float(tau))
+    return X
+
+
+def llr(p, X, dim, it, call, precision="f64"):
+    """update_threshold + thresh on a copy of X: returns (X, smax)."""
+    X, Nt, Ne, Nv = _img5(np.array(X), precision)
+    smax = _lib(precision).orc_llr(ctypes.byref(p), X.ctypes.data, Nt, Ne, Nv.ctypes.data, int(dim), int(it),
+                                   int(call))
+    return X, float(smax)
 
 
 class OracleOp:
