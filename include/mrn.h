@@ -261,6 +261,43 @@ int mrn_dcf_grid_size(const mrn_dcf_params *p, int32_t S[3]);
 int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const float *ky, const float *kz,
                  float *kw, double *sums, void *stream);
 
+/* ---- Locally low-rank threshold over frames ------------------------------
+ * LOWRANKCOIL::update_threshold + ::thresh (clear.cpp:233-556) as recon_lib
+ * applies them after each gradient step (recon_lib.cpp:2117-2126), for an
+ * image of Nt x Ne frames, X[(t*Ne + e)][Nz][Ny][Nx] complex64 on the device.
+ * Blocks of b0 x b1 x b2 voxels (clamped to the volume; remainders untouched)
+ * form (frames of the other index x voxels) x (frames of `dim`) matrices;
+ * smax = median of the blocks' nonzero largest singular values times
+ * 1 + amp*exp(-iter/t2); then block_iter passes, each with its own block
+ * shift (periodic), soft-threshold every block's singular values by
+ * alpha*smax/block_iter (alpha = alpha_time for dim 0, alpha_encode for 1;
+ * 0 = update_threshold only, as the reference's early return).
+ * Shifts: pass k of call c moves axis q by splitmix64(seed, 3*(c*block_iter
+ * + k) + q) % b - b/2 (the reference's rand() % b - b/2 cannot be replayed). */
+typedef struct mrn_llr_params {
+  int32_t block[3];     /* -clear_nx/-clear_ny/-clear_nz, default 4 (clear.cpp:44-48) */
+  int32_t block_iter;   /* -clear_block_iter, default 4 */
+  float alpha_time;     /* -clear_alpha_time, default 0 */
+  float alpha_encode;   /* -clear_alpha_encode, default 0 */
+  float amp;            /* -clear_amp, default 4 */
+  float t2;             /* -clear_t2, default 6 */
+  uint64_t seed;        /* block shift stream */
+  int32_t reserved[4];
+} mrn_llr_params;
+void mrn_llr_params_default(mrn_llr_params *p);
+int mrn_llr_parse_args(mrn_llr_params *p, int argc, const char **argv);
+/* One update_threshold + thresh call; N = {Nx, Ny, Nz}; at most 32 frames
+ * along `dim` and about 4096 matrix entries per block.  smax (optional, host)
+ * receives the threshold scale.  Returns when done; errors: mrn_last_error(NULL). */
+int mrn_llr(const mrn_llr_params *p, void *X, int Nt, int Ne, const int32_t N[3], int dim, int iter,
+            int64_t call, float *smax, void *stream);
+/* Time LLR (dim 0 over the op's frames) inside each FISTA iteration, after the
+ * gradient step and before the L1 step; NULL or alpha_time = 0 turns it off.
+ * The iteration index drives amp's decay and the shift stream's call index. */
+int mrn_set_fista_llr(mrn_op *op, const mrn_llr_params *p);
+/* smax of the last FISTA iteration's low-rank step. */
+int mrn_get_fista_llr_smax(mrn_op *op, float *smax, void *stream);
+
 const char *mrn_status_string(int status);
 /* Message of the op's last failure ("" if none); for op = NULL, the message of
  * this thread's last failed call that takes no op (mrn_dcf_*). */

@@ -9,5 +9,6 @@ from ._ffi import MrnError, lib  # noqa: F401
 from .gridfft import GridFFT  # noqa: F401
 from .autograd import SenseEncoding  # noqa: F401
 from .dcf import dcf_calc, dcf_params  # noqa: F401
+from .llr import llr_params, llr_threshold  # noqa: F401
 
-__all__ = ["GridFFT", "MrnError", "SenseEncoding", "dcf_calc", "dcf_params", "lib"]
+__all__ = ["GridFFT", "MrnError", "SenseEncoding", "dcf_calc", "dcf_params", "llr_params", "llr_threshold", "lib"]

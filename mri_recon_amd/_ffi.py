@@ -24,9 +24,24 @@ EXPORTS = [
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
     "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
     "mrn_get_fista_thresholds", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
-    "mrn_dcf_calc",
+    "mrn_dcf_calc", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
+    "mrn_get_fista_llr_smax",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
+
+
+class LlrParams(ctypes.Structure):
+    """mrn_llr_params (include/mrn.h)."""
+    _fields_ = [
+        ("block", ctypes.c_int32 * 3),
+        ("block_iter", ctypes.c_int32),
+        ("alpha_time", ctypes.c_float),
+        ("alpha_encode", ctypes.c_float),
+        ("amp", ctypes.c_float),
+        ("t2", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
 
 
 class DcfParams(ctypes.Structure):
@@ -122,6 +137,11 @@ def lib():
             "mrn_dcf_parse_args": (i32, [ctypes.POINTER(DcfParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
             "mrn_dcf_grid_size": (i32, [ctypes.POINTER(DcfParams), vp]),
             "mrn_dcf_calc": (i32, [ctypes.POINTER(DcfParams), i64, vp, vp, vp, vp, vp, vp]),
+            "mrn_llr_params_default": (None, [ctypes.POINTER(LlrParams)]),
+            "mrn_llr_parse_args": (i32, [ctypes.POINTER(LlrParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
+            "mrn_llr": (i32, [ctypes.POINTER(LlrParams), vp, i32, i32, vp, i32, i32, i64, vp, vp]),
+            "mrn_set_fista_llr": (i32, [vp, ctypes.POINTER(LlrParams)]),
+            "mrn_get_fista_llr_smax": (i32, [vp, vp, vp]),
             "mrn_sync": (i32, [vp, vp]),
             "mrn_status_string": (ctypes.c_char_p, [i32]),
             "mrn_last_error": (ctypes.c_char_p, [vp]),

@@ -641,11 +641,14 @@ int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, doubl
   const float stf = (float)step;
   const bool l1 = op->l1_fraction > 0.0f;
   if (l1) MRN_TRY(ensure_l1_state(op));  // before any capture
+  MRN_TRY(llr_fista_prepare(op));
   auto body = [&](hipStream_t s) -> int {
     MRN_TRY(op_fista_momentum(op, X, op->vP, n, s));
     for (int f = 0; f < op->F; f++)
       MRN_TRY(grad_frame(op, f, X + f * op->nvox, op->vR + f * op->nvox, s));
     MRN_TRY(op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s));
+    // locally low-rank threshold over the frames (recon_lib.cpp:2117-2121)
+    if (op->llr_on) MRN_TRY(llr_fista_step(op, X, s));
     // L1 wavelet soft threshold after the gradient step (recon_lib.cpp:2145-2150)
     if (l1) MRN_TRY(l1_fista_step(op, X, s));
     return MRN_OK;

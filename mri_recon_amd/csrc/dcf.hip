@@ -33,6 +33,11 @@ namespace mrn {
 thread_local std::string g_free_err;  // message of the last failed call without an op
 
 const char *free_error() { return g_free_err.c_str(); }
+void free_err_clear() { g_free_err.clear(); }
+int free_fail(int status, const char *msg) {
+  g_free_err = msg ? msg : "";
+  return status;
+}
 
 namespace {
 

@@ -129,6 +129,13 @@ struct mrn_op {
   void *dthr = nullptr;        // threshold search state
   int *dsh = nullptr;          // cycle-spin shifts [MRN_MAX_SPINS][3]
   double *dthr_log = nullptr;  // threshold per FISTA iteration
+  // locally low-rank step of FISTA over the frames (off unless alpha_time > 0)
+  bool llr_on = false;
+  mrn_llr_params llr{};
+  double *llr_sblocks = nullptr;  // largest singular value of each block
+  long long llr_nblk = 0;
+  void *llr_sel = nullptr;        // median selection state
+  double *llr_scal = nullptr;     // tau, smax
 };
 
 namespace mrn {
@@ -211,6 +218,11 @@ __device__ __forceinline__ float tap_weight(const Geom &g, int a, const float *t
 
 // message of the last failed call without an op (dcf.hip)
 const char *free_error();
+void free_err_clear();
+int free_fail(int status, const char *msg);
+// locally low-rank FISTA step (llr.hip)
+int llr_fista_prepare(mrn_op *op);
+int llr_fista_step(mrn_op *op, float2 *X, hipStream_t st);
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Renumber them so that each XCD runs runs of `run` consecutive work items:

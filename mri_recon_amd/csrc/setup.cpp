@@ -400,6 +400,9 @@ int mrn_op_destroy(mrn_op *op) {
   hipFree(op->dthr);
   hipFree(op->dsh);
   hipFree(op->dthr_log);
+  hipFree(op->llr_sblocks);
+  hipFree(op->llr_sel);
+  hipFree(op->llr_scal);
   if (op->stream) hipStreamDestroy(op->stream);
   delete op;
   return MRN_OK;

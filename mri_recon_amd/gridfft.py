@@ -305,6 +305,20 @@ class GridFFT:
         lv = (ctypes.c_int32 * 3)(*levels)
         self._c(lib().mrn_set_fista_l1(self._h, float(fraction), lv, int(spins), ctypes.c_uint64(seed)))
 
+    def set_fista_llr(self, params=None, **kw):
+        """Apply the locally low-rank threshold over the frames inside each FISTA
+        iteration (before the L1 step); ``params`` from ``llr.llr_params`` or its
+        keywords; alpha_time 0 (or params None with no keywords) turns it off."""
+        from .llr import llr_params
+        if params is None and kw:
+            params = llr_params(**kw)
+        self._c(lib().mrn_set_fista_llr(self._h, ctypes.byref(params) if params is not None else None))
+
+    def fista_llr_smax(self):
+        out = ctypes.c_float()
+        self._c(lib().mrn_get_fista_llr_smax(self._h, ctypes.byref(out), _stream()))
+        return out.value
+
     def fista_thresholds(self, n):
         out = (ctypes.c_double * max(n, 1))()
         self._c(lib().mrn_get_fista_thresholds(self._h, out, int(n), _stream()))
