@@ -79,6 +79,38 @@ def llr_params(alpha_time=0.0, alpha_encode=0.0, block=(4, 4, 4), block_iter=4, 
     return p
 
 
+class GateParams(ctypes.Structure):
+    """Gated frame selection settings (gating.cpp:84-112 defaults)."""
+    _fields_ = [
+        ("gate_type", ctypes.c_int32),
+        ("vs_type", ctypes.c_int32),
+        ("frames", ctypes.c_int32),
+        ("tornado_shape", ctypes.c_int32),
+        ("wdth_low", ctypes.c_int32),
+        ("wdth_high", ctypes.c_int32),
+        ("kmax", ctypes.c_float),
+        ("min_quantile", ctypes.c_float),
+        ("max_quantile", ctypes.c_float),
+        ("ecg_outlier_reject", ctypes.c_int32),
+        ("reset_dens", ctypes.c_int32),
+    ]
+
+
+GATE_TYPES = {"none": 0, "retro_ecg": 1, "ecg": 2, "resp": 3, "time": 4, "prep": 5}
+VS_TYPES = {"tornado": 0, "none": 1, "hist": 2}
+TORNADO = {"flat": 0, "radial": 1, "vipr": 2}
+
+
+def gate_params(frames, gate="time", vs="none", shape="vipr", wdth_low=1, wdth_high=4, kmax=128.0,
+                min_quantile=0.0, max_quantile=1.0, ecg_outlier_reject=True, reset_dens=False):
+    p = GateParams()
+    p.gate_type, p.vs_type, p.frames, p.tornado_shape = GATE_TYPES[gate], VS_TYPES[vs], int(frames), TORNADO[shape]
+    p.wdth_low, p.wdth_high, p.kmax = int(wdth_low), int(wdth_high), float(kmax)
+    p.min_quantile, p.max_quantile = float(min_quantile), float(max_quantile)
+    p.ecg_outlier_reject, p.reset_dens = int(bool(ecg_outlier_reject)), int(bool(reset_dens))
+    return p
+
+
 _LIBS = {}
 
 
@@ -153,6 +185,12 @@ def _lib(precision):
         lib.orc_llr_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, vp, dbl]
         lib.orc_llr.argtypes = [LP, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64]
         lib.orc_llr.restype = ctypes.c_float
+        GP = ctypes.POINTER(GateParams)
+        lib.orc_quantile5.argtypes = [vp, i64, dbl]
+        lib.orc_quantile5.restype = dbl
+        lib.orc_gate_scale.argtypes = [GP, i64, vp, vp]
+        lib.orc_gate_scale.restype = ctypes.c_int
+        lib.orc_pregate.argtypes = [GP, i64, i64, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
         _LIBS[precision] = lib
     return _LIBS[precision]
 
@@ -362,6 +400,43 @@ def llr(p, X, dim, it, call, precision="f64"):
     smax = _lib(precision).orc_llr(ctypes.byref(p), X.ctypes.data, Nt, Ne, Nv.ctypes.data, int(dim), int(it),
                                    int(call))
     return X, float(smax)
+
+
+def quantile5(v, q):
+    s = np.sort(np.asarray(v, np.float64))
+    return _lib("f64").orc_quantile5(s.ctypes.data, s.size, float(q))
+
+
+def gate_scale(p, times):
+    """init_time_resolved's scaling of per-readout gate times: (times, info)."""
+    t = np.array(times, np.float64)
+    info = np.zeros(3)
+    if _lib("f64").orc_gate_scale(ctypes.byref(p), t.size, t.ctypes.data, info.ctypes.data) != 0:
+        raise ValueError("unknown gating setting")
+    return t, info
+
+
+def pregate(p, gt, kx, ky, kz, kw, kdata, resp=None):
+    """pregate_data for one encoding.  kx.. are (R, xres) float32, kdata
+    (C, R, xres) complex64, gt (R,) scaled gate times; returns (counts, kx, ky,
+    kz, kw, kdata) with every frame's samples in sample order, frame after
+    frame, and kdata frame-major as [frame][C][n_frame]."""
+    R, xres = np.shape(kx)
+    gt = np.ascontiguousarray(gt, np.float64)
+    tr = [np.ascontiguousarray(a, np.float32) for a in (kx, ky, kz, kw)]
+    kd = np.ascontiguousarray(kdata, np.complex64)
+    C = kd.shape[0]
+    rs = None if resp is None else np.ascontiguousarray(resp, np.float32)
+    counts = np.zeros(p.frames, np.int64)
+    lib = _lib("f64")
+    args = [ctypes.byref(p), xres, R, gt.ctypes.data, None if rs is None else rs.ctypes.data] + \
+        [a.ctypes.data for a in tr] + [C, kd.ctypes.data, counts.ctypes.data]
+    lib.orc_pregate(*(args + [None] * 5))
+    n = int(counts.sum())
+    outs = [np.zeros(max(n, 1), np.float32) for _ in range(4)]
+    kdo = np.zeros(max(n * C, 1), np.complex64)
+    lib.orc_pregate(*(args + [o.ctypes.data for o in outs] + [kdo.ctypes.data]))
+    return (counts,) + tuple(o[:n] for o in outs) + (kdo[:n * C],)
 
 
 class OracleOp:
