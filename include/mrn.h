@@ -298,6 +298,47 @@ int mrn_set_fista_llr(mrn_op *op, const mrn_llr_params *p);
 /* smax of the last FISTA iteration's low-rank step. */
 int mrn_get_fista_llr_smax(mrn_op *op, float *smax, void *stream);
 
+/* ---- Gated frame selection --------------------------------------------
+ * GATING::init_time_resolved's gate-time scaling (gating.cpp:880-1078) and
+ * RECON::pregate_data (recon_lib.cpp:675-776) with GATING::weight_data /
+ * hist_weight / tornado_weight (gating.cpp:1082-1200).  Enum values are the
+ * reference's (gating.h:27-42). */
+enum { MRN_VS_TORNADO = 0, MRN_VS_NONE = 1, MRN_VS_HIST = 2 };
+enum { MRN_TORNADO_FLAT = 0, MRN_TORNADO_RADIAL = 1, MRN_TORNADO_VIPR = 2 };
+enum { MRN_GATE_NONE = 0, MRN_GATE_RETRO_ECG = 1, MRN_GATE_ECG = 2, MRN_GATE_RESP = 3, MRN_GATE_TIME = 4,
+       MRN_GATE_PREP = 5 };
+typedef struct mrn_gate_params {
+  int32_t gate_type;          /* -gating_type: sets how the time range is found */
+  int32_t vs_type;            /* -viewshare_type tornado | none | hist (default none) */
+  int32_t frames;             /* -rcframes */
+  int32_t tornado_shape;      /* default VIPR (|k|^2 widening) */
+  int32_t wdth_low, wdth_high;/* -vs_wdth_low/-vs_wdth_high, default 1 and 4 frames */
+  float kmax;                 /* tornado |k| scale (the reference takes max |k| of encoding 0) */
+  float min_quantile, max_quantile; /* -gate_min/max_quantile (respiratory range), 0 and 1 */
+  int32_t ecg_outlier_reject; /* -ecg_out_reject / -wo_ecg_out_reject, default on */
+  int32_t reset_dens;         /* 1: kw = 1 for kept samples (recon's reset_dens) */
+  int32_t reserved[4];
+} mrn_gate_params;
+void mrn_gate_params_default(mrn_gate_params *p);
+int mrn_gate_parse_args(mrn_gate_params *p, int argc, const char **argv);
+/* Host, in place: per-readout gate times -> frame coordinates (min/max range;
+ * [0, 2 x median) for retrospective ECG, outliers >= 1.3 x that dropped first;
+ * respiratory quantiles (Hyndman-Fan 5) widened by half a frame), then floor
+ * (no sharing) or rank bins (hist); info = {offset, scale, temporal resolution}. */
+int mrn_gate_scale_times(const mrn_gate_params *p, int64_t R, double *times, double info[3]);
+/* Device: R readouts of xres samples; gt [R] scaled gate times (double),
+ * resp [R] respiratory weights or NULL; kx..kw [R*xres]; kdata [C][R*xres]
+ * complex64.  counts (host, [frames]) receives the samples per frame.  With
+ * kx_out NULL only the counts are computed; otherwise the outputs (sum of
+ * counts entries; kdata_out C times that) receive every frame's samples in
+ * order, frame after frame, k-space as [frame][C][count], kw scaled by
+ * 1e6/count (or 1 with reset_dens).  A sample can be in several frames under
+ * tornado sharing.  Returns when done; errors: mrn_last_error(NULL). */
+int mrn_pregate(const mrn_gate_params *p, int64_t xres, int64_t R, const double *gt, const float *resp,
+                const float *kx, const float *ky, const float *kz, const float *kw, int C, const void *kdata,
+                int64_t *counts, float *kx_out, float *ky_out, float *kz_out, float *kw_out, void *kdata_out,
+                void *stream);
+
 const char *mrn_status_string(int status);
 /* Message of the op's last failure ("" if none); for op = NULL, the message of
  * this thread's last failed call that takes no op (mrn_dcf_*). */

@@ -10,5 +10,7 @@ from .gridfft import GridFFT  # noqa: F401
 from .autograd import SenseEncoding  # noqa: F401
 from .dcf import dcf_calc, dcf_params  # noqa: F401
 from .llr import llr_params, llr_threshold  # noqa: F401
+from .gate import gate_params, gate_scale_times, pregate  # noqa: F401
 
-__all__ = ["GridFFT", "MrnError", "SenseEncoding", "dcf_calc", "dcf_params", "llr_params", "llr_threshold", "lib"]
+__all__ = ["GridFFT", "MrnError", "SenseEncoding", "dcf_calc", "dcf_params", "llr_params", "llr_threshold", "gate_params", "gate_scale_times",
+           "pregate", "lib"]

@@ -25,9 +25,28 @@ EXPORTS = [
     "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
     "mrn_get_fista_thresholds", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
     "mrn_dcf_calc", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
-    "mrn_get_fista_llr_smax",
+    "mrn_get_fista_llr_smax", "mrn_gate_params_default", "mrn_gate_parse_args", "mrn_gate_scale_times",
+    "mrn_pregate",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
+
+
+class GateParams(ctypes.Structure):
+    """mrn_gate_params (include/mrn.h)."""
+    _fields_ = [
+        ("gate_type", ctypes.c_int32),
+        ("vs_type", ctypes.c_int32),
+        ("frames", ctypes.c_int32),
+        ("tornado_shape", ctypes.c_int32),
+        ("wdth_low", ctypes.c_int32),
+        ("wdth_high", ctypes.c_int32),
+        ("kmax", ctypes.c_float),
+        ("min_quantile", ctypes.c_float),
+        ("max_quantile", ctypes.c_float),
+        ("ecg_outlier_reject", ctypes.c_int32),
+        ("reset_dens", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
 
 
 class LlrParams(ctypes.Structure):
@@ -142,6 +161,11 @@ def lib():
             "mrn_llr": (i32, [ctypes.POINTER(LlrParams), vp, i32, i32, vp, i32, i32, i64, vp, vp]),
             "mrn_set_fista_llr": (i32, [vp, ctypes.POINTER(LlrParams)]),
             "mrn_get_fista_llr_smax": (i32, [vp, vp, vp]),
+            "mrn_gate_params_default": (None, [ctypes.POINTER(GateParams)]),
+            "mrn_gate_parse_args": (i32, [ctypes.POINTER(GateParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
+            "mrn_gate_scale_times": (i32, [ctypes.POINTER(GateParams), i64, vp, vp]),
+            "mrn_pregate": (i32, [ctypes.POINTER(GateParams), i64, i64, vp, vp, vp, vp, vp, vp, i32, vp, vp,
+                                  vp, vp, vp, vp, vp, vp]),
             "mrn_sync": (i32, [vp, vp]),
             "mrn_status_string": (ctypes.c_char_p, [i32]),
             "mrn_last_error": (ctypes.c_char_p, [vp]),
