@@ -343,10 +343,95 @@ def next_rows(a, op, N, kx, ky, kz, M, log):
             tt.append(time.perf_counter() - t0)
         per[frac] = (tt[1] - tt[0]) / 2
     op.set_fista_l1(0.0)
+    out["llr"] = llr_timing(a, N, log)
+    out["pregate"] = pregate_timing(a, kx, ky, kz, M, log)
     out["fista"] = {"plain_ms_per_iter": per[0.0] * 1e3, "l1_wavelet_ms_per_iter": per[0.5] * 1e3,
                     "l1": "fraction 0.5, 4 cycle spins, Daubechies-4 levels (4,4,4)"}
     log("FISTA: %.1f ms/iter plain, %.1f ms/iter with the L1 wavelet step" % (per[0.0] * 1e3, per[0.5] * 1e3))
     return out
+
+
+def llr_timing(a, N, log):
+    """Locally low-rank step (defaults: 4^3 blocks, 4 shifted passes) on an
+    N^3 x 8-frame image, with the oracle on a 32^3 x 8 sample scaled by voxels."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import llr_threshold
+    F = 8
+    g = torch.Generator(device="cuda").manual_seed(5)
+    X = torch.randn((F, 1, N, N, N), dtype=torch.complex64, device="cuda", generator=g)
+    llr_threshold(X, alpha_time=0.05, call=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for r in range(reps):
+        llr_threshold(X, alpha_time=0.05, iteration=r, call=r + 1)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    res = {"ms": ms, "frames": F, "image": [N, N, N], "blocks": (N // 4) ** 3, "passes": 5,
+           "GBps": 5 * 2 * 8 * F * N ** 3 / (ms * 1e-3) / 1e9,
+           "bytes_model": "update_threshold reads the image once, each of the 4 shifted passes reads and writes it"}
+    del X
+    log("LLR (%d^3 x %d frames, 4 passes): %.1f ms" % (N, F, ms))
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        n = 32
+        Xs = (np.random.default_rng(6).standard_normal((F, 1, n, n, n)) + 0j).astype(np.complex64)
+        t0 = time.perf_counter()
+        orc.llr(orc.llr_params(alpha_time=0.05), Xs, 0, 0, 0, "f32")
+        t = (time.perf_counter() - t0) * (N / n) ** 3
+        res["cpu_baseline"] = {"ms": t * 1e3, "cores": orc.threads(), "kind": "port",
+                               "sample": "one oracle call on a %d^3 x %d-frame image, scaled by voxels" % (n, F)}
+        log("LLR oracle estimate: %.1f s" % t)
+    return res
+
+
+def pregate_timing(a, kx, ky, kz, M, log):
+    """Frame selection of the bench trajectory into 20 frames (no sharing and
+    tornado sharing) with 32 coils of k-space."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import gate_params, gate_scale_times, pregate
+    R = M // 256
+    C = 32
+    kd = torch.zeros((C, R, 256), dtype=torch.complex64, device="cuda")
+    t = np.linspace(0.0, 1.0, R)
+    trj = [v.view(R, 256) for v in (kx, ky, kz)]
+    kw = torch.ones((R, 256), dtype=torch.float32, device="cuda")
+    res = {"readouts": R, "samples": M, "coils": C, "frames": 20}
+    for vs in ("none", "tornado"):
+        p = gate_params(20, vs=vs, kmax=128.0)
+        gt = torch.from_numpy(gate_scale_times(p, t)[0]).cuda()
+        ms = float("inf")
+        for rep in range(4):  # the first call sizes the output allocations
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            n, fr = pregate(p, gt, *trj, kw, kd)
+            torch.cuda.synchronize()
+            if rep:
+                ms = min(ms, (time.perf_counter() - t0) * 1e3)
+            del fr
+        kept = sum(n)
+        res[vs] = {"ms": ms, "kept_samples": kept,
+                   "GBps_out": kept * (16 + 8 * C) / (ms * 1e-3) / 1e9}
+        log("pregate %s: %.1f ms, %d samples kept" % (vs, ms, kept))
+    del kd
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        Rs = R // 20
+        h = [v[:Rs].cpu().numpy() for v in trj]
+        kds = (np.zeros((C, Rs, 256)) + 0j).astype(np.complex64)
+        p = orc.gate_params(20, vs="none")
+        gt, _ = orc.gate_scale(p, t[:Rs])
+        t0 = time.perf_counter()
+        orc.pregate(p, gt, *h, np.ones_like(h[0]), kds)
+        tc = (time.perf_counter() - t0) * 20
+        res["cpu_baseline"] = {"ms": tc * 1e3, "cores": 1, "kind": "port",
+                               "sample": "oracle selection of %d of %d readouts (no sharing), x20" % (Rs, R)}
+        log("pregate oracle estimate: %.1f s" % tc)
+    return res
 
 
 def profiled_traffic(kernel):
