@@ -38,7 +38,10 @@ struct LlrGeom {
   int block_iter;
   unsigned long long seed;
   long long nvox;
-  int lds_G, lds_V, lds_A, lds_R;  // byte offsets in dynamic LDS
+  int lds_G, lds_V, lds_A, lds_R;  // byte offsets in one block's LDS
+  int lds_block;                   // bytes of LDS per block (16-byte multiple)
+  int lpb;                         // lanes per block: 64, or 32 (two blocks per wavefront)
+  long long nblk;
 };
 
 constexpr int LLR_MAX_N = 32;
@@ -55,9 +58,11 @@ __host__ __device__ inline unsigned long long splitmix(unsigned long long seed, 
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
-// row-major N x N complex matrices in LDS
-#define GM(r, c) G[(r) * N + (c)]
-#define VM(r, c) V[(r) * N + (c)]
+// row-major N x N complex matrices in LDS, rows padded to N + 1 entries and
+// A's columns to Np + 1, so that lanes walking a column (or reading several
+// columns at one row) fall on different LDS banks
+#define GM(r, c) G[(r) * LDG + (c)]
+#define VM(r, c) V[(r) * LDG + (c)]
 
 // Pair k of round r of the round-robin schedule over n + 1 players (n odd):
 // (n, r) and ((r + k) mod n, (r - k) mod n).  Index n is the dummy when N is odd.
@@ -65,24 +70,34 @@ __device__ inline void rr_pair(int r, int k, int n, int &p, int &q) {
   if (k == 0) {
     p = n;
     q = r;
-  } else {
-    p = (r + k) % n;
-    q = (r - k + n) % n;
+  } else {  // r, k < n
+    p = r + k;
+    if (p >= n) p -= n;
+    q = r - k;
+    if (q < 0) q += n;
   }
 }
 
-template <bool THRESH>
+// LPB lanes per block: with LPB = 32 a wavefront works on two blocks, each
+// half with its own LDS region (small N leaves most of 64 lanes idle in the
+// rotation rounds otherwise).
+template <bool THRESH, int LPB>
 __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeom g, int pass,
                                                   const int *__restrict__ counter, long long call,
                                                   const double *__restrict__ tau_dev, double *__restrict__ smax_out) {
-  extern __shared__ __align__(16) unsigned char lds[];
+  extern __shared__ __align__(16) unsigned char lds_all[];
+  constexpr int BPW = 64 / LPB;
+  const int half_id = threadIdx.x / LPB;
+  unsigned char *lds = lds_all + half_id * g.lds_block;
   double2 *G = reinterpret_cast<double2 *>(lds + g.lds_G);
   double2 *V = reinterpret_cast<double2 *>(lds + g.lds_V);
   float2 *A = reinterpret_cast<float2 *>(lds + g.lds_A);
   double *R = reinterpret_cast<double *>(lds + g.lds_R);  // rotations (c, s, e.x, e.y), then f
-  const int lane = threadIdx.x;
-  const int N = g.N, Np = g.Np;
-  const long long b = blockIdx.x;
+  const int lane = threadIdx.x % LPB;
+  const int N = g.N, Np = g.Np, LDG = N + 1, LDA = Np + 1;
+  const long long bq = (long long)blockIdx.x * BPW + half_id;
+  const bool live = bq < g.nblk;  // a padding half repeats the last block and writes nothing
+  const long long b = live ? bq : g.nblk - 1;
   const int bi = (int)(b / ((long long)g.nb[1] * g.nb[2])), bj = (int)((b / g.nb[2]) % g.nb[1]),
             bk = (int)(b % g.nb[2]);
   int sh[3] = {0, 0, 0};
@@ -94,29 +109,39 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
               g.bs[q] / 2;
   }
   const int o0 = bi * g.bs[0] + sh[0], o1 = bj * g.bs[1] + sh[1], o2 = bk * g.bs[2] + sh[2];
-  // element idx of A (column-major: col = idx / Np, row = idx % Np) -> X offset
-  auto xoff = [&](int idx) -> long long {
-    const int col = idx / Np, row = idx % Np;
-    const int o = row / g.bv, v = row % g.bv;
-    const int ii = v % g.bs[0], jj = (v / g.bs[0]) % g.bs[1], kk = v / (g.bs[0] * g.bs[1]);
-    const int t = g.dim == 0 ? col : o, e = g.dim == 0 ? o : col;
-    const int px = (o0 + ii + g.Nv[0]) % g.Nv[0], py = (o1 + jj + g.Nv[1]) % g.Nv[1],
-              pz = (o2 + kk + g.Nv[2]) % g.Nv[2];
-    return ((long long)t * g.Ne + e) * g.nvox + ((long long)pz * g.Nv[1] + py) * g.Nv[0] + px;
+  // row r of A = (other frame index o, voxel v) with x fastest: its voxel
+  // offset in a frame and o; column c adds the frame (t, e) offset
+  auto row_of = [&](int row, int &o) -> long long {
+    o = row / g.bv;
+    const int v = row - o * g.bv;
+    const int kk = v / (g.bs[0] * g.bs[1]), rem = v - kk * g.bs[0] * g.bs[1];
+    const int jj = rem / g.bs[0], ii = rem - jj * g.bs[0];
+    int px = o0 + ii, py = o1 + jj, pz = o2 + kk;  // within (-Nv, 2 Nv)
+    px += px < 0 ? g.Nv[0] : (px >= g.Nv[0] ? -g.Nv[0] : 0);
+    py += py < 0 ? g.Nv[1] : (py >= g.Nv[1] ? -g.Nv[1] : 0);
+    pz += pz < 0 ? g.Nv[2] : (pz >= g.Nv[2] ? -g.Nv[2] : 0);
+    return ((long long)pz * g.Nv[1] + py) * g.Nv[0] + px;
+  };
+  auto frame_of = [&](int col, int o) -> long long {
+    return g.dim == 0 ? ((long long)col * g.Ne + o) * g.nvox : ((long long)o * g.Ne + col) * g.nvox;
   };
   // 1. gather
-  for (int idx = lane; idx < Np * N; idx += 64) A[idx] = X[xoff(idx)];
+  for (int row = lane; row < Np; row += LPB) {
+    int o;
+    const long long vo = row_of(row, o);
+    for (int col = 0; col < N; col++) A[col * LDA + row] = X[frame_of(col, o) + vo];
+  }
   __syncthreads();
   // 2. Gram matrix, upper triangle by lanes, mirrored
   const int npair = N * (N + 1) / 2;
-  for (int u = lane; u < npair; u += 64) {
+  for (int u = lane; u < npair; u += LPB) {
     int a = 0, rem = u;
     while (rem >= N - a) {
       rem -= N - a;
       a++;
     }
     const int c = a + rem;
-    const float2 *Aa = A + (long long)a * Np, *Ac = A + (long long)c * Np;
+    const float2 *Aa = A + (long long)a * LDA, *Ac = A + (long long)c * LDA;
     double sr = 0.0, si = 0.0;
     for (int r = 0; r < Np; r++) {
       const double xr = Aa[r].x, xi = Aa[r].y, yr = Ac[r].x, yi = Ac[r].y;
@@ -126,23 +151,28 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
     GM(a, c) = make_double2(sr, si);
     GM(c, a) = make_double2(sr, -si);
   }
-  for (int u = lane; u < N * N; u += 64) VM(u / N, u % N) = make_double2(u / N == u % N ? 1.0 : 0.0, 0.0);
+  for (int u = lane; u < N * N; u += LPB) VM(u / N, u % N) = make_double2(u / N == u % N ? 1.0 : 0.0, 0.0);
   __syncthreads();
   // 3. Jacobi sweeps
   const int Npl = N + (N & 1), n = Npl - 1, half = Npl / 2;
   for (int sweep = 0; sweep < LLR_SWEEPS; sweep++) {
     double off = 0.0, dia = 0.0;
-    for (int u = lane; u < N * N; u += 64) {
+    for (int u = lane; u < N * N; u += LPB) {
       const double2 v = GM(u / N, u % N);
       const double m = v.x * v.x + v.y * v.y;
       if (u / N == u % N) dia += m;
       else off += m;
     }
-    for (int s = 32; s > 0; s >>= 1) {
+    for (int s = LPB / 2; s > 0; s >>= 1) {
       off += __shfl_xor(off, s);
       dia += __shfl_xor(dia, s);
     }
-    if (off <= 1e-30 * dia || dia == 0.0) break;
+    // both halves of a wavefront run the same number of sweeps (the rounds
+    // hold barriers); each half stops rotating once its own block converged
+    const bool done = off <= 1e-24 * dia || dia == 0.0;
+    bool all_done = done;
+    if (BPW == 2) all_done = done && __shfl_xor(done ? 1 : 0, 32) != 0;
+    if (all_done) break;
     for (int r = 0; r < n; r++) {
       if (lane < half) {
         int p, q;
@@ -151,10 +181,12 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
         if (p < N && q < N) {
           const double al = GM(p, p).x, be = GM(q, q).x;
           const double2 ga = GM(p, q);
-          const double ag = hypot(ga.x, ga.y);
-          if (ag > 0.0) {
+          // Gram entries are sums of squared floats: no overflow in these squares;
+          // z * z overflowing to inf gives t = 0, the right limit
+          const double ag = sqrt(ga.x * ga.x + ga.y * ga.y);
+          if (ag > 0.0 && !done) {
             const double z = (be - al) / (2.0 * ag);
-            const double t = (z >= 0.0 ? 1.0 : -1.0) / (fabs(z) + hypot(1.0, z));
+            const double t = (z >= 0.0 ? 1.0 : -1.0) / (fabs(z) + sqrt(1.0 + z * z));
             c = 1.0 / sqrt(1.0 + t * t);
             s = c * t;
             ex = ga.x / ag;
@@ -168,7 +200,7 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
       }
       __syncthreads();
       // columns p, q of G and V: A <- A J
-      for (int it = lane; it < half * N; it += 64) {
+      for (int it = lane; it < half * N; it += LPB) {
         const int k = it / N, rr = it % N;
         int p, q;
         rr_pair(r, k, n, p, q);
@@ -190,7 +222,7 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
       }
       __syncthreads();
       // rows p, q of G: A <- J^H A, and the (p, q) pair set to 0
-      for (int it = lane; it < half * N; it += 64) {
+      for (int it = lane; it < half * N; it += LPB) {
         const int k = it / N, cc = it % N;
         int p, q;
         rr_pair(r, k, n, p, q);
@@ -210,20 +242,20 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
   // 4. results
   if (!THRESH) {
     double m = 0.0;
-    for (int a = lane; a < N; a += 64) m = fmax(m, sqrt(fmax(GM(a, a).x, 0.0)));
-    for (int s = 32; s > 0; s >>= 1) m = fmax(m, __shfl_xor(m, s));
-    if (lane == 0) smax_out[b] = m;
+    for (int a = lane; a < N; a += LPB) m = fmax(m, sqrt(fmax(GM(a, a).x, 0.0)));
+    for (int s = LPB / 2; s > 0; s >>= 1) m = fmax(m, __shfl_xor(m, s));
+    if (lane == 0 && live) smax_out[b] = m;
     return;
   }
   const double tau = *tau_dev;
   double *f = R;
-  for (int a = lane; a < N; a += 64) {
+  for (int a = lane; a < N; a += LPB) {
     const double sv = sqrt(fmax(GM(a, a).x, 0.0));
     f[a] = sv > 0.0 ? fmax(sv - tau, 0.0) / sv : 0.0;
   }
   __syncthreads();
   // P = V diag(f) V^H into G's space
-  for (int u = lane; u < N * N; u += 64) {
+  for (int u = lane; u < N * N; u += LPB) {
     const int a = u / N, c = u % N;
     double pr = 0.0, pi = 0.0;
     for (int k = 0; k < N; k++) {
@@ -236,16 +268,19 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
   }
   __syncthreads();
   // A P back to the image
-  for (int idx = lane; idx < Np * N; idx += 64) {
-    const int col = idx / Np, row = idx % Np;
-    double xr = 0.0, xi = 0.0;
-    for (int a = 0; a < N; a++) {
-      const float2 av = A[(long long)a * Np + row];
-      const double2 pv = GM(a, col);
-      xr += (double)av.x * pv.x - (double)av.y * pv.y;
-      xi += (double)av.x * pv.y + (double)av.y * pv.x;
+  for (int row = lane; row < Np; row += LPB) {
+    int o;
+    const long long vo = row_of(row, o);
+    for (int col = 0; col < N; col++) {
+      double xr = 0.0, xi = 0.0;
+      for (int a = 0; a < N; a++) {
+        const float2 av = A[(long long)a * LDA + row];
+        const double2 pv = GM(a, col);
+        xr += (double)av.x * pv.x - (double)av.y * pv.y;
+        xi += (double)av.x * pv.y + (double)av.y * pv.x;
+      }
+      if (live) X[frame_of(col, o) + vo] = make_float2((float)xr, (float)xi);
     }
-    X[xoff(idx)] = make_float2((float)xr, (float)xi);
   }
 }
 #undef GM
@@ -370,16 +405,19 @@ static int llr_geom(const mrn_llr_params *p, int Nt, int Ne, const int32_t N[3],
   if (g.N > LLR_MAX_N) return (err = "more than 32 frames along the low-rank dimension", MRN_EUNSUPPORTED);
   const int Npl = g.N + (g.N & 1);
   g.lds_G = 0;
-  g.lds_V = g.lds_G + 16 * g.N * g.N;
-  g.lds_R = g.lds_V + 16 * g.N * g.N;
+  g.lds_V = g.lds_G + 16 * g.N * (g.N + 1);
+  g.lds_R = g.lds_V + 16 * g.N * (g.N + 1);
   g.lds_A = g.lds_R + 8 * (4 * (Npl / 2) > g.N ? 4 * (Npl / 2) : g.N);
   g.lds_A = (g.lds_A + 15) & ~15;
-  const long long total = g.lds_A + 8ll * g.Np * g.N;
+  const long long total = g.lds_A + 8ll * (g.Np + 1) * g.N;
   if (total > LLR_LDS) return (err = "low-rank block matrix too large for LDS (Np x N > about 4096)", MRN_EUNSUPPORTED);
+  g.lds_block = (int)((total + 15) & ~15ll);
+  g.lpb = (g.N <= 16 && 2ll * g.lds_block <= LLR_LDS) ? 32 : 64;
+  g.nblk = (long long)g.nb[0] * g.nb[1] * g.nb[2];
   return MRN_OK;
 }
 
-static size_t llr_lds(const LlrGeom &g) { return (size_t)g.lds_A + 8ull * g.Np * g.N; }
+static size_t llr_lds(const LlrGeom &g) { return (size_t)g.lds_block * (64 / g.lpb); }
 static long long llr_blocks(const LlrGeom &g) { return (long long)g.nb[0] * g.nb[1] * g.nb[2]; }
 
 // update_threshold + thresh on stream st.  Scratch: sblocks [nblocks] doubles,
@@ -394,13 +432,31 @@ static hipError_t llr_launch(const LlrGeom &g, float2 *X, float alpha, float amp
   const size_t lds = llr_lds(g);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void *)k_llr_block<false>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
-    hipFuncSetAttribute((const void *)k_llr_block<true>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
+    hipFuncSetAttribute((const void *)k_llr_block<false, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
+    hipFuncSetAttribute((const void *)k_llr_block<true, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
+    hipFuncSetAttribute((const void *)k_llr_block<false, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
+    hipFuncSetAttribute((const void *)k_llr_block<true, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
     attr = true;
   }
-  if (nblk > 0)
-    hipLaunchKernelGGL(k_llr_block<false>, dim3((unsigned)nblk), dim3(64), lds, st, X, g, 0, counter, call,
-                       (const double *)nullptr, sblocks);
+  const unsigned nwg = (unsigned)((nblk + (64 / g.lpb) - 1) / (64 / g.lpb));
+  auto blocks = [&](bool thresh, int pass) {
+    if (g.lpb == 32) {
+      if (thresh)
+        hipLaunchKernelGGL((k_llr_block<true, 32>), dim3(nwg), dim3(64), lds, st, X, g, pass, counter, call, tau,
+                           (double *)nullptr);
+      else
+        hipLaunchKernelGGL((k_llr_block<false, 32>), dim3(nwg), dim3(64), lds, st, X, g, 0, counter, call,
+                           (const double *)nullptr, sblocks);
+    } else {
+      if (thresh)
+        hipLaunchKernelGGL((k_llr_block<true, 64>), dim3(nwg), dim3(64), lds, st, X, g, pass, counter, call, tau,
+                           (double *)nullptr);
+      else
+        hipLaunchKernelGGL((k_llr_block<false, 64>), dim3(nwg), dim3(64), lds, st, X, g, 0, counter, call,
+                           (const double *)nullptr, sblocks);
+    }
+  };
+  if (nblk > 0) blocks(false, 0);
   hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, st, sel);
   const unsigned gb = (unsigned)((nblk + 255) / 256 < 1024 ? (nblk + 255) / 256 : 1024);
   if (nblk > 0) {
@@ -414,9 +470,7 @@ static hipError_t llr_launch(const LlrGeom &g, float2 *X, float alpha, float amp
   hipLaunchKernelGGL(k_llr_finish, dim3(1), dim3(1), 0, st, sel, amp, t2, counter, iter, alpha, g.block_iter, tau,
                      smax);
   if (alpha != 0.0f && nblk > 0)
-    for (int pass = 0; pass < g.block_iter; pass++)
-      hipLaunchKernelGGL(k_llr_block<true>, dim3((unsigned)nblk), dim3(64), lds, st, X, g, pass, counter, call, tau,
-                         (double *)nullptr);
+    for (int pass = 0; pass < g.block_iter; pass++) blocks(true, pass);
   return hipGetLastError();
 }
 
