@@ -62,7 +62,7 @@ __device__ __forceinline__ bool selected(const GateDev &g, const double *__restr
 __global__ __launch_bounds__(256) void k_gate_count(GateDev g, const double *__restrict__ gt,
                                                     const float *__restrict__ resp, const float *__restrict__ kx,
                                                     const float *__restrict__ ky, const float *__restrict__ kz,
-                                                    int *__restrict__ bcnt) {
+                                                    int64_t *__restrict__ bcnt) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   const int t = blockIdx.y;
   const bool s = i < g.M && selected(g, gt, resp, kx, ky, kz, i, t);
@@ -262,7 +262,7 @@ int mrn_pregate(const mrn_gate_params *p, int64_t xres, int64_t R, const double 
   if (g.nb > 0x7fffffffll) return free_fail(MRN_EUNSUPPORTED, "too many samples");
   hipStream_t st = (hipStream_t)stream;
   const int64_t nflat = (int64_t)g.frames * g.nb;
-  int *bcnt = nullptr;
+  int64_t *bcnt = nullptr;  // 64-bit: the scan accumulates in its input type
   int64_t *boff = nullptr, *fb = nullptr;
   void *tmp = nullptr;
   size_t tb = 0;
@@ -270,10 +270,10 @@ int mrn_pregate(const mrn_gate_params *p, int64_t xres, int64_t R, const double 
   std::vector<int64_t> hfb(g.frames + 1);
   auto run = [&]() -> hipError_t {
     hipError_t e;
-    if ((e = hipMalloc(&bcnt, sizeof(int) * (nflat + 1))) != hipSuccess) return e;
+    if ((e = hipMalloc(&bcnt, sizeof(int64_t) * (nflat + 1))) != hipSuccess) return e;
     if ((e = hipMalloc(&boff, sizeof(int64_t) * (nflat + 1))) != hipSuccess) return e;
     if ((e = hipMalloc(&fb, sizeof(int64_t) * (g.frames + 1))) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(bcnt + nflat, 0, sizeof(int), st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(bcnt + nflat, 0, sizeof(int64_t), st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_gate_count, dim3((unsigned)g.nb, g.frames), dim3(256), 0, st, g, gt, resp, kx, ky, kz,
                        bcnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;

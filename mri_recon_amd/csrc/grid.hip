@@ -836,12 +836,10 @@ static size_t gather_tile_lds(const mrn_op *op, int nt) {
 template <int NT, int CCT>
 static int launch_gather_tile(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
                               const float2 *sub, int wflag, size_t lds, hipStream_t st) {
-  static bool attr = false;
-  if (!attr) {
-    MRN_HIP(op, hipFuncSetAttribute((const void *)k_gather_tile<NT, CCT>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
-    attr = true;
-  }
+  // once per instantiation (thread-safe static initialisation)
+  static const hipError_t attr = hipFuncSetAttribute((const void *)k_gather_tile<NT, CCT>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+  MRN_HIP(op, attr);
   const Geom &g = op->g;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   hipLaunchKernelGGL((k_gather_tile<NT, CCT>), dim3(fr.ngsub, op->nch), dim3(256), lds, st, grid, out, fr.coords,

@@ -45,7 +45,7 @@ struct LlrGeom {
 };
 
 constexpr int LLR_MAX_N = 32;
-constexpr int LLR_LDS = 64 * 1024;
+constexpr int LLR_LDS = 64 * 1024;  // the default dynamic LDS limit of a launch
 constexpr int LLR_SWEEPS = 30;
 
 __host__ __device__ inline unsigned long long splitmix(unsigned long long seed, unsigned long long index) {
@@ -430,14 +430,6 @@ static hipError_t llr_launch(const LlrGeom &g, float2 *X, float alpha, float amp
   SelState *sel = (SelState *)sel_v;
   const long long nblk = llr_blocks(g);
   const size_t lds = llr_lds(g);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_llr_block<false, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
-    hipFuncSetAttribute((const void *)k_llr_block<true, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
-    hipFuncSetAttribute((const void *)k_llr_block<false, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
-    hipFuncSetAttribute((const void *)k_llr_block<true, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, LLR_LDS);
-    attr = true;
-  }
   const unsigned nwg = (unsigned)((nblk + (64 / g.lpb) - 1) / (64 / g.lpb));
   auto blocks = [&](bool thresh, int pass) {
     if (g.lpb == 32) {
