@@ -48,7 +48,7 @@ def rocprof_summary(root):
     for f in sorted(glob.glob(os.path.join(root, "**", "*kernel_stats.csv"), recursive=True)):
         rows = sorted(csv.DictReader(open(f)), key=lambda r: -float(r["TotalDurationNs"]))
         out.append("%-64s %6s %11s %10s %6s" % ("kernel (kernel_stats.csv)", "calls", "avg_us", "total_ms", "pct"))
-        for r in rows[:30]:
+        for r in rows[:40]:
             out.append("%-64s %6s %11.1f %10.2f %6.2f" % (r["Name"][:64], r["Calls"], float(r["AverageNs"]) / 1e3,
                                                           float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
     for f in sorted(glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True)):
@@ -57,7 +57,7 @@ def rocprof_summary(root):
             acc[(r["Kernel_Name"][:64], r["Counter_Name"])].append(float(r["Counter_Value"]))
         out.append("%-64s %-11s %5s %15s %15s" % ("kernel (counter_collection.csv)", "counter", "n",
                                                   "mean/dispatch", "max/dispatch"))
-        for (k, c), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:30]:
+        for (k, c), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:80]:
             out.append("%-64s %-11s %5d %15.1f %15.1f" % (k, c, len(v), sum(v) / len(v), max(v)))
     print("\n".join(out))
 
@@ -72,7 +72,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--no-next", action="store_true", help="skip the DCF / L1-FISTA timings")
+    ap.add_argument("--no-next", action="store_true", help="skip the DCF / LLR / gating / L1-FISTA timings")
+    ap.add_argument("--next-only", action="store_true",
+                    help="only the DCF / LLR / gating / L1-FISTA timings (no CPU leg; for rocprofv3)")
     ap.add_argument("--profile", action="store_true", help="batched iterations only (for rocprofv3)")
     ap.add_argument("--small", action="store_true", help="128^3 x 8 coils, 4096 x 64 radial (os 1.5, W=4)")
     ap.add_argument("--rocprof-summary", metavar="DIR", help="summarise rocprofv3 CSV output and exit")
@@ -118,6 +120,11 @@ def main():
         t0 = time.time()
         op.autotune()
         log("autotune %.1fs" % (time.time() - t0))
+
+    if a.next_only:
+        a.no_cpu = True
+        print(json.dumps({"next_rows": next_rows(a, op, N, kx, ky, kz, M, log)}))
+        return
 
     # ---- timed region: batched, graph-captured CG --------------------------
     X = op.cg_init()
