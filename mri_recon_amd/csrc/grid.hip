@@ -556,6 +556,11 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
   __shared__ int s_rp[4][SG_MAXROWS + 1];    // visits before each row
   __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
   __shared__ float2 s_d[4][SG_BATCH * CCT];  // [sample][coil] data
+  extern __shared__ float tab_s[];           // kernel tables (g.ntab floats)
+  // tables in LDS: a lookup waits for LDS, not for the L1/L2 round trip
+  // (before any wavefront can leave: the whole workgroup reaches the barrier)
+  load_tables(tab_s, tables, g.ntab);
+  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
@@ -616,7 +621,15 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
       }
       i = (int64_t)s_rs[wave][lo] + (v - s_rp[wave][lo]);
     }
-    const float4 k = ok ? coords[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // every lane loads (past the end: sample 0, a valid index), so that the
+    // coordinate and coil loads of the batch are in flight together
+    const float4 kraw = coords[i];
+    float2 dv[CPL];
+    if (q * CPL < CCT) {
+#pragma unroll
+      for (int c = 0; c < CPL; c++) dv[c] = kch[i * CCT + q * CPL + c];
+    }
+    const float4 k = ok ? kraw : make_float4(0.f, 0.f, 0.f, 0.f);
     const float dk[3] = {k.x, k.y, k.z};
     unsigned long long hit = ~0ull;
 #pragma unroll
@@ -625,7 +638,7 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
       const int l = t0[a] + q;
       bool in = ok && axis_range(g, a, dk[a], st, en);
       in = in && l >= st && l <= en;
-      s_w[wave][sl * 12 + a * 4 + q] = in ? tap_weight(g, a, tables + g.toff[a], dk[a], l) : 0.0f;
+      s_w[wave][sl * 12 + a * 4 + q] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
       unsigned long long m = __ballot(in);
       m |= m >> 1;
       m |= m >> 2;
@@ -634,17 +647,18 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
     hit &= 0x1111111111111111ull;
     if (q * CPL < CCT && (ok || MM)) {  // MM reads every sample of the batch: zeros past the end
 #pragma unroll
-      for (int c = 0; c < CPL; c++)
-        s_d[wave][sl * CCT + q * CPL + c] = ok ? kch[i * CCT + q * CPL + c] : make_float2(0.f, 0.f);
+      for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = ok ? dv[c] : make_float2(0.f, 0.f);
     }
     wave_sync();
     // ---- B: accumulate the samples that reach the tile
     if constexpr (MM) {
       const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
       const int c0 = lane & 31, c1 = c0 + 32;
+      // no per-pair skip: straight-line code lets every LDS read of the batch
+      // issue before the first product (a sample off the tile has weight 0)
+      if (hit)
 #pragma unroll
       for (int kt = 0; kt < SG_BATCH / 2; kt++) {
-        if (((hit >> (8 * kt)) & 0x11ull) == 0) continue;  // neither sample reaches the tile
         const int smp = 2 * kt + (lane >> 5);
         const float *wp = &s_w[wave][smp * 12];
         const float b = dflat[smp * 2 * CCT + c0];
@@ -926,7 +940,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   const Tiles &t = op->t;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   dim3 g3((fr.nsub + 3) / 4, op->nch);
-  const size_t sh = 0;
+  const size_t sh = sizeof(float) * g.ntab;  // kernel tables in LDS
   const int rows_axis = (t.T + t.NT - 2) / t.B + 2;  // bins per axis a tile can reach
   if (rows_axis * rows_axis > SG_MAXROWS) return fail(op, MRN_EUNSUPPORTED, "scatter neighbourhood too large");
   if (g.CC == 16 && !op->scatter_valu) {
