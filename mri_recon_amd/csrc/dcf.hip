@@ -2,6 +2,12 @@
 // (recon_lib.cpp:854-917) over DCFgridFFT's real-valued polynomial-kernel
 // gridder (DCFgridFFT.cpp:96-440).
 //
+// The samples are first sorted by 8x8x8-cell brick of the grid (one device
+// radix sort): a wavefront's samples then touch neighbouring cells, and the
+// scatter's atomics and the gather's reads hit lines already in L2.  The
+// iterations run on the sorted copy; the weights are put back in the
+// caller's order at the end.
+//
 // The trajectory stays resident; one call runs every iteration on the stream:
 //   kw = 1; repeat iters: X = 0; X += A^T kw (scatter); kw2 = A X (gather);
 //   kw = |kw2| < 1e-3 ? 0 : kw / kw2;   then kw = |kw * scale^3|.
@@ -25,6 +31,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+
+#include <hipcub/hipcub.hpp>
 
 #include "mrn.h"
 
@@ -233,6 +241,42 @@ __global__ void k_dcf_finish(float *__restrict__ kw, const float *__restrict__ k
   kw[i] = fabsf((float)__dmul_rn((double)kw[i], __dmul_rn(__dmul_rn(s, s), s)));
 }
 
+// sort key: the 8^3-cell brick of the sample's nearest grid cell
+__global__ void k_dcf_keys(const float *__restrict__ kx, const float *__restrict__ ky,
+                           const float *__restrict__ kz, int64_t M, DcfGeom g, uint32_t *__restrict__ keys,
+                           uint32_t *__restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  float k[3];
+  kr_of(g, kx[i], ky[i], kz[i], k);
+  uint32_t key = 0;
+  for (int a = 2; a >= 0; a--) {
+    const float dk = __fadd_rn(__fmul_rn(k[a], g.gx), g.c[a]);
+    int c = (int)floorf(dk);
+    c = c < 0 ? 0 : (c > g.S[a] - 1 ? g.S[a] - 1 : c);
+    key = key * (uint32_t)((g.S[a] + 7) >> 3) + (uint32_t)(c >> 3);
+  }
+  keys[i] = key;
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void k_dcf_gather3(const uint32_t *__restrict__ perm, const float *__restrict__ kx,
+                              const float *__restrict__ ky, const float *__restrict__ kz, int64_t M,
+                              float *__restrict__ sx, float *__restrict__ sy, float *__restrict__ sz) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const uint32_t i = perm[j];
+  sx[j] = kx[i];
+  sy[j] = ky[i];
+  sz[j] = kz[i];
+}
+
+__global__ void k_dcf_unsort(const uint32_t *__restrict__ perm, const float *__restrict__ ws, int64_t M,
+                             float *__restrict__ kw) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) kw[perm[j]] = ws[j];
+}
+
 int64_t nblocks(int64_t n, int b) { return (n + b - 1) / b; }
 
 }  // namespace
@@ -306,11 +350,15 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
   g.acc = p->acc;
   g.len = len;
   if (M == 0) return MRN_OK;
+  if (M > 0x7fffffffll) return dcf_fail(MRN_EUNSUPPORTED, "more than 2^31 - 1 samples");
   hipStream_t st = (hipStream_t)stream;
   const int64_t nS = (int64_t)g.S[0] * g.S[1] * g.S[2];
   float *tab_h = new float[len];
   dcf_table(p->dwin, p->kernel, tab_h);
-  float *X = nullptr, *tab = nullptr;
+  float *X = nullptr, *tab = nullptr, *srt = nullptr;  // srt: sorted kx, ky, kz, kw
+  uint32_t *keys = nullptr, *perm = nullptr;           // [2][M] each
+  void *tmp = nullptr;
+  size_t tb = 0;
   double *dsum = nullptr;
   int rc = MRN_OK;
   auto run = [&]() -> int {
@@ -320,7 +368,23 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
     DCF_HIP(hipMemcpyAsync(tab, tab_h, sizeof(float) * len, hipMemcpyHostToDevice, st));
     const size_t lds = sizeof(float) * len;
     const int fillb = 4096;
-    hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, kw, M, 1.0f);
+    const dim3 gs((unsigned)nblocks(M, 256));
+    // brick order
+    DCF_HIP(hipMalloc(&keys, sizeof(uint32_t) * 2 * M));
+    DCF_HIP(hipMalloc(&perm, sizeof(uint32_t) * 2 * M));
+    DCF_HIP(hipMalloc(&srt, sizeof(float) * 4 * M));
+    hipLaunchKernelGGL(k_dcf_keys, gs, dim3(256), 0, st, kx, ky, kz, M, g, keys, perm);
+    DCF_HIP(hipGetLastError());
+    const uint64_t nkeys = (uint64_t)((g.S[0] + 7) >> 3) * ((g.S[1] + 7) >> 3) * ((g.S[2] + 7) >> 3);
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < nkeys) bits++;
+    DCF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys + M, perm, perm + M, (int)M, 0, bits, st));
+    DCF_HIP(hipMalloc(&tmp, tb));
+    DCF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys + M, perm, perm + M, (int)M, 0, bits, st));
+    const uint32_t *order = perm + M;
+    float *sx = srt, *sy = srt + M, *sz = srt + 2 * M, *sw = srt + 3 * M;
+    hipLaunchKernelGGL(k_dcf_gather3, gs, dim3(256), 0, st, order, kx, ky, kz, M, sx, sy, sz);
+    hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, sw, M, 1.0f);
     DCF_HIP(hipGetLastError());
     if (p->iters > 0) {
       // zero the per-iteration sums (a double 0 is two float 0s)
@@ -328,15 +392,15 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
                          (int64_t)2 * p->iters, 0.0f);
       DCF_HIP(hipGetLastError());
     }
-    const dim3 gs((unsigned)nblocks(M, 256));
     for (int it = 0; it < p->iters; it++) {
       hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, X, nS, 0.0f);
-      hipLaunchKernelGGL(k_dcf_scatter, gs, dim3(256), lds, st, X, kw, kx, ky, kz, M, g, tab);
-      hipLaunchKernelGGL(k_dcf_gather_update, gs, dim3(256), lds, st, X, kw, kx, ky, kz, M, g, tab,
+      hipLaunchKernelGGL(k_dcf_scatter, gs, dim3(256), lds, st, X, sw, sx, sy, sz, M, g, tab);
+      hipLaunchKernelGGL(k_dcf_gather_update, gs, dim3(256), lds, st, X, sw, sx, sy, sz, M, g, tab,
                          sums ? dsum + it : nullptr);
       DCF_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_dcf_finish, gs, dim3(256), 0, st, kw, kx, ky, kz, M, g);
+    hipLaunchKernelGGL(k_dcf_finish, gs, dim3(256), 0, st, sw, sx, sy, sz, M, g);
+    hipLaunchKernelGGL(k_dcf_unsort, gs, dim3(256), 0, st, order, sw, M, kw);
     DCF_HIP(hipGetLastError());
     if (sums && p->iters > 0)
       DCF_HIP(hipMemcpyAsync(sums, dsum, sizeof(double) * p->iters, hipMemcpyDeviceToHost, st));
@@ -348,6 +412,10 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
   hipFree(X);
   hipFree(tab);
   hipFree(dsum);
+  hipFree(keys);
+  hipFree(perm);
+  hipFree(srt);
+  hipFree(tmp);
   delete[] tab_h;
   return rc;
 }
