@@ -341,7 +341,8 @@ int mrn_pregate(const mrn_gate_params *p, int64_t xres, int64_t R, const double 
 
 const char *mrn_status_string(int status);
 /* Message of the op's last failure ("" if none); for op = NULL, the message of
- * this thread's last failed call that takes no op (mrn_dcf_*). */
+ * this thread's last failed call that takes no op (mrn_op_create, mrn_dcf_*,
+ * mrn_llr, mrn_gate_*, mrn_pregate). */
 const char *mrn_last_error(const mrn_op *op);
 
 #ifdef __cplusplus

@@ -216,6 +216,7 @@ int mrn_parse_args(mrn_params *p, int argc, const char **argv) {
 int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   if (!pin || !out) return MRN_EINVAL;
   *out = nullptr;
+  mrn::free_err_clear();
   const mrn_params &p = *pin;
   for (int a = 0; a < 3; a++)
     if (p.N[a] <= 0) return MRN_EINVAL;
@@ -250,7 +251,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
     g.dwin[a] = (p.dwin[a] == -1.0f) ? default_dwin(p.kernel) : p.dwin[a];
     if (!(g.dwin[a] > 0.0f) || g.dwin[a] > 3.5f) {
       delete op;
-      return MRN_EUNSUPPORTED;
+      return mrn::free_fail(MRN_EUNSUPPORTED, "kernel half width must lie in (0, 3.5]");
     }
   }
   g.k_rad = p.k_rad;
@@ -302,18 +303,20 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->scatter_valu = p.scatter_valu != 0;
   if (4 * (g.ntab + 4) > 64 * 1024) {
     delete op;
-    return MRN_EUNSUPPORTED;
+    return mrn::free_fail(MRN_EUNSUPPORTED, "kernel tables larger than 64 KB of LDS");
   }
   if (batched) {
     for (int a = 0; a < 3; a++) {
       if (!g.fft_in[a] || !fft_size_supported(g.S[a])) {
         delete op;
-        return MRN_EUNSUPPORTED;  // the plain form covers these geometries
+        return mrn::free_fail(MRN_EUNSUPPORTED,
+                              "the batched form needs a transform on every axis and grid sizes with "
+                              "factors 2, 3 and 5; the plain form covers this geometry");
       }
     }
     if (op->t.nbins > (1 << 30)) {
       delete op;
-      return MRN_EUNSUPPORTED;
+      return mrn::free_fail(MRN_EUNSUPPORTED, "grid too large for the sample bins");
     }
   }
   op->frames.resize(op->F);
