@@ -13,6 +13,10 @@
 // Errors: the reference's methods return void and print or exit; these throw
 // mrn::Error (status code and message) and never exit.
 //
+// Beside the operator: mrn::dcf_calc (RECON::dcf_calc), mrn::LowRankFrames
+// (LOWRANKCOIL's update_threshold + thresh over frames) and mrn::GatedFrames
+// (GATING's time scaling and RECON::pregate_data).
+//
 // mrn::SenseOperator is the batched form: all coils (and gated frames) of one
 // acquisition resident on the device, binned gridding, and the SENSE-CG /
 // power-method / FISTA loops of recon_lib.cpp (1523-2212) with FP64 scalars on
@@ -33,10 +37,12 @@ class Error : public std::runtime_error {
   int status;
 };
 
+// op = nullptr: the message of the calling thread's last op-less failure
 inline void check(int s, const mrn_op *op, const char *what) {
   if (s == MRN_OK) return;
   std::string m = std::string(what) + ": " + mrn_status_string(s);
-  if (op && mrn_last_error(op)[0]) m += std::string(" (") + mrn_last_error(op) + ")";
+  const char *le = mrn_last_error(op);
+  if (le && le[0]) m += std::string(" (") + le + ")";
   throw Error(s, m);
 }
 
@@ -251,6 +257,77 @@ class SenseOperator {
 
  private:
   mrn_op *op_ = nullptr;
+};
+
+// Pipe density compensation of one encoding (RECON::dcf_calc,
+// recon_lib.cpp:854-917): kw (device, M) receives the weights; returns the
+// per-iteration sums of |kw| the reference prints.
+inline std::vector<double> dcf_calc(const mrn_dcf_params &p, int64_t M, const float *kx, const float *ky,
+                                    const float *kz, float *kw, void *stream = nullptr) {
+  std::vector<double> sums(p.iters > 0 ? p.iters : 1);
+  check(mrn_dcf_calc(&p, M, kx, ky, kz, kw, sums.data(), stream), nullptr, "dcf_calc");
+  sums.resize(p.iters > 0 ? p.iters : 0);
+  return sums;
+}
+
+// LOWRANKCOIL for an image of Nt x Ne frames (clear.cpp:233-556): each call is
+// update_threshold(X, dim, iter) followed by thresh(X, dim), as recon_lib does
+// after a gradient step (recon_lib.cpp:2117-2126).  The call counter numbers
+// the block-shift draws.
+class LowRankFrames {
+ public:
+  mrn_llr_params p;
+  void *stream = nullptr;
+  LowRankFrames() { mrn_llr_params_default(&p); }
+  LowRankFrames(int numarg, const char **pstring) {
+    mrn_llr_params_default(&p);
+    check(mrn_llr_parse_args(&p, numarg, pstring), nullptr, "LowRankFrames");
+  }
+  // returns smax (the median block singular value times the iteration factor)
+  float threshold(cfloat *X, int Nt, int Ne, int Nx, int Ny, int Nz, int dim, int iter) {
+    const int32_t N[3] = {Nx, Ny, Nz};
+    float smax = 0.f;
+    check(mrn_llr(&p, X, Nt, Ne, N, dim, iter, calls_++, &smax, stream), nullptr, "LowRankFrames::threshold");
+    return smax;
+  }
+
+ private:
+  int64_t calls_ = 0;
+};
+
+// GATING + RECON::pregate_data for one encoding: scale the per-readout gate
+// times on the host, then select each frame's samples on the device.
+class GatedFrames {
+ public:
+  mrn_gate_params p;
+  void *stream = nullptr;
+  double offset = 0.0, scale = 0.0, temporal_resolution = 0.0;
+  GatedFrames() { mrn_gate_params_default(&p); }
+  GatedFrames(int numarg, const char **pstring) {
+    mrn_gate_params_default(&p);
+    check(mrn_gate_parse_args(&p, numarg, pstring), nullptr, "GatedFrames");
+  }
+  // gating.cpp:880-1078, in place on host times (one per readout)
+  void scale_times(std::vector<double> &times) {
+    double info[3];
+    check(mrn_gate_scale_times(&p, (int64_t)times.size(), times.data(), info), nullptr, "scale_times");
+    offset = info[0];
+    scale = info[1];
+    temporal_resolution = info[2];
+  }
+  // samples per frame (outputs null) or the frame-major outputs (sized by the
+  // sum of the counts of a first call); device pointers throughout
+  std::vector<int64_t> pregate(int64_t xres, int64_t R, const double *gt, const float *resp, const float *kx,
+                               const float *ky, const float *kz, const float *kw, int C, const cfloat *kdata,
+                               float *kx_out = nullptr, float *ky_out = nullptr, float *kz_out = nullptr,
+                               float *kw_out = nullptr, cfloat *kdata_out = nullptr) {
+    std::vector<int64_t> counts(p.frames > 0 ? p.frames : 1);
+    check(mrn_pregate(&p, xres, R, gt, resp, kx, ky, kz, kw, C, kdata, counts.data(), kx_out, ky_out, kz_out,
+                      kw_out, kdata_out, stream),
+          nullptr, "pregate");
+    counts.resize(p.frames > 0 ? p.frames : 0);
+    return counts;
+  }
 };
 
 }  // namespace mrn

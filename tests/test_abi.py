@@ -74,7 +74,9 @@ def test_cpp_facade_compiles_against_header(tmp_path):
     src = tmp_path / "use.cpp"
     src.write_text('#include "mrn_gridfft.hpp"\n'
                    'int main() { mrn::gridFFT g; g.overgrid = 1.25f; mrn_params p; '
-                   'mrn_params_default(&p); return (int)(g.k_rad != p.k_rad); }\n')
+                   'mrn_params_default(&p); mrn::LowRankFrames l; mrn::GatedFrames gf; '
+                   'mrn_dcf_params d; mrn_dcf_params_default(&d); (void)&mrn::dcf_calc; '
+                   'return (int)(g.k_rad != p.k_rad) + l.p.block_iter * 0 + gf.p.frames * 0; }\n')
     r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
