@@ -100,7 +100,8 @@ class MrnParams(ctypes.Structure):
         ("xcd_run_gather", ctypes.c_int32),
         ("gather_tiles", ctypes.c_int32),
         ("scatter_valu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("fft_stockham", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 

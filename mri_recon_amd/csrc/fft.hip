@@ -346,7 +346,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z(
     if (lok && zi < a.N[2]) {
       const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
       xv[j] = X[r + xi];
-      sv[j] = sm[r * CC + gl];
+      sv[j] = sm[(((int64_t)y * a.N[0] + xi) * a.N[2] + zi) * CC + (gl & (CC - 1))];
       wzv[j] = wz[ogz + zi];
     }
   }
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   for (int j = 0; j < NJ; j++) {
     const int zi = z0 + 16 * j;
     smv[j] = make_float2(0.f, 0.f);
-    if (xok && zi < a.N[2]) smv[j] = sm[(((int64_t)zi * a.N[1] + y) * a.N[0] + xi) * CC + c];
+    if (xok && zi < a.N[2]) smv[j] = sm[(((int64_t)y * a.N[0] + xi) * a.N[2] + zi) * CC + c];
   }
   {
     const int p0 = z0;
@@ -457,6 +457,293 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
         X[r] = make_float2(o.x + t.x, o.y + t.y);
       }
     }
+  }
+}
+
+// ------------------------------------------------ four-step, register-resident
+// For S = 16 P (64 <= S <= 512) a workgroup still owns 16 lines, but thread
+// (l, t) = (tid % 16, tid / 16) keeps the P points p = t + 16 j of line l in
+// registers, and
+//   X[k1 + P k2] = sum_t W_16^(t k2) W_S^(t k1) sum_j x[t + 16 j] W_P^(j k1)
+// (W_n = exp(DIR 2 pi i / n)).  Rows: the P-point DFT over j in registers and
+// the twiddle W_S^(t k1); one LDS transpose; columns: thread (l, u) takes
+// k1 = u, u + 16, ... < P and runs the 16-point DFT over t in registers,
+// which gives the points k1 + P k2.  One LDS round trip per pass instead of
+// one per radix stage.  Butterflies and twiddles are those of fft_stage.
+
+// LDS row length: a multiple of 2 float2 (16-byte aligned columns), 2 mod 32
+// so that the 16 lines of a column read start in distinct banks.
+__host__ __device__ constexpr int fft4_ld(int S) { return S + (34 - S % 32) % 32; }
+__host__ __device__ constexpr bool fft4_ok(int S) { return S % 16 == 0 && S >= 64 && S <= 512; }
+
+// Stockham stages of a P-point DFT held in registers (natural order in and
+// out); W_P^m = tws[m TSC], tws the S-point table.
+template <int P, int DIR, int TSC, int STAGE, int NS>
+__device__ __forceinline__ void reg_fft(float2 *v, const float2 *tws) {
+  constexpr int R = radix_at(P, STAGE);
+  if constexpr (R != 0) {
+    constexpr int NBL = P / R, TSTEP = P / (NS * R);
+    float2 o[P];
+#pragma unroll
+    for (int j = 0; j < NBL; j++) {
+      float2 x[R];
+#pragma unroll
+      for (int k = 0; k < R; k++) x[k] = v[j + k * NBL];
+      const int jm = j % NS;
+#pragma unroll
+      for (int k = 1; k < R; k++)
+        if (jm * k != 0) {
+          float2 tw = tws[jm * k * TSTEP * TSC];
+          if (DIR > 0) tw.y = -tw.y;
+          x[k] = cmul(x[k], tw);
+        }
+      dft<R, DIR>(x);
+      const int d = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int k = 0; k < R; k++) o[d + k * NS] = x[k];
+    }
+#pragma unroll
+    for (int q = 0; q < P; q++) v[q] = o[q];
+    reg_fft<P, DIR, TSC, STAGE + 1, NS * R>(v, tws);
+  }
+}
+
+// Rows: DFT over j, twiddle, write column entries row[k1 * 16 + t].
+template <int S, int DIR>
+__device__ __forceinline__ void fft4_rows(float2 *v, int t, float2 *row, const float2 *tws) {
+  constexpr int P = S / 16;
+  reg_fft<P, DIR, 16, 0, 1>(v, tws);
+#pragma unroll
+  for (int k1 = 0; k1 < P; k1++) {
+    float2 x = v[k1];
+    if (k1 > 0) {
+      float2 tw = tws[t * k1];
+      if (DIR > 0) tw.y = -tw.y;
+      x = cmul(x, tw);
+    }
+    row[k1 * 16 + t] = x;
+  }
+}
+
+// Columns: the 16 entries of column k1 (16-byte aligned), DFT over t.
+template <int S, int DIR>
+__device__ __forceinline__ void fft4_cols(float2 *w, const float2 *col, const float2 *tws) {
+  const float4 *c4 = reinterpret_cast<const float4 *>(col);
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const float4 q = c4[n];
+    w[2 * n] = make_float2(q.x, q.y);
+    w[2 * n + 1] = make_float2(q.z, q.w);
+  }
+  reg_fft<16, DIR, S / 16, 0, 1>(w, tws);
+}
+
+// Sum over groups of `cc` adjacent lanes (1 < cc <= 16, a power of two) by
+// DPP inside rows of 16 lanes; every lane of a group gets the same sum, added
+// in the order of the xor butterflies 1, 2, 4, 8.  All lanes of a row must
+// be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float2 group_sum(float2 t, int cc) {
+  if (cc > 1) t = make_float2(dpp_add<0xB1>(t.x), dpp_add<0xB1>(t.y));    // quad_perm [1,0,3,2]
+  if (cc > 2) t = make_float2(dpp_add<0x4E>(t.x), dpp_add<0x4E>(t.y));    // quad_perm [2,3,0,1]
+  if (cc > 4) t = make_float2(dpp_add<0x141>(t.x), dpp_add<0x141>(t.y));  // row_half_mirror
+  if (cc > 8) t = make_float2(dpp_add<0x140>(t.x), dpp_add<0x140>(t.y));  // row_mirror
+  return t;
+}
+
+// Grid -> grid pass, lines as in k_fft_lines (any LinesArgs layout).
+template <int S, int DIR>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__ grid, LinesArgs a,
+                                                            const float2 *__restrict__ tw) {
+  constexpr int P = S / 16, LD = fft4_ld(S);
+  __shared__ float4 buf4[FFT_LINES * LD / 2];
+  __shared__ float2 tws[S];
+  float2 *buf = reinterpret_cast<float2 *>(buf4);
+  const int64_t base = a.off0 + (int64_t)blockIdx.y * a.ystride + (int64_t)blockIdx.z * a.zstride;
+  const int l = threadIdx.x & (FFT_LINES - 1), t = threadIdx.x >> 4;
+  const int gl = blockIdx.x * FFT_LINES + l;
+  const bool ok = gl < a.nlines;
+  const int r = ok ? gl / a.run : 0, cc = ok ? gl - r * a.run : 0;
+  float2 *ln = grid + base + (int64_t)r * a.rstride + cc;
+  float2 v[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) {
+    const int p = t + 16 * j;
+    v[j] = make_float2(0.f, 0.f);
+    if (ok && p >= a.ld0 && p < a.ld1) v[j] = ln[(int64_t)p * a.pstride];
+  }
+  load_twiddles<S>(tws, tw);
+  __syncthreads();
+  fft4_rows<S, DIR>(v, t, buf + l * LD, tws);
+  __syncthreads();
+  for (int k1 = t; k1 < P; k1 += 16) {
+    float2 w[16];
+    fft4_cols<S, DIR>(w, buf + l * LD + k1 * 16, tws);
+    if (ok) {
+#pragma unroll
+      for (int k2 = 0; k2 < 16; k2++) {
+        const int p = k1 + P * k2;
+        if (p >= a.st0 && p < a.st1) ln[(int64_t)p * a.pstride] = scl(w[k2], a.scale);
+      }
+    }
+  }
+}
+
+// k_fft_pre_z on the four-step transform: same workgroups, same products.
+template <int S>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
+    float2 *__restrict__ grid, const float2 *__restrict__ X, const float2 *__restrict__ smaps,
+    const float *__restrict__ wx, const float *__restrict__ wy, const float *__restrict__ wz,
+    PrePostArgs a, const float2 *__restrict__ tw) {
+  constexpr int P = S / 16, LD = fft4_ld(S);
+  __shared__ float4 buf4[FFT_LINES * LD / 2];
+  __shared__ float2 tws[S];
+  float2 *buf = reinterpret_cast<float2 *>(buf4);
+  const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
+  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);
+  const int nl = a.N[0] * CC;
+  const int ogz = a.og[2];
+  const int l = threadIdx.x & (FFT_LINES - 1), t = threadIdx.x >> 4;
+  const int gl = bx * FFT_LINES + l;
+  const bool lok = gl < nl;
+  const int xi = gl >> a.lgCC;
+  const float wyv = wy[a.og[1] + y];
+  const float wxv = lok ? wx[a.og[0] + xi] : 0.f;
+  const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
+  float2 xv[P], sv[P];
+  float wzv[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) {
+    const int zi = t + 16 * j - ogz;
+    if (lok && zi >= 0 && zi < a.N[2]) {
+      const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
+      xv[j] = X[r + xi];
+      sv[j] = sm[(((int64_t)y * a.N[0] + xi) * a.N[2] + zi) * CC + (gl & (CC - 1))];
+      wzv[j] = wz[ogz + zi];
+    }
+  }
+  float2 v[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) {
+    const int zi = t + 16 * j - ogz;
+    v[j] = make_float2(0.f, 0.f);
+    if (lok && zi >= 0 && zi < a.N[2]) {
+      const float wt = (wzv[j] * wyv) * wxv;
+      v[j] = cmul(make_float2(wt * xv[j].x, wt * xv[j].y), sv[j]);
+    }
+  }
+  load_twiddles<S>(tws, tw);
+  __syncthreads();
+  fft4_rows<S, -1>(v, t, buf + l * LD, tws);
+  __syncthreads();
+  float2 *gch = grid + (int64_t)ch * a.chunk_grid + ((int64_t)(a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl;
+  const int64_t pstride = (int64_t)a.S[1] * a.S[0] * CC;
+  for (int k1 = t; k1 < P; k1 += 16) {
+    float2 w[16];
+    fft4_cols<S, -1>(w, buf + l * LD + k1 * 16, tws);
+    if (lok) {
+#pragma unroll
+      for (int k2 = 0; k2 < 16; k2++) gch[(int64_t)(k1 + P * k2) * pstride] = scl(w[k2], a.scale);
+    }
+  }
+}
+
+// One column of k_fft_post_z4: the 16-point DFT, window x conj(coil map),
+// the sum over the CC coils of a voxel (lanes of a row), X (+)= at the crop z.
+template <int S>
+__device__ __forceinline__ void post_col(float2 *__restrict__ X, const float2 *col, const float2 *tws,
+                                         const float *wzs, const float2 *smv, const float2 *xo, int k1,
+                                         float wyx, bool xok, int c, int64_t xr, int64_t zst,
+                                         const PrePostArgs &a) {
+  constexpr int P = S / 16;
+  float2 w[16];
+  fft4_cols<S, 1>(w, col, tws);
+#pragma unroll
+  for (int k2 = 0; k2 < 16; k2++) {
+    const int z = k1 + P * k2, zi = z - a.og[2];
+    const bool zok = zi >= 0 && zi < a.N[2];
+    float2 tt = make_float2(0.f, 0.f);
+    if (xok && zok) {
+      const float wt = wzs[z] * wyx;
+      const float2 im = scl(w[k2], a.scale);
+      tt = cmulc(make_float2(wt * im.x, wt * im.y), smv[k2]);
+    }
+    tt = group_sum(tt, a.CC);
+    if (xok && zok && c == 0) {
+      const int64_t r = xr + (int64_t)zi * zst;
+      X[r] = a.overwrite ? tt : make_float2(xo[k2].x + tt.x, xo[k2].y + tt.y);
+    }
+  }
+}
+
+// k_fft_post_z on the four-step transform: thread (l, u) ends with the z
+// points k1 + P k2 of its line (k1 = u, then u + 16 where P > 16).  The coil
+// maps of its first column (and X where it accumulates) load with the line,
+// before the transform; a second column's load when it starts.  The z window
+// sits in LDS beside the twiddles.
+template <int S>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
+    const float2 *__restrict__ grid, float2 *__restrict__ X, const float2 *__restrict__ smaps,
+    const float *__restrict__ wx, const float *__restrict__ wy, const float *__restrict__ wz,
+    PrePostArgs a, const float2 *__restrict__ tw) {
+  constexpr int P = S / 16, LD = fft4_ld(S);
+  __shared__ float4 buf4[FFT_LINES * LD / 2];
+  __shared__ float2 tws[S];
+  __shared__ float wzs[S];
+  float2 *buf = reinterpret_cast<float2 *>(buf4);
+  const int CC = a.CC, y = blockIdx.y, ch = a.ch;
+  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);
+  const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
+  const int nxl = FFT_LINES / CC;
+  const int ogz = a.og[2];
+  const int l = threadIdx.x & (FFT_LINES - 1), t = threadIdx.x >> 4;
+  const int c = l & (CC - 1), xl = l >> a.lgCC;
+  const int xi = bx * nxl + xl;
+  const bool xok = xi < a.N[0];
+  const int gl = g0 + l;
+  // maps of voxel column (y, xi), coil c: [z] at stride CC
+  const float2 *smc = smaps + (int64_t)ch * a.chunk_smap + (((int64_t)y * a.N[0] + xi) * a.N[2]) * CC + c;
+  const int64_t xr = (int64_t)y * a.N[0] + xi, zst = (int64_t)a.N[1] * a.N[0];
+  float2 smv[16], xo[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; k2++) {
+    const int zi = t + P * k2 - ogz;
+    smv[k2] = xo[k2] = make_float2(0.f, 0.f);
+    if (xok && t < P && zi >= 0 && zi < a.N[2]) {
+      smv[k2] = smc[(int64_t)zi * CC];
+      if (!a.overwrite && c == 0) xo[k2] = X[xr + (int64_t)zi * zst];
+    }
+  }
+  const float2 *src = grid + (int64_t)ch * a.chunk_grid + ((int64_t)(a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl;
+  const int64_t pstride = (int64_t)a.S[1] * a.S[0] * CC;
+  float2 v[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) {
+    v[j] = make_float2(0.f, 0.f);
+    if (gl < nl) v[j] = src[(int64_t)(t + 16 * j) * pstride];
+  }
+  load_twiddles<S>(tws, tw);
+  for (int q = threadIdx.x; q < S; q += FFT_THREADS) wzs[q] = wz[q];
+  __syncthreads();
+  fft4_rows<S, 1>(v, t, buf + l * LD, tws);
+  __syncthreads();
+  const float wyx = wy[a.og[1] + y] * (xok ? wx[a.og[0] + xi] : 0.f);
+  if (t < P)
+    post_col<S>(X, buf + l * LD + t * 16, tws, wzs, smv, xo, t, wyx, xok, c, xr, zst, a);
+  for (int k1 = t + 16; k1 < P; k1 += 16) {  // uniform over the 16 lanes of a row
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+      const int zi = k1 + P * k2 - ogz;
+      smv[k2] = xo[k2] = make_float2(0.f, 0.f);
+      if (xok && zi >= 0 && zi < a.N[2]) {
+        smv[k2] = smc[(int64_t)zi * CC];
+        if (!a.overwrite && c == 0) xo[k2] = X[xr + (int64_t)zi * zst];
+      }
+    }
+    post_col<S>(X, buf + l * LD + k1 * 16, tws, wzs, smv, xo, k1, wyx, xok, c, xr, zst, a);
   }
 }
 
@@ -515,13 +802,26 @@ bool fft_size_supported(int S) {
 int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 g3,
               const float2 *tw, hipStream_t st) {
   if (g3.x == 0 || g3.y == 0 || g3.z == 0) return MRN_OK;
+  // four-step kernel where the 16 lines of a workgroup share each point's
+  // 128-byte row (runs of at least 16 lines); the others keep k_fft_lines'
+  // layouts, which coalesce short runs
+  const bool four = a.run >= FFT_LINES && !op->fft_stockham;
   switch (S) {
-#define MRN_CASE_LINES(n)                                                                  \
-  case n:                                                                                  \
-    if (dir < 0)                                                                           \
-      hipLaunchKernelGGL((k_fft_lines<n, -1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw); \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_fft_lines<n, 1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw);  \
+#define MRN_CASE_LINES(n)                                                                    \
+  case n:                                                                                    \
+    if constexpr (fft4_ok(n)) {                                                              \
+      if (four) {                                                                            \
+        if (dir < 0)                                                                         \
+          hipLaunchKernelGGL((k_fft_lines4<n, -1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw); \
+        else                                                                                 \
+          hipLaunchKernelGGL((k_fft_lines4<n, 1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw);  \
+        break;                                                                               \
+      }                                                                                      \
+    }                                                                                        \
+    if (dir < 0)                                                                             \
+      hipLaunchKernelGGL((k_fft_lines<n, -1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw);   \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_fft_lines<n, 1>), g3, dim3(FFT_THREADS), 0, st, grid, a, tw);    \
     break;
     MRN_FFT_SIZES(MRN_CASE_LINES)
 #undef MRN_CASE_LINES
@@ -541,6 +841,13 @@ int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
   switch (a.S[2]) {
 #define MRN_CASE_PRE(n)                                                                    \
   case n:                                                                                  \
+    if constexpr (fft4_ok(n)) {                                                            \
+      if (!op->fft_stockham) {                                                             \
+        hipLaunchKernelGGL((k_fft_pre_z4<n>), g3, dim3(FFT_THREADS), 0, st, grid, X, smaps,  \
+                           op->win_d[0], op->win_d[1], op->win_d[2], a, op->tw_d[2]);       \
+        break;                                                                             \
+      }                                                                                    \
+    }                                                                                      \
     hipLaunchKernelGGL((k_fft_pre_z<n>), g3, dim3(FFT_THREADS), 0, st, grid, X, smaps,     \
                        op->win_d[0], op->win_d[1], op->win_d[2], a, op->tw_d[2]);          \
     break;
@@ -559,6 +866,13 @@ int fft_post_z(mrn_op *op, const float2 *grid, float2 *X, const float2 *smaps,
   switch (a.S[2]) {
 #define MRN_CASE_POST(n)                                                                   \
   case n:                                                                                  \
+    if constexpr (fft4_ok(n)) {                                                            \
+      if (!op->fft_stockham) {                                                             \
+        hipLaunchKernelGGL((k_fft_post_z4<n>), g3, dim3(FFT_THREADS), 0, st, grid, X, smaps, \
+                           op->win_d[0], op->win_d[1], op->win_d[2], a, op->tw_d[2]);       \
+        break;                                                                             \
+      }                                                                                    \
+    }                                                                                      \
     hipLaunchKernelGGL((k_fft_post_z<n>), g3, dim3(FFT_THREADS), 0, st, grid, X, smaps,    \
                        op->win_d[0], op->win_d[1], op->win_d[2], a, op->tw_d[2]);          \
     break;

@@ -489,6 +489,236 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float2 *__restrict__ 
   }
 }
 
+// Make this wavefront's LDS writes visible to its other lanes (LDS executes a
+// wavefront's instructions in order; the fence keeps the compiler from moving
+// LDS accesses across this point and waits for the writes).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+typedef float mrn_v16f __attribute__((ext_vector_type(16)));
+
+// Tile-staged gather on the matrix cores (16-coil blocks).  A workgroup
+// stages the cells that the home samples of work item gsubs[blockIdx.x] can
+// reach (R = NT + 3 per axis, x padded to the even RX) in LDS, as
+// k_gather_tile does; then each wavefront takes batches of 32 of those
+// samples and forms
+//   out[32 samples][32 reals] = W[32][cells] G[cells][32]
+// with v_mfma_f32_32x32x2f32, W the samples' separable cell weights (zero
+// outside their taps) and G the staged coil values.  Each staged value
+// serves the 32 samples of a batch in one operand read, where the per-lane
+// form reads every tap of every sample.  Step A: lane s < 32 locates sample
+// s and writes its weights per axis and offset to LDS as [axis][offset][s]
+// (consecutive lanes, consecutive banks), with its index, kw and taper.
+// Step B walks the cells as (z, y) rows of x pairs: lane l takes x = 2p +
+// l/32, so A = (wz wy) wx with wx fixed per lane for the whole batch (held
+// in registers) and wz wy fixed per row, and B = G[cell][l%32].  A sample
+// whose taps leave the region (NT + 1 taps on an axis, its coordinate
+// exactly on the window edge) gets zero weights and is computed afterwards
+// tap by tap from the grid.  The weight products and the epilogue
+// (residual, kw * taper) are those of k_gather; the sum over cells runs in
+// another order.
+template <int NT>
+__global__ __launch_bounds__(256) void k_gather_mm(const float2 *__restrict__ grid, float2 *__restrict__ out,
+                                                   const float4 *__restrict__ coords,
+                                                   const float2 *__restrict__ sub,
+                                                   const int32_t *__restrict__ bin_start,
+                                                   const int4 *__restrict__ gsubs, Geom g, Tiles t,
+                                                   const float *__restrict__ tables, int64_t chunk_grid,
+                                                   int64_t chunk_k, const float *__restrict__ taper,
+                                                   int weighted) {
+  constexpr int CCT = 16, HC = 8, R = NT + 3, RX = (R + 1) & ~1, XP = RX / 2;
+  extern __shared__ float4 reg[];  // [R][R][RX][HC]: cell c holds reals 32c .. 32c+31
+  __shared__ int s_rs[32], s_rp[33];
+  __shared__ float s_w[4][3][RX][32];  // [wave][axis][offset][sample]
+  __shared__ int64_t s_idx[4][32];     // internal sample index, -1 if none
+  __shared__ float s_kw[4][32], s_tp[4][32];
+  const int4 sp = gsubs[blockIdx.x];
+  const int tile = sp.x;
+  const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
+  const int t0[3] = {tix * 4, tiy * 4, tiz * 4};
+  const int ch = blockIdx.y;
+  const float4 *G4 = reinterpret_cast<const float4 *>(grid + ch * chunk_grid);
+  // ---- stage the region: every load of a thread issued before its LDS writes
+  constexpr int NE = R * R * RX * HC, RN = (NE + 255) / 256;
+  {
+    float4 v[RN];
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+      const int e = threadIdx.x + 256 * j;
+      const int q = e % HC, cell = e / HC;
+      const int rx = cell % RX, ry = (cell / RX) % R, rz = cell / (RX * R);
+      const int gx = t0[0] + rx, gy = t0[1] + ry, gz = t0[2] + rz;
+      v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e < NE && gx < g.S[0] && gy < g.S[1] && gz < g.S[2])
+        v[j] = G4[(((int64_t)gz * g.S[1] + gy) * g.S[0] + gx) * HC + q];
+    }
+#pragma unroll
+    for (int j = 0; j < RN; j++) {
+      const int e = threadIdx.x + 256 * j;
+      if (e < NE) reg[e] = v[j];
+    }
+  }
+  // ---- home rows of the tile and their prefix counts
+  int bx0, bx1, by0, by1, bz0, bz1;
+  home_bins(t, 0, tix, bx0, bx1);
+  home_bins(t, 1, tiy, by0, by1);
+  home_bins(t, 2, tiz, bz0, bz1);
+  const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
+  if (threadIdx.x < 64) {
+    const int r = threadIdx.x;
+    int n = 0;
+    if (r < nrows) {
+      const int brow = ((bz0 + r / nry) * t.nb[1] + by0 + r % nry) * t.nb[0];
+      const int s0 = bin_start[brow + bx0];
+      n = bin_start[brow + bx1 + 1] - s0;
+      s_rs[r] = s0;
+    }
+    int x = n;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (r >= o) x += y;
+    }
+    if (r < nrows) s_rp[r + 1] = x;
+    if (r == 0) s_rp[0] = 0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float *regf = reinterpret_cast<const float *>(reg);
+  const int col = lane & 31, kh = lane >> 5;  // MFMA row / column index, k half
+  for (int base = sp.y + 32 * wave; base < sp.z; base += 128) {
+    // ---- A: samples of the batch, their weights per axis
+    unsigned long long slow = 0;  // samples computed tap by tap
+    if (lane < 32) {
+      const int v = base + lane;
+      int64_t i = -1;
+      float kw = 0.f, tp = 0.f;
+      bool local = false;
+      float dk[3] = {0.f, 0.f, 0.f};
+      int s3[3] = {0, 0, 0}, n3[3] = {0, 0, 0};
+      if (v < sp.z) {
+        int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_rp[mid] <= v) lo = mid;
+          else hi = mid - 1;
+        }
+        i = (int64_t)s_rs[lo] + (v - s_rp[lo]);
+        const float4 k = coords[i];
+        kw = k.w;
+        tp = weighted ? taper[i] : 1.0f;
+        dk[0] = k.x;
+        dk[1] = k.y;
+        dk[2] = k.z;
+        local = true;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          int e;
+          axis_range(g, a, dk[a], s3[a], e);
+          n3[a] = e - s3[a] + 1;
+          local = local && n3[a] <= NT && s3[a] - t0[a] >= 0 && s3[a] - t0[a] + n3[a] <= R;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int r = 0; r < RX; r++) {
+          const int tt = r - (s3[a] - t0[a]);
+          const bool in = local && tt >= 0 && tt < n3[a];
+          s_w[wave][a][r][lane] = in ? tap_weight(g, a, tables + g.toff[a], dk[a], t0[a] + r) : 0.0f;
+        }
+      s_idx[wave][lane] = i;
+      s_kw[wave][lane] = kw;
+      s_tp[wave][lane] = tp;
+      slow = __ballot(i >= 0 && !local);
+    }
+    slow = __shfl(slow, 0);  // the wave-wide mask from lane 0
+    wave_sync();
+    // ---- B: out[32][32] = W[32][cells] G[cells][32]
+    float wx[XP];
+#pragma unroll
+    for (int p = 0; p < XP; p++) wx[p] = s_w[wave][0][2 * p + kh][col];
+    mrn_v16f acc0 = {}, acc1 = {};
+    for (int cz = 0; cz < R; cz++) {
+      const float wz = s_w[wave][2][cz][col];
+      for (int cy = 0; cy < R; cy++) {
+        const float wzy = wz * s_w[wave][1][cy][col];
+        const float *brow = regf + ((cz * R + cy) * RX + kh) * 32 + col;
+#pragma unroll
+        for (int p = 0; p < XP; p++) {
+          const float a = wzy * wx[p];
+          const float b = brow[64 * p];
+          if (p & 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc1, 0, 0, 0);
+          else acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc0, 0, 0, 0);
+        }
+      }
+    }
+    // ---- epilogue: lane holds sample 8(r/4) + 4 kh + r%4, real col
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int smp = 8 * (r >> 2) + 4 * kh + (r & 3);
+      const int64_t i = s_idx[wave][smp];
+      if (i < 0 || ((slow >> smp) & 1ull)) continue;
+      float val = acc0[r] + acc1[r];
+      float *o = reinterpret_cast<float *>(out + ch * chunk_k + i * CCT) + col;
+      if (sub) val -= reinterpret_cast<const float *>(sub + ch * chunk_k + i * CCT)[col];
+      if (weighted) val = (val * s_kw[wave][smp]) * s_tp[wave][smp];
+      *o = val;
+    }
+    // ---- samples whose taps leave the region: lane = coil, taps from the grid
+    while (slow) {
+      const int smp = __builtin_ctzll(slow);
+      slow &= slow - 1;
+      if (lane < CCT) {
+        const int64_t i = s_idx[wave][smp];
+        const float4 k = coords[i];
+        const float dk[3] = {k.x, k.y, k.z};
+        int s3[3], n3[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          int e;
+          axis_range(g, a, dk[a], s3[a], e);
+          n3[a] = e - s3[a] + 1;
+        }
+        const float2 *Gc = grid + ch * chunk_grid + lane;
+        float ar = 0.f, ai = 0.f;
+        for (int lz = s3[2]; lz < s3[2] + n3[2]; lz++) {
+          const float wz = tap_weight(g, 2, tables + g.toff[2], dk[2], lz);
+          for (int ly = s3[1]; ly < s3[1] + n3[1]; ly++) {
+            const float wzy = wz * tap_weight(g, 1, tables + g.toff[1], dk[1], ly);
+            const float2 *row = Gc + ((int64_t)lz * g.S[1] + ly) * g.S[0] * CCT;
+            for (int lx = s3[0]; lx < s3[0] + n3[0]; lx++) {
+              const float ww = wzy * tap_weight(g, 0, tables + g.toff[0], dk[0], lx);
+              const float2 q = row[(int64_t)lx * CCT];
+              ar += ww * q.x;
+              ai += ww * q.y;
+            }
+          }
+        }
+        const int64_t oi = ch * chunk_k + i * CCT + lane;
+        if (sub) {
+          const float2 d = sub[oi];
+          ar -= d.x;
+          ai -= d.y;
+        }
+        if (weighted) {
+          const float tp = taper[i];
+          ar = (ar * k.w) * tp;
+          ai = (ai * k.w) * tp;
+        }
+        out[oi] = make_float2(ar, ai);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// LDS bytes of the matrix-core gather's region (dynamic part)
+static size_t gather_mm_lds(int nt) {
+  const size_t R = (size_t)nt + 3, RX = (R + 1) & ~(size_t)1;
+  return R * R * RX * 16 * sizeof(float2);
+}
+
 // ------------------------------------------------------------ scatter
 __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Geom g, Tiles t,
                              int64_t chunk_grid) {
@@ -539,14 +769,6 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 // FP32 throughout; only the order of the additions differs.
 #define SG_BATCH 16
 #define SG_MAXROWS 144
-// Make this wavefront's LDS writes visible to its other lanes (LDS executes a
-// wavefront's instructions in order; the fence keeps the compiler from moving
-// LDS accesses across this point and waits for the writes).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-typedef float mrn_v16f __attribute__((ext_vector_type(16)));
 template <int CCT, bool MM>
 __global__ __launch_bounds__(256, 5) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
@@ -862,6 +1084,21 @@ static int launch_gather_tile(mrn_op *op, const float2 *grid, float2 *out, const
   return MRN_OK;
 }
 
+template <int NT>
+static int launch_gather_mm(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
+                            const float2 *sub, int wflag, hipStream_t st) {
+  static const hipError_t attr = hipFuncSetAttribute((const void *)k_gather_mm<NT>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+  MRN_HIP(op, attr);
+  const Geom &g = op->g;
+  int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
+  const size_t lds = gather_mm_lds(NT);
+  hipLaunchKernelGGL((k_gather_mm<NT>), dim3(fr.ngsub, op->nch), dim3(256), lds, st, grid, out, fr.coords, sub,
+                     fr.bin_start, fr.gsubs, g, op->t, op->tables_d, cg, ck, fr.taper, wflag);
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
 // Direct gather over all samples, or over the samples listed in `list`.
 static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
                                 const float2 *sub, int wflag, const int32_t *list, int64_t nlist,
@@ -903,13 +1140,13 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
     int rc = MRN_EUNSUPPORTED;
     switch (nt * 100 + g.CC) {
       case 308: rc = launch_gather_tile<3, 8>(op, grid, out, fr, sub, wf, lds, st); break;
-      case 316: rc = launch_gather_tile<3, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 316: rc = launch_gather_mm<3>(op, grid, out, fr, sub, wf, st); break;
       case 408: rc = launch_gather_tile<4, 8>(op, grid, out, fr, sub, wf, lds, st); break;
-      case 416: rc = launch_gather_tile<4, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 416: rc = launch_gather_mm<4>(op, grid, out, fr, sub, wf, st); break;
       case 508: rc = launch_gather_tile<5, 8>(op, grid, out, fr, sub, wf, lds, st); break;
-      case 516: rc = launch_gather_tile<5, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 516: rc = launch_gather_mm<5>(op, grid, out, fr, sub, wf, st); break;
       case 608: rc = launch_gather_tile<6, 8>(op, grid, out, fr, sub, wf, lds, st); break;
-      case 616: rc = launch_gather_tile<6, 16>(op, grid, out, fr, sub, wf, lds, st); break;
+      case 616: rc = launch_gather_tile<6, 16>(op, grid, out, fr, sub, wf, lds, st); break;  // region above 96 KB
       default: break;
     }
     if (rc != MRN_EUNSUPPORTED) {

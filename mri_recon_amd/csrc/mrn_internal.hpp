@@ -92,7 +92,7 @@ struct mrn_op {
   float *tables_d = nullptr;   // packed tables
   float *win_d[3] = {nullptr, nullptr, nullptr};
   float2 *tw_d[3] = {nullptr, nullptr, nullptr};  // forward twiddles exp(-2 pi i q / S)
-  float2 *smaps = nullptr;     // [nch][Nz][Ny][Nx][CC]
+  float2 *smaps = nullptr;     // batched: [nch][Ny][Nx][Nz][CC]; plain: [C][Nz][Ny][Nx]
   float2 *grid = nullptr;      // [nch][Sz][Sy][Sx][CC]
   float2 *cgrid = nullptr;     // plain form: one coil grid [Sz][Sy][Sx]
   float2 *kwork = nullptr;     // [nch][Mv_max][CC]
@@ -122,6 +122,7 @@ struct mrn_op {
   // L1 wavelet proximal step of FISTA (fraction 0 = none)
   bool gather_tiles = false;   // tile-staged gather over the dense tiles
   bool scatter_valu = false;   // VALU scatter accumulate (else matrix cores for 16-coil blocks)
+  bool fft_stockham = false;   // LDS Stockham FFT passes (else four-step where the size allows)
   float l1_fraction = 0.0f;
   int l1_levels[3] = {4, 4, 4};
   int l1_spins = 4;

@@ -36,16 +36,23 @@ __global__ void k_copy2(float2 *dst, const float2 *src, int64_t n) {
     dst[i] = src[i];
 }
 
-// caller coil maps [C][nvox] -> coil blocks [nch][nvox][CC] (zero padded)
-__global__ void k_smaps_in(float2 *dst, const float2 *src, int64_t nvox, int C, int CC, int lgCC,
-                           int64_t total) {
+// caller coil maps [C][z][y][x] -> coil blocks [nch][y][x][z][CC] (zero
+// padded).  z inside y and x: a z pass reads a voxel column's maps as one
+// contiguous run instead of one 8*CC-byte piece per z plane (and page).
+__global__ void k_smaps_in(float2 *dst, const float2 *src, int64_t nvox, int Nx, int Ny, int Nz, int C,
+                           int CC, int lgCC, int64_t total) {
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int cc = (int)(idx & (CC - 1));
-    const int64_t r = (idx >> lgCC) % nvox;
-    const int ch = (int)((idx >> lgCC) / nvox);
+    int64_t q = idx >> lgCC;
+    const int z = (int)(q % Nz);
+    q /= Nz;
+    const int x = (int)(q % Nx);
+    q /= Nx;
+    const int y = (int)(q % Ny);
+    const int ch = (int)(q / Ny);
     const int c = ch * CC + cc;
-    dst[idx] = c < C ? src[(int64_t)c * nvox + r] : make_float2(0.f, 0.f);
+    dst[idx] = c < C ? src[(int64_t)c * nvox + ((int64_t)z * Ny + y) * Nx + x] : make_float2(0.f, 0.f);
   }
 }
 
@@ -347,8 +354,8 @@ int op_copy(mrn_op *op, float2 *d, const float2 *s, int64_t n, hipStream_t st) {
 }
 int op_smaps_in(mrn_op *op, float2 *dst, const float2 *src, hipStream_t st) {
   const int64_t total = (int64_t)op->nch * op->nvox * op->g.CC;
-  hipLaunchKernelGGL(k_smaps_in, dim3(gs(total)), dim3(256), 0, st, dst, src, op->nvox, op->C,
-                     op->g.CC, op->g.lgCC, total);
+  hipLaunchKernelGGL(k_smaps_in, dim3(gs(total)), dim3(256), 0, st, dst, src, op->nvox, op->g.N[0],
+                     op->g.N[1], op->g.N[2], op->C, op->g.CC, op->g.lgCC, total);
   LAUNCH_CHECK(op);
   return MRN_OK;
 }

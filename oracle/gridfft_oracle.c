@@ -212,20 +212,26 @@ static void build_windows(orc_op *op) { /* gridFFT.cpp:550-631 */
 orc_op *orc_create(const orc_params *p) { /* gridFFT.cpp:475-643 */
   orc_op *op = (orc_op *)calloc(1, sizeof(orc_op));
   op->p = *p;
+  /* The geometry is the reference's float arithmetic in both builds: grid_x
+   * is a float member, Sx = (int)(grid_x * Nx), og = (int)((float)Nx *
+   * (grid_x - 1) / 2.0).  In double, 36 * (64/36 - 1) / 2 falls just below
+   * 14 and truncates to 13 -- another grid offset, not a rounding effect. */
+  float gf[3];
   for (int a = 0; a < 3; a++) {
     op->N[a] = p->N[a];
     op->dwin[a] = p->dwin[a];
-    real g = p->grid[a];
+    float g = (float)p->grid[a];
     if (p->grid_in[a] == 1) {
-      if (g == -1) g = (real)(16.0 * ceil(((float)p->overgrid * (float)p->N[a]) / 16.0) / (real)p->N[a]);
+      if (g == -1.0f) g = (float)(16.0 * ceil(((float)p->overgrid * (float)p->N[a]) / 16.0) / (float)p->N[a]);
     } else {
-      g = 1;
+      g = 1.0f;
     }
-    op->grid[a] = g;
+    gf[a] = g;
+    op->grid[a] = (real)g;
     op->S[a] = (int)(g * p->N[a]);
   }
   build_tables(op);
-  for (int a = 0; a < 3; a++) op->og[a] = (int)((real)op->N[a] * (op->grid[a] - 1) / 2.0);
+  for (int a = 0; a < 3; a++) op->og[a] = (int)((float)op->N[a] * (gf[a] - 1) / 2.0);
   build_windows(op);
   return op;
 }

@@ -79,3 +79,35 @@ def test_cartesian_kz_without_z_transform():
 @pytest.mark.parametrize("chunk", [1, 2, 4, 8])
 def test_coil_blocks(chunk):
     check(dict(overgrid=1.25, dwin=2.5, coil_chunk=chunk), 16, 8, 55, forms=("batched",))
+
+
+@pytest.mark.parametrize("N,C,os_", [(48, 16, 1.5), (64, 3, 1.5), (96, 2, 1.5), ((40, 48, 64), 5, 1.5),
+                                     (64, 16, 2.5), (128, 2, 2.0), (160, 1, 2.0)])
+def test_four_step_fft_vs_oracle_and_stockham(N, C, os_):
+    """The four-step register FFT (the default for grid edges S = 16 P,
+    64 <= S <= 512; here P = 5, 6, 9, 10, 16, 20, a non-cubic grid 64 x 80 x
+    96, coil blocks of 1, 2, 4, 8 and 16) meets the oracle's bar for A and
+    A^H (the module's: relative to the FP32 oracle's own distance from FP64
+    where that is larger, as at overgrid 2.5), and agrees to FP32 rounding
+    with the LDS Stockham passes (fft_stockham=True), which sum in another
+    order."""
+    traj, X, smaps, d = problem(N, C, 61)
+    o32 = orc.OracleOp(N, overgrid=os_, dwin=2.5, precision="f32")
+    assert all(s >= 64 and s % 16 == 0 for s in o32.S)
+    out = []
+    for stockham in (False, True):
+        op = GridFFT(N, ncoils=C, overgrid=os_, dwin=2.5, fft_stockham=stockham)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        out.append((op.backward(to_dev(X)).cpu().numpy(), op.forward(to_dev(d), accumulate=False).cpu().numpy()))
+        del op
+    o64 = orc.OracleOp(N, overgrid=os_, dwin=2.5, precision="f64")
+    A32, H32 = o32.image_to_kspace(X, smaps, traj), o32.kspace_to_image(d, smaps, traj)
+    A64 = o64.image_to_kspace(X.astype(np.complex128), smaps.astype(np.complex128), traj)
+    H64 = o64.kspace_to_image(d.astype(np.complex128), smaps.astype(np.complex128), traj)
+    bar_a, bar_h = max(1e-5, 1.5 * rel(A32, A64)), max(1e-5, 1.5 * rel(H32, H64))
+    a, h = out[0]
+    errs = (rel(a, A32), rel(a, A64), rel(h, H32), rel(h, H64), rel(a, out[1][0]), rel(h, out[1][1]))
+    print("four-step S=%s C=%d: A %.2e/%.2e AH %.2e/%.2e  vs Stockham %.2e %.2e" % (tuple(o32.S), C, *errs))
+    assert errs[0] <= bar_a and errs[1] <= 1e-4 and errs[2] <= bar_h and errs[3] <= 1e-4
+    assert errs[4] <= 1e-6 and errs[5] <= 1e-6

@@ -41,6 +41,23 @@ def test_geometry_kat(N, os_, S, og):
     assert abs(g["grid"][0] - S / N) < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("N,os_,S,og", [
+    (36, 1.5, 64, 14),  # float grid 64/36 rounds up: 36 * 0.77777779 / 2 = 14.0000002
+    (48, 1.5, 80, 15),  # float grid 80/48 rounds down: 48 * 0.66666663 / 2 = 15.999999
+    (40, 1.5, 64, 12),
+])
+def test_geometry_float_arithmetic_in_both_builds(N, os_, S, og, precision):
+    """grid_x is a float member in the reference (gridFFT.h), so the grid
+    offset og = (int)((float)N * (grid - 1) / 2.0) truncates the float value
+    in the FP64 oracle too: N = 48 gives 15, not the centred 16, and N = 36
+    gives 14 where double arithmetic would give 13."""
+    g32 = np.float32(16.0 * math.ceil(np.float32(os_) * np.float32(N) / 16.0) / N)
+    assert int(np.float32(N) * (g32 - np.float32(1)) / 2.0) == og
+    g = orc.OracleOp(N, overgrid=os_, precision=precision).geometry()
+    assert tuple(g["S"]) == (S, S, S) and tuple(g["og"]) == (og, og, og)
+
+
 @pytest.mark.parametrize("N,os_,dwin,beta", [
     (128, 1.5, 2.0, 7.779),    # beta_min dominates
     (256, 1.25, 2.5, 12.175),  # beta_min dominates (textbook Beatty with W=5 gives 8.996)
