@@ -101,7 +101,7 @@ class MrnParams(ctypes.Structure):
         ("gather_tiles", ctypes.c_int32),
         ("scatter_valu", ctypes.c_int32),
         ("fft_stockham", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("xcd_run_fft", ctypes.c_int32),
     ]
 
 

@@ -67,6 +67,7 @@ static PrePostArgs prepost(const mrn_op *op, int ch, int overwrite, int CC) {
   a.ch = ch;
   a.overwrite = overwrite;
   a.scale = fft_scale(op->g.S[2]);
+  a.xrun = op->xcd_run_fft;
   return a;
 }
 
@@ -90,6 +91,7 @@ static LinesArgs ypass_args(const mrn_op *op, bool inverse) {
   y.st0 = inverse ? g.og[1] : 0;
   y.st1 = inverse ? g.og[1] + g.N[1] : g.S[1];
   y.scale = fft_scale(g.S[1]);
+  y.xrun = op->xcd_run_fft;  // neighbouring 128-byte runs of a row per XCD
   return y;
 }
 

@@ -323,9 +323,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z(
   __shared__ float2 tws[S];
   load_twiddles<S>(tws, tw);
   const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
-  // 16 consecutive workgroups (x columns) per XCD: with 16 coils per cell a
-  // workgroup reads one x of the image, and X's 128-byte lines hold 16 x
-  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);
+  // runs of consecutive workgroups (x columns) per XCD: neighbouring 128-byte
+  // rows of the grid and of X share an L2
+  const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
   const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   const int ogz = a.og[2];
   zero_outside<S>(buf, ogz, ogz + a.N[2]);
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   __shared__ float2 tws[S];
   load_twiddles<S>(tws, tw);
   const int CC = a.CC, y = blockIdx.y, ch = a.ch;
-  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);  // as in k_fft_pre_z
+  const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);  // as in k_fft_pre_z
   const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   constexpr int NJ = (S + 15) / 16;
   const int nxl = FFT_LINES / CC;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
   float2 *buf = reinterpret_cast<float2 *>(buf4);
   const int64_t base = a.off0 + (int64_t)blockIdx.y * a.ystride + (int64_t)blockIdx.z * a.zstride;
   const int l = threadIdx.x & (FFT_LINES - 1), t = threadIdx.x >> 4;
-  const int gl = blockIdx.x * FFT_LINES + l;
+  const int gl = xcd_block(blockIdx.x, gridDim.x, a.xrun) * FFT_LINES + l;
   const bool ok = gl < a.nlines;
   const int r = ok ? gl / a.run : 0, cc = ok ? gl - r * a.run : 0;
   float2 *ln = grid + base + (int64_t)r * a.rstride + cc;
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
   __shared__ float2 tws[S];
   float2 *buf = reinterpret_cast<float2 *>(buf4);
   const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
-  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);
+  const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
   const int nl = a.N[0] * CC;
   const int ogz = a.og[2];
   const int l = threadIdx.x & (FFT_LINES - 1), t = threadIdx.x >> 4;
@@ -695,7 +695,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
   __shared__ float wzs[S];
   float2 *buf = reinterpret_cast<float2 *>(buf4);
   const int CC = a.CC, y = blockIdx.y, ch = a.ch;
-  const int bx = xcd_block(blockIdx.x, gridDim.x, 16);
+  const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
   const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   const int nxl = FFT_LINES / CC;
   const int ogz = a.og[2];
@@ -725,8 +725,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
     v[j] = make_float2(0.f, 0.f);
     if (gl < nl) v[j] = src[(int64_t)(t + 16 * j) * pstride];
   }
-  load_twiddles<S>(tws, tw);
-  for (int q = threadIdx.x; q < S; q += FFT_THREADS) wzs[q] = wz[q];
+  for (int q = threadIdx.x; q < S; q += FFT_THREADS) {
+    tws[q] = tw[q];
+    wzs[q] = wz[q];
+  }
   __syncthreads();
   fft4_rows<S, 1>(v, t, buf + l * LD, tws);
   __syncthreads();

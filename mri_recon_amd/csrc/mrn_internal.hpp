@@ -123,6 +123,7 @@ struct mrn_op {
   bool gather_tiles = false;   // tile-staged gather over the dense tiles
   bool scatter_valu = false;   // VALU scatter accumulate (else matrix cores for 16-coil blocks)
   bool fft_stockham = false;   // LDS Stockham FFT passes (else four-step where the size allows)
+  int xcd_run_fft = 8;         // y and z FFT pass workgroups per XCD run
   float l1_fraction = 0.0f;
   int l1_levels[3] = {4, 4, 4};
   int l1_spins = 4;
@@ -248,6 +249,7 @@ struct LinesArgs {
   int nlines;                      // valid lines in the group dimension
   int ld0, ld1, st0, st1;          // loaded / stored point ranges
   float scale;
+  int xrun;                        // workgroup groups per XCD run (xcd_block; 0/1 = hardware order)
 };
 
 struct PrePostArgs {
@@ -258,6 +260,7 @@ struct PrePostArgs {
   int ch;               // coil block (post: one launch per block)
   int overwrite;        // post: X = sum instead of X += sum
   float scale;
+  int xrun;             // x-column workgroups per XCD run (xcd_block)
 };
 
 // host-side launchers (defined in the .hip files)

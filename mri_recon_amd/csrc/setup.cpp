@@ -302,6 +302,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->gather_tiles = p.gather_tiles != 0;
   op->scatter_valu = p.scatter_valu != 0;
   op->fft_stockham = p.fft_stockham != 0;
+  op->xcd_run_fft = p.xcd_run_fft > 0 ? p.xcd_run_fft : 8;
   if (4 * (g.ntab + 4) > 64 * 1024) {
     delete op;
     return mrn::free_fail(MRN_EUNSUPPORTED, "kernel tables larger than 64 KB of LDS");

@@ -58,7 +58,8 @@ class GridFFT:
     def __init__(self, N, ncoils=1, nframes=1, overgrid=1.5, dwin=-1.0, kernel="kaiser",
                  grid_in=(1, 1, 1), fft_in=(1, 1, 1), grid=-1.0, grid_scale=1.0, k_rad=9999.0,
                  kernel_mode="table", form="batched", coil_chunk=0, tile=0, max_visits=0,
-                 bin_edge=0, xcd_run=0, xcd_run_gather=0, gather_tiles=False, scatter_valu=False, fft_stockham=False, args=None):
+                 bin_edge=0, xcd_run=0, xcd_run_gather=0, gather_tiles=False, scatter_valu=False, fft_stockham=False, xcd_run_fft=0,
+                 args=None):
         L = lib()
         p = MrnParams()
         L.mrn_params_default(ctypes.byref(p))
@@ -86,6 +87,7 @@ class GridFFT:
         p.gather_tiles = int(bool(gather_tiles))
         p.scatter_valu = int(bool(scatter_valu))
         p.fft_stockham = int(bool(fft_stockham))
+        p.xcd_run_fft = int(xcd_run_fft)
         if args:  # gridFFT::read_commandline flags
             arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
             check(L.mrn_parse_args(ctypes.byref(p), len(args), arr))
