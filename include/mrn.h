@@ -72,6 +72,7 @@ typedef struct mrn_params {
   int32_t scatter_valu;  /* 1 = vector-ALU scatter accumulate instead of the matrix cores (default 0) */
   int32_t fft_stockham;  /* 1 = LDS Stockham FFT passes instead of the four-step register form (default 0) */
   int32_t xcd_run_fft;   /* y and z FFT pass workgroups per XCD run (1 = hardware order); 0 = auto (8) */
+  int32_t gather_coils;  /* coils per lane in the gather: 1, 2, 4 or 8; 0 = auto */
 } mrn_params;
 
 typedef struct mrn_op mrn_op;

@@ -102,6 +102,7 @@ class MrnParams(ctypes.Structure):
         ("scatter_valu", ctypes.c_int32),
         ("fft_stockham", ctypes.c_int32),
         ("xcd_run_fft", ctypes.c_int32),
+        ("gather_coils", ctypes.c_int32),
     ]
 
 

@@ -226,6 +226,20 @@ struct CoilVec<2> {
     return r;
   }
 };
+template <int NC>  // NC = 4, 8: NC / 2 16-byte loads of consecutive coils
+struct CoilVec {
+  float2 v[NC];
+  __device__ __forceinline__ static CoilVec load(const float2 *p) {
+    CoilVec r;
+#pragma unroll
+    for (int h = 0; h < NC / 2; h++) {
+      const float4 q = reinterpret_cast<const float4 *>(p)[h];
+      r.v[2 * h] = make_float2(q.x, q.y);
+      r.v[2 * h + 1] = make_float2(q.z, q.w);
+    }
+    return r;
+  }
+};
 
 template <int NT, int NC>
 __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
@@ -241,7 +255,8 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
   const int64_t gid = (int64_t)xcd_block(blockIdx.x, gridDim.x, xrun) * blockDim.x + threadIdx.x;
-  const int lgl = g.lgCC - (NC == 2 ? 1 : 0);  // log2 of the lanes per sample
+  constexpr int LGNC = NC == 8 ? 3 : NC == 4 ? 2 : NC == 2 ? 1 : 0;
+  const int lgl = g.lgCC - LGNC;  // log2 of the lanes per sample
   const int c = (int)(gid & ((1 << lgl) - 1)) * NC;
   int64_t i = gid >> lgl;  // sample (or position in `list` when given)
   if (list) {
@@ -1104,19 +1119,24 @@ static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, con
                                 const float2 *sub, int wflag, const int32_t *list, int64_t nlist,
                                 hipStream_t st) {
   const Geom &g = op->g;
-  const int nc = g.CC >= 2 ? 2 : 1;  // coils per lane
+  // coils per lane: 4 by default.  A wave then serves 16 samples (4 lanes
+  // each) instead of 8, whose taps overlap more, so fewer of its loads miss
+  // the L1 (16-coil blocks at 256^3: 7.8 ms with 2 coils per lane, 6.8 with
+  // 4, 13.5 with 8, 9.9 with 1)
+  int nc = op->gather_coils > 0 ? op->gather_coils : 4;
+  if (nc > g.CC) nc = g.CC;
   const int64_t n = list ? nlist : fr.Mv;
   if (n == 0) return MRN_OK;
   dim3 g3(nblk(n * (g.CC / nc), 256), op->nch);
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
-  switch (taps_main(g) * 2 + (nc - 1)) {
+  switch (taps_main(g) * 16 + nc) {
 #define MRN_G1(n, c)                                                                                \
-  case n * 2 + c - 1:                                                                               \
+  case n * 16 + c:                                                                                  \
     hipLaunchKernelGGL((k_gather<n, c>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,     \
                        fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag,            \
                        op->t.xrun_gather, list, nlist);                                             \
     break;
-#define MRN_G(n) MRN_G1(n, 1) MRN_G1(n, 2)
+#define MRN_G(n) MRN_G1(n, 1) MRN_G1(n, 2) MRN_G1(n, 4) MRN_G1(n, 8)
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
 #undef MRN_G
 #undef MRN_G1

@@ -124,6 +124,7 @@ struct mrn_op {
   bool scatter_valu = false;   // VALU scatter accumulate (else matrix cores for 16-coil blocks)
   bool fft_stockham = false;   // LDS Stockham FFT passes (else four-step where the size allows)
   int xcd_run_fft = 8;         // y and z FFT pass workgroups per XCD run
+  int gather_coils = 0;        // coils per lane in the direct gather (0 = auto)
   float l1_fraction = 0.0f;
   int l1_levels[3] = {4, 4, 4};
   int l1_spins = 4;

@@ -303,6 +303,12 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->scatter_valu = p.scatter_valu != 0;
   op->fft_stockham = p.fft_stockham != 0;
   op->xcd_run_fft = p.xcd_run_fft > 0 ? p.xcd_run_fft : 8;
+  if (p.gather_coils != 0 && p.gather_coils != 1 && p.gather_coils != 2 && p.gather_coils != 4 &&
+      p.gather_coils != 8) {
+    delete op;
+    return mrn::free_fail(MRN_EINVAL, "gather_coils must be 0 (auto), 1, 2, 4 or 8");
+  }
+  op->gather_coils = p.gather_coils;
   if (4 * (g.ntab + 4) > 64 * 1024) {
     delete op;
     return mrn::free_fail(MRN_EUNSUPPORTED, "kernel tables larger than 64 KB of LDS");

@@ -111,3 +111,23 @@ def test_four_step_fft_vs_oracle_and_stockham(N, C, os_):
     print("four-step S=%s C=%d: A %.2e/%.2e AH %.2e/%.2e  vs Stockham %.2e %.2e" % (tuple(o32.S), C, *errs))
     assert errs[0] <= bar_a and errs[1] <= 1e-4 and errs[2] <= bar_h and errs[3] <= 1e-4
     assert errs[4] <= 1e-6 and errs[5] <= 1e-6
+
+
+@pytest.mark.parametrize("C", [16, 8, 3])
+def test_gather_coils_per_lane_same_bits(C):
+    """The direct gather with 1, 2, 4 or 8 coils per lane (default 4; capped
+    at the coil block) sums every coil's taps in the same order: A and the
+    weighted gather inside E'E give the same bits for every setting."""
+    traj, X, smaps, d = problem(32, C, 71)
+    outs = []
+    for nc in (1, 2, 4, 8):
+        op = GridFFT(32, ncoils=C, overgrid=1.5, dwin=2.5, gather_coils=nc)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        outs.append((op.backward(to_dev(X)).cpu().numpy(), op.normal(to_dev(X)).cpu().numpy()))
+    for a, n in outs[1:]:
+        assert np.array_equal(a, outs[0][0]) and np.array_equal(n, outs[0][1])
+    o32 = orc.OracleOp(32, overgrid=1.5, dwin=2.5, precision="f32")
+    assert rel(outs[2][0], o32.image_to_kspace(X, smaps, traj)) <= 1e-5
+    with pytest.raises(MrnError, match="gather_coils"):
+        GridFFT(32, ncoils=C, gather_coils=3)
