@@ -844,28 +844,35 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
   mrn_v16f acc0 = {}, acc1 = {};  // MM: cells 0-31 and 32-63
   const int sl = lane >> 2, q = lane & 3;
   constexpr int CPL = CCT >= 4 ? CCT / 4 : 1;  // coil values a lane stages
-  for (int base = vbeg; base < vend; base += SG_BATCH) {
-    // ---- A: weights and data of the batch
-    const int v = base + sl;
-    const bool ok = v < vend;
-    int64_t i = 0;
-    if (ok) {
-      int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_rp[wave][mid] <= v) lo = mid;
-        else hi = mid - 1;
-      }
-      i = (int64_t)s_rs[wave][lo] + (v - s_rp[wave][lo]);
+  // internal index of visit v of this work item (0 past the end: a valid
+  // index, so that every lane loads and a batch's loads fly together)
+  auto visit_index = [&](int v) -> int64_t {
+    if (v >= vend) return 0;
+    int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_rp[wave][mid] <= v) lo = mid;
+      else hi = mid - 1;
     }
-    // every lane loads (past the end: sample 0, a valid index), so that the
-    // coordinate and coil loads of the batch are in flight together
-    const float4 kraw = coords[i];
-    float2 dv[CPL];
+    return (int64_t)s_rs[wave][lo] + (v - s_rp[wave][lo]);
+  };
+  // the batch's coordinates and coil values; MM loads the next batch's while
+  // the matrix cores work on this one
+  float4 kraw;
+  float2 dv[CPL];
+  auto load_batch = [&](int64_t i) {
+    kraw = coords[i];
     if (q * CPL < CCT) {
 #pragma unroll
       for (int c = 0; c < CPL; c++) dv[c] = kch[i * CCT + q * CPL + c];
     }
+  };
+  if (MM && vbeg < vend) load_batch(visit_index(vbeg + sl));
+  for (int base = vbeg; base < vend; base += SG_BATCH) {
+    // ---- A: weights and data of the batch
+    const int v = base + sl;
+    const bool ok = v < vend;
+    if (!MM) load_batch(visit_index(v));
     const float4 k = ok ? kraw : make_float4(0.f, 0.f, 0.f, 0.f);
     const float dk[3] = {k.x, k.y, k.z};
     unsigned long long hit = ~0ull;
@@ -886,13 +893,16 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
 #pragma unroll
       for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = ok ? dv[c] : make_float2(0.f, 0.f);
     }
+    if (MM && base + SG_BATCH < vend) load_batch(visit_index(base + SG_BATCH + sl));
     wave_sync();
     // ---- B: accumulate the samples that reach the tile
     if constexpr (MM) {
       const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
       const int c0 = lane & 31, c1 = c0 + 32;
-      // no per-pair skip: straight-line code lets every LDS read of the batch
-      // issue before the first product (a sample off the tile has weight 0)
+      // no per-pair skip: straight-line code (a sample off the tile has
+      // weight 0).  Reading every operand of the batch first, with the 16
+      // products back to back, needs 120 VGPRs (4 waves per SIMD) and was
+      // slower, 8.9 ms against 7.9.
       if (hit)
 #pragma unroll
       for (int kt = 0; kt < SG_BATCH / 2; kt++) {
