@@ -292,23 +292,32 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
         w[a][t] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], s[a] + t) : 0.0f;
         o[a][t] = in ? s[a] + t : s[a];
       }
+    // rows of NT taps, software-pipelined: row r + 1's loads are issued
+    // before row r's products (two register buffers), and the scheduling
+    // barriers keep the compiler from trading them back for occupancy
+    CoilVec<NC> vb[2][NT];
+    auto load_row = [&](int r, CoilVec<NC> *v) {
+      const float2 *row = G + ((int64_t)o[2][r / NT] * g.S[1] + o[1][r % NT]) * g.S[0] * g.CC;
 #pragma unroll
-    for (int tz = 0; tz < NT; tz++)
+      for (int tx = 0; tx < NT; tx++) v[tx] = CoilVec<NC>::load(row + (int64_t)o[0][tx] * g.CC);
+    };
+    load_row(0, vb[0]);
 #pragma unroll
-      for (int ty = 0; ty < NT; ty++) {
-        const float wzy = w[2][tz] * w[1][ty];
-        const float2 *row = G + ((int64_t)o[2][tz] * g.S[1] + o[1][ty]) * g.S[0] * g.CC;
+    for (int r = 0; r < NT * NT; r++) {
+      if (r + 1 < NT * NT) load_row(r + 1, vb[(r + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      const float wzy = w[2][r / NT] * w[1][r % NT];
 #pragma unroll
-        for (int tx = 0; tx < NT; tx++) {
-          const float ww = wzy * w[0][tx];
-          const CoilVec<NC> v = CoilVec<NC>::load(row + (int64_t)o[0][tx] * g.CC);
+      for (int tx = 0; tx < NT; tx++) {
+        const float ww = wzy * w[0][tx];
 #pragma unroll
-          for (int q = 0; q < NC; q++) {
-            ar[q] += ww * v.v[q].x;
-            ai[q] += ww * v.v[q].y;
-          }
+        for (int q = 0; q < NC; q++) {
+          ar[q] += ww * vb[r & 1][tx].v[q].x;
+          ai[q] += ww * vb[r & 1][tx].v[q].y;
         }
       }
+      __builtin_amdgcn_sched_barrier(0);
+    }
   } else {
     for (int lz = s[2]; lz < s[2] + n[2]; lz++) {
       const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
@@ -1131,8 +1140,8 @@ static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, con
   const Geom &g = op->g;
   // coils per lane: 4 by default.  A wave then serves 16 samples (4 lanes
   // each) instead of 8, whose taps overlap more, so fewer of its loads miss
-  // the L1 (16-coil blocks at 256^3: 7.8 ms with 2 coils per lane, 6.8 with
-  // 4, 13.5 with 8, 9.9 with 1)
+  // the L1 (16-coil blocks at 256^3, pipelined rows: 5.5 ms with 4 coils per
+  // lane, 5.7 with 2, 7.5 with 8)
   int nc = op->gather_coils > 0 ? op->gather_coils : 4;
   if (nc > g.CC) nc = g.CC;
   const int64_t n = list ? nlist : fr.Mv;
