@@ -801,6 +801,7 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
   __shared__ int s_rs[4][SG_MAXROWS];        // first sample of each bin row
   __shared__ int s_rp[4][SG_MAXROWS + 1];    // visits before each row
   __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
+  __shared__ float s_yz[4][MM ? SG_BATCH * 16 : 1];  // MM: [sample][z][y] products wz * wy
   __shared__ float2 s_d[4][SG_BATCH * CCT];  // [sample][coil] data
   extern __shared__ float tab_s[];           // kernel tables (g.ntab floats)
   // tables in LDS: a lookup waits for LDS, not for the L1/L2 round trip
@@ -885,19 +886,30 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
     const float4 k = ok ? kraw : make_float4(0.f, 0.f, 0.f, 0.f);
     const float dk[3] = {k.x, k.y, k.z};
     unsigned long long hit = ~0ull;
+    float wa[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       int st = 0, en = -1;
       const int l = t0[a] + q;
       bool in = ok && axis_range(g, a, dk[a], st, en);
       in = in && l >= st && l <= en;
-      s_w[wave][sl * 12 + a * 4 + q] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
+      wa[a] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
+      s_w[wave][sl * 12 + a * 4 + q] = wa[a];
       unsigned long long m = __ballot(in);
       m |= m >> 1;
       m |= m >> 2;
       hit &= m;  // bit 4s: sample s has a tap in the tile on every axis so far
     }
     hit &= 0x1111111111111111ull;
+    if constexpr (MM) {
+      // lane q of the sample's quad: products wz[q] * wy[0..3], wy by DPP
+      // broadcast inside the quad (quad_perm [y, y, y, y])
+#define MRN_YZ(y)                                                                                      \
+  s_yz[wave][sl * 16 + q * 4 + y] =                                                                   \
+      wa[2] * __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(wa[1]), 0x55 * y, 0xF, 0xF, false));
+      MRN_YZ(0) MRN_YZ(1) MRN_YZ(2) MRN_YZ(3)
+#undef MRN_YZ
+    }
     if (q * CPL < CCT && (ok || MM)) {  // MM reads every sample of the batch: zeros past the end
 #pragma unroll
       for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = ok ? dv[c] : make_float2(0.f, 0.f);
@@ -916,10 +928,12 @@ __global__ __launch_bounds__(256, 5) void k_scatter_gd(
 #pragma unroll
       for (int kt = 0; kt < SG_BATCH / 2; kt++) {
         const int smp = 2 * kt + (lane >> 5);
-        const float *wp = &s_w[wave][smp * 12];
+        // cell c = 16 z + 4 y + x: (wz wy) wx, with wz wy from s_yz[16 z + 4 y]
+        // so that a step's three LDS reads are independent
+        const float wx = s_w[wave][smp * 12 + (c0 & 3)];
         const float b = dflat[smp * 2 * CCT + c0];
-        const float a0 = (wp[8 + (c0 >> 4)] * wp[4 + ((c0 >> 2) & 3)]) * wp[c0 & 3];
-        const float a1 = (wp[8 + (c1 >> 4)] * wp[4 + ((c1 >> 2) & 3)]) * wp[c1 & 3];
+        const float a0 = s_yz[wave][smp * 16 + (c0 >> 2)] * wx;
+        const float a1 = s_yz[wave][smp * 16 + (c1 >> 2)] * wx;
         acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b, acc1, 0, 0, 0);
       }
