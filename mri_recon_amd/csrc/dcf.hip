@@ -26,6 +26,7 @@
 // cell differs from run to run; the gather sums in the reference's tap order.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -51,6 +52,9 @@ namespace {
 
 constexpr int DCF_MOD = 600;          // table points per cell (DCFgridFFT.cpp:101-103)
 constexpr int DCF_MAX_TABLE = 12288;  // table floats held in LDS (48 KB)
+// sort brick edge in cells (5.12 M radial samples, 553^3: 4^3 bricks 221 ms
+// for 20 iterations, 2^3 249 ms, 8^3 with a 4096-cell box 262 ms)
+constexpr int DCF_BRICK = 4;
 
 struct DcfGeom {
   int S[3];
@@ -144,34 +148,102 @@ __global__ void k_dcf_fill(float *v, int64_t n, float value) {
     v[i] = value;
 }
 
-// DCFgridFFT::forward (DCFgridFFT.cpp:171-289): one thread per sample.
-__global__ __launch_bounds__(256) void k_dcf_scatter(float *__restrict__ X, const float *__restrict__ kw,
-                                                     const float *__restrict__ kx, const float *__restrict__ ky,
-                                                     const float *__restrict__ kz, int64_t M, DcfGeom g,
-                                                     const float *__restrict__ tab) {
-  extern __shared__ float s_tab[];
+// DCFgridFFT::forward (DCFgridFFT.cpp:171-289), one thread per sample: kernel
+// weights with the reference's float/double promotions, grid += w * kw.
+// The samples run in 4^3-brick order, so
+// in the dense centre of k-space a workgroup's 256 samples reach a box of a
+// few thousand cells.  When the box (the bounding box of the taps of the
+// workgroup's samples) holds at most DCF_BOX cells, the taps are summed
+// there with FP64 LDS atomics (ds_add_f64; FP32 LDS atomics are ~20x
+// slower on gfx950) and each touched cell gets one float atomic in the grid,
+// instead of one per tap.  Larger boxes (sparse regions) take the direct
+// path.  The grid sums were already unordered (float atomics); a box only
+// sums its part in double before rounding once.
+constexpr int DCF_BOX = 8192;  // FP64 LDS box cells (64 KB): 221 ms vs 241 with 4096
+__global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, const float *__restrict__ kw,
+                                                         const float *__restrict__ kx, const float *__restrict__ ky,
+                                                         const float *__restrict__ kz, int64_t M, DcfGeom g,
+                                                         const float *__restrict__ tab) {
+  extern __shared__ double dsh[];
+  double *s_acc = dsh;
+  float *s_tab = reinterpret_cast<float *>(dsh + DCF_BOX);
+  __shared__ int s_red[6][256 / 64];
   load_table(s_tab, tab, g.len);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  float k[3];
-  const float kr = kr_of(g, kx[i], ky[i], kz[i], k);
-  const float scale = scale_fwd(g, kr);
-  const float temp = kw[i];
-  if (temp == 0.0f || !(scale > 0.0f)) return;  // a scale <= 0 (acc < 1, large |k|) has no taps
-  float dk[3];
-  int s[3], e[3];
-  if (!axis(g, 0, k[0], scale, dk[0], s[0], e[0])) return;
-  if (!axis(g, 1, k[1], scale, dk[1], s[1], e[1])) return;
-  if (!axis(g, 2, k[2], scale, dk[2], s[2], e[2])) return;
-  for (int lz = s[2]; lz <= e[2]; lz++) {
-    const float wtz = (float)lerp(s_tab, dk[2], lz, scale);
-    for (int ly = s[1]; ly <= e[1]; ly++) {
-      const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk[1], ly, scale));
-      float *row = X + ((int64_t)lz * g.S[1] + ly) * g.S[0];
-      for (int lx = s[0]; lx <= e[0]; lx++) {
-        const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
-        atomicAdd(row + lx, __fmul_rn(wtx, temp));
+  bool live = false;
+  float temp = 0.0f, scale = 0.0f, dk[3] = {0.f, 0.f, 0.f};
+  int s[3] = {0, 0, 0}, e[3] = {-1, -1, -1};
+  if (i < M) {
+    float k[3];
+    scale = scale_fwd(g, kr_of(g, kx[i], ky[i], kz[i], k));
+    temp = kw[i];
+    live = temp != 0.0f && scale > 0.0f;  // a scale <= 0 (acc < 1, large |k|) has no taps
+    live = live && axis(g, 0, k[0], scale, dk[0], s[0], e[0]);
+    live = live && axis(g, 1, k[1], scale, dk[1], s[1], e[1]);
+    live = live && axis(g, 2, k[2], scale, dk[2], s[2], e[2]);
+  }
+  // bounding box of the workgroup's taps
+  int r6[6];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    r6[a] = live ? s[a] : INT_MAX;
+    r6[3 + a] = live ? e[a] : INT_MIN;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      r6[a] = min(r6[a], __shfl_xor(r6[a], o));
+      r6[3 + a] = max(r6[3 + a], __shfl_xor(r6[3 + a], o));
+    }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int a = 0; a < 6; a++) s_red[a][threadIdx.x >> 6] = r6[a];
+  __syncthreads();
+  int lo[3], ext[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int mn = INT_MAX, mx = INT_MIN;
+    for (int w = 0; w < 256 / 64; w++) {
+      mn = min(mn, s_red[a][w]);
+      mx = max(mx, s_red[3 + a][w]);
+    }
+    lo[a] = mn;
+    ext[a] = mx >= mn ? mx - mn + 1 : 0;
+  }
+  const int64_t vol = (int64_t)ext[0] * ext[1] * ext[2];  // 0: no live sample
+  if (vol == 0) return;                                     // (uniform)
+  const bool boxed = vol <= DCF_BOX;
+  if (boxed) {
+    for (int q = threadIdx.x; q < vol; q += blockDim.x) s_acc[q] = 0.0;
+    __syncthreads();
+  }
+  if (live) {
+    for (int lz = s[2]; lz <= e[2]; lz++) {
+      const float wtz = (float)lerp(s_tab, dk[2], lz, scale);
+      for (int ly = s[1]; ly <= e[1]; ly++) {
+        const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk[1], ly, scale));
+        float *row = X + ((int64_t)lz * g.S[1] + ly) * g.S[0];
+        double *brow = s_acc + ((lz - lo[2]) * ext[1] + (ly - lo[1])) * ext[0] - lo[0];
+        for (int lx = s[0]; lx <= e[0]; lx++) {
+          const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
+          const float v = __fmul_rn(wtx, temp);
+          if (boxed)
+            atomicAdd(brow + lx, (double)v);
+          else
+            atomicAdd(row + lx, v);
+        }
       }
+    }
+  }
+  if (boxed) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < vol; q += blockDim.x) {
+      const double v = s_acc[q];
+      if (v == 0.0) continue;
+      const int x = q % ext[0], yz = q / ext[0];
+      const int y = yz % ext[1], z = yz / ext[1];
+      atomicAdd(X + ((int64_t)(lo[2] + z) * g.S[1] + (lo[1] + y)) * g.S[0] + lo[0] + x, (float)v);
     }
   }
 }
@@ -241,7 +313,7 @@ __global__ void k_dcf_finish(float *__restrict__ kw, const float *__restrict__ k
   kw[i] = fabsf((float)__dmul_rn((double)kw[i], __dmul_rn(__dmul_rn(s, s), s)));
 }
 
-// sort key: the 8^3-cell brick of the sample's nearest grid cell
+// sort key: the DCF_BRICK^3-cell brick of the sample's nearest grid cell
 __global__ void k_dcf_keys(const float *__restrict__ kx, const float *__restrict__ ky,
                            const float *__restrict__ kz, int64_t M, DcfGeom g, uint32_t *__restrict__ keys,
                            uint32_t *__restrict__ idx) {
@@ -254,7 +326,7 @@ __global__ void k_dcf_keys(const float *__restrict__ kx, const float *__restrict
     const float dk = __fadd_rn(__fmul_rn(k[a], g.gx), g.c[a]);
     int c = (int)floorf(dk);
     c = c < 0 ? 0 : (c > g.S[a] - 1 ? g.S[a] - 1 : c);
-    key = key * (uint32_t)((g.S[a] + 7) >> 3) + (uint32_t)(c >> 3);
+    key = key * (uint32_t)((g.S[a] + DCF_BRICK - 1) / DCF_BRICK) + (uint32_t)(c / DCF_BRICK);
   }
   keys[i] = key;
   idx[i] = (uint32_t)i;
@@ -367,6 +439,11 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
     DCF_HIP(hipMalloc(&dsum, sizeof(double) * (p->iters > 0 ? p->iters : 1)));
     DCF_HIP(hipMemcpyAsync(tab, tab_h, sizeof(float) * len, hipMemcpyHostToDevice, st));
     const size_t lds = sizeof(float) * len;
+    const size_t lds_box = sizeof(double) * DCF_BOX + lds;
+    static const hipError_t box_attr = hipFuncSetAttribute((const void *)k_dcf_scatter_box,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           (int)(sizeof(double) * DCF_BOX + sizeof(float) * DCF_MAX_TABLE));
+    DCF_HIP(box_attr);
     const int fillb = 4096;
     const dim3 gs((unsigned)nblocks(M, 256));
     // brick order
@@ -375,7 +452,8 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
     DCF_HIP(hipMalloc(&srt, sizeof(float) * 4 * M));
     hipLaunchKernelGGL(k_dcf_keys, gs, dim3(256), 0, st, kx, ky, kz, M, g, keys, perm);
     DCF_HIP(hipGetLastError());
-    const uint64_t nkeys = (uint64_t)((g.S[0] + 7) >> 3) * ((g.S[1] + 7) >> 3) * ((g.S[2] + 7) >> 3);
+    const uint64_t nkeys = (uint64_t)((g.S[0] + DCF_BRICK - 1) / DCF_BRICK) * ((g.S[1] + DCF_BRICK - 1) / DCF_BRICK) *
+                           ((g.S[2] + DCF_BRICK - 1) / DCF_BRICK);
     int bits = 1;
     while (bits < 32 && (1ull << bits) < nkeys) bits++;
     DCF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys + M, perm, perm + M, (int)M, 0, bits, st));
@@ -394,7 +472,7 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
     }
     for (int it = 0; it < p->iters; it++) {
       hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, X, nS, 0.0f);
-      hipLaunchKernelGGL(k_dcf_scatter, gs, dim3(256), lds, st, X, sw, sx, sy, sz, M, g, tab);
+      hipLaunchKernelGGL(k_dcf_scatter_box, gs, dim3(256), lds_box, st, X, sw, sx, sy, sz, M, g, tab);
       hipLaunchKernelGGL(k_dcf_gather_update, gs, dim3(256), lds, st, X, sw, sx, sy, sz, M, g, tab,
                          sums ? dsum + it : nullptr);
       DCF_HIP(hipGetLastError());
