@@ -95,7 +95,7 @@ static LinesArgs ypass_args(const mrn_op *op, bool inverse) {
   return y;
 }
 
-static LinesArgs xpass_args(const mrn_op *op, bool inverse) {
+static LinesArgs xpass_args(const mrn_op *op, bool inverse, const FrameData *fr) {
   const Geom &g = op->g;
   const int CC = g.CC;
   LinesArgs x{};
@@ -111,44 +111,54 @@ static LinesArgs xpass_args(const mrn_op *op, bool inverse) {
   x.st0 = inverse ? g.og[0] : 0;
   x.st1 = inverse ? g.og[0] + g.N[0] : g.S[0];
   x.scale = fft_scale(g.S[0]);
+  // the frame's k-space support: the forward pass stores, and the inverse
+  // pass loads, only the x cells that some sample's taps reach
+  x.rng = (fr && fft_x_pruned(op)) ? fr->xrng : nullptr;
+  x.rng_rows = g.S[1];
+  x.rng_store = inverse ? 0 : 1;
+  x.rng_load = inverse ? 1 : 0;
   return x;
 }
 
 // forward stage q of the batched transform: 0 set_image+z, 1 y, 2 x
-static int fwd_stage(mrn_op *op, int q, const float2 *X, hipStream_t st) {
+static int fwd_stage(mrn_op *op, int q, const float2 *X, const FrameData *fr, hipStream_t st) {
   const Geom &g = op->g;
   if (q == 0) return fft_pre_z(op, op->grid, X, op->smaps, prepost(op, 0, 0, g.CC), st);
-  const LinesArgs a = (q == 1) ? ypass_args(op, false) : xpass_args(op, false);
+  const LinesArgs a = (q == 1) ? ypass_args(op, false) : xpass_args(op, false, fr);
   const int S = g.S[q == 1 ? 1 : 0];
   return fft_lines(op, S, -1, op->grid, a, dim3(cdiv(a.nlines, FFT_LINES), g.S[2], op->nch),
                    op->tw_d[q == 1 ? 1 : 0], st);
 }
 
 // inverse stage q: 0 x, 1 y, 2 z+accumulate over the coil blocks in order
-static int adj_stage(mrn_op *op, int q, float2 *X, bool overwrite, hipStream_t st) {
+static int adj_stage(mrn_op *op, int q, float2 *X, bool overwrite, const FrameData *fr, hipStream_t st) {
   const Geom &g = op->g;
   if (q == 2) {
     for (int ch = 0; ch < op->nch; ch++)
       MRN_TRY(fft_post_z(op, op->grid, X, op->smaps, prepost(op, ch, overwrite && ch == 0, g.CC), st));
     return MRN_OK;
   }
-  const LinesArgs a = (q == 1) ? ypass_args(op, true) : xpass_args(op, true);
+  const LinesArgs a = (q == 1) ? ypass_args(op, true) : xpass_args(op, true, fr);
   const int S = g.S[q == 1 ? 1 : 0];
   return fft_lines(op, S, 1, op->grid, a, dim3(cdiv(a.nlines, FFT_LINES), g.S[2], op->nch),
                    op->tw_d[q == 1 ? 1 : 0], st);
 }
 
 // image -> full k-space grid of every coil block
-static int fwd_grid(mrn_op *op, const float2 *X, hipStream_t st) {
-  for (int q = 0; q < 3; q++) MRN_TRY(fwd_stage(op, q, X, st));
+// fr: the frame whose samples read the grid (its k-space support), or null
+static int fwd_grid(mrn_op *op, const float2 *X, const FrameData *fr, hipStream_t st) {
+  for (int q = 0; q < 3; q++) MRN_TRY(fwd_stage(op, q, X, fr, st));
   return MRN_OK;
 }
 
-int fwd_grid_pub(mrn_op *op, const float2 *X, hipStream_t st) { return fwd_grid(op, X, st); }
+int fwd_grid_pub(mrn_op *op, const float2 *X, const FrameData *fr, hipStream_t st) {
+  return fwd_grid(op, X, fr, st);
+}
 
 // full grids of every coil block -> image (X = or += sum over coils)
-static int adj_grid(mrn_op *op, float2 *X, bool overwrite, hipStream_t st) {
-  for (int q = 0; q < 3; q++) MRN_TRY(adj_stage(op, q, X, overwrite, st));
+// fr: the frame whose samples were scattered into the grid, or null
+static int adj_grid(mrn_op *op, float2 *X, bool overwrite, const FrameData *fr, hipStream_t st) {
+  for (int q = 0; q < 3; q++) MRN_TRY(adj_stage(op, q, X, overwrite, fr, st));
   return MRN_OK;
 }
 
@@ -249,10 +259,10 @@ static int check_frame(mrn_op *op, int f, bool need_kdata) {
 static int normal_frame(mrn_op *op, int f, const float2 *X, float2 *out, hipStream_t st) {
   FrameData &fr = op->frames[f];
   if (batched(op)) {
-    MRN_TRY(fwd_grid(op, X, st));
+    MRN_TRY(fwd_grid(op, X, &fr, st));
     MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, true, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-    return adj_grid(op, out, true, st);
+    return adj_grid(op, out, true, &fr, st);
   }
   MRN_TRY(op_zero(op, out, op->nvox, st));
   for (int c = 0; c < op->C; c++) {
@@ -267,10 +277,10 @@ static int normal_frame(mrn_op *op, int f, const float2 *X, float2 *out, hipStre
 static int grad_frame(mrn_op *op, int f, const float2 *X, float2 *R, hipStream_t st) {
   FrameData &fr = op->frames[f];
   if (batched(op)) {
-    MRN_TRY(fwd_grid(op, X, st));
+    MRN_TRY(fwd_grid(op, X, &fr, st));
     MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, fr.kdata, true, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-    return adj_grid(op, R, true, st);
+    return adj_grid(op, R, true, &fr, st);
   }
   MRN_TRY(op_zero(op, R, op->nvox, st));
   for (int c = 0; c < op->C; c++) {
@@ -288,7 +298,7 @@ static int adjoint_data_frame(mrn_op *op, int f, float2 *R, hipStream_t st) {
   if (batched(op)) {
     MRN_TRY(launch_weight(op, op->kwork, fr.kdata, fr, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-    return adj_grid(op, R, true, st);
+    return adj_grid(op, R, true, &fr, st);
   }
   MRN_TRY(op_zero(op, R, op->nvox, st));
   for (int c = 0; c < op->C; c++)
@@ -459,7 +469,7 @@ int mrn_image_to_kspace(mrn_op *op, int f, const void *X, void *data, const void
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
   if (batched(op)) {
-    MRN_TRY(fwd_grid(op, (const float2 *)X, st));
+    MRN_TRY(fwd_grid(op, (const float2 *)X, &fr, st));
     MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, false, st));
     return op_permute_out(op, (float2 *)data, op->kwork, fr, (const float2 *)sub, st);
   }
@@ -480,7 +490,7 @@ int mrn_kspace_to_image(mrn_op *op, int f, const void *data, void *X, int accumu
     MRN_TRY(op_permute_in(op, op->kwork, (const float2 *)data, fr, st));
     MRN_TRY(launch_weight(op, op->kwork, op->kwork, fr, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-    return adj_grid(op, (float2 *)X, !accumulate, st);
+    return adj_grid(op, (float2 *)X, !accumulate, &fr, st);
   }
   if (!accumulate) MRN_TRY(op_zero(op, (float2 *)X, op->nvox, st));
   for (int c = 0; c < op->C; c++)
@@ -743,7 +753,7 @@ int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[MRN_NS
     int e = 0;
     MRN_HIP(op, hipEventRecord(ev[e++], st));
     for (int q = 0; q < 3; q++) {
-      MRN_TRY(fwd_stage(op, q, X, st));
+      MRN_TRY(fwd_stage(op, q, X, &fr, st));
       MRN_HIP(op, hipEventRecord(ev[e++], st));
     }
     MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, true, st));
@@ -753,7 +763,7 @@ int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[MRN_NS
     MRN_TRY(launch_scatter_tiles(op, op->grid, op->kwork, fr, st));
     MRN_HIP(op, hipEventRecord(ev[e++], st));
     for (int q = 0; q < 3; q++) {
-      MRN_TRY(adj_stage(op, q, op->vX0, true, st));
+      MRN_TRY(adj_stage(op, q, op->vX0, true, &fr, st));
       MRN_HIP(op, hipEventRecord(ev[e++], st));
     }
     MRN_HIP(op, hipEventSynchronize(ev[MRN_NSTAGES]));

@@ -568,12 +568,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
   const bool ok = gl < a.nlines;
   const int r = ok ? gl / a.run : 0, cc = ok ? gl - r * a.run : 0;
   float2 *ln = grid + base + (int64_t)r * a.rstride + cc;
+  int ld0 = a.ld0, ld1 = a.ld1, st0 = a.st0, st1 = a.st1;
+  if (a.rng && ok) {  // k-space support of this row (x passes)
+    const int2 rr = a.rng[(int64_t)blockIdx.y * a.rng_rows + r];
+    if (a.rng_load) {
+      ld0 = max(ld0, rr.x);
+      ld1 = min(ld1, rr.y + 1);
+    }
+    if (a.rng_store) {
+      st0 = max(st0, rr.x);
+      st1 = min(st1, rr.y + 1);
+    }
+  }
   float2 v[P];
 #pragma unroll
   for (int j = 0; j < P; j++) {
     const int p = t + 16 * j;
     v[j] = make_float2(0.f, 0.f);
-    if (ok && p >= a.ld0 && p < a.ld1) v[j] = ln[(int64_t)p * a.pstride];
+    if (ok && p >= ld0 && p < ld1) v[j] = ln[(int64_t)p * a.pstride];
   }
   load_twiddles<S>(tws, tw);
   __syncthreads();
@@ -586,7 +598,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
 #pragma unroll
       for (int k2 = 0; k2 < 16; k2++) {
         const int p = k1 + P * k2;
-        if (p >= a.st0 && p < a.st1) ln[(int64_t)p * a.pstride] = scl(w[k2], a.scale);
+        if (p >= st0 && p < st1) ln[(int64_t)p * a.pstride] = scl(w[k2], a.scale);
       }
     }
   }
@@ -789,6 +801,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_dft_lines(float2 *__restrict__ 
 #define MRN_FFT_SIZES(X) \
   X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(144) X(160) \
   X(192) X(240) X(256) X(288) X(320) X(384) X(400) X(432) X(480) X(512)
+
+// The batched x passes run the four-step kernel with the frame's k-space
+// support (forward: stores, inverse: loads limited to the support).
+bool fft_x_pruned(const mrn_op *op) {
+  return op->p.form != MRN_FORM_PLAIN && op->g.CC >= FFT_LINES && !op->fft_stockham && fft4_ok(op->g.S[0]) &&
+         op->g.fft_in[0];
+}
 
 bool fft_size_supported(int S) {
   switch (S) {

@@ -15,7 +15,7 @@
 #include "mrn_internal.hpp"
 
 namespace mrn {
-int fwd_grid_pub(mrn_op *op, const float2 *X, hipStream_t st);
+int fwd_grid_pub(mrn_op *op, const float2 *X, const FrameData *fr, hipStream_t st);
 int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, bool, hipStream_t);
 int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
@@ -129,7 +129,7 @@ int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float 
   FrameData &fr = op->frames[f];
   MRN_HIP(op, hipMemsetAsync(grad_k, 0, sizeof(float) * 3 * fr.M, st));
   if (fr.Mv == 0) return MRN_OK;
-  MRN_TRY(fwd_grid_pub(op, X, st));
+  MRN_TRY(fwd_grid_pub(op, X, &fr, st));
   MRN_TRY(op_permute_in(op, op->kwork, gup, fr, st));
   const Geom &g = op->g;
   dim3 g3((unsigned)((fr.Mv * g.CC + 255) / 256), op->nch);

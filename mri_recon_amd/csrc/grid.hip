@@ -18,6 +18,8 @@
 // flushed with global atomics.
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
+
 #include "mrn_internal.hpp"
 
 namespace mrn {
@@ -106,8 +108,10 @@ __host__ __device__ __forceinline__ void tile_bins(const Tiles &t, int a, int ti
   if (b1 > t.nb[a] - 1) b1 = t.nb[a] - 1;
 }
 
+// xrng (null: every tile): a tile without samples gets no work item when no
+// row's k-space support reaches its cells -- the inverse x pass never reads them
 __global__ void k_tile_nsub(const int32_t *__restrict__ bin_start, Tiles t, int32_t *nsub,
-                            int32_t *visits) {
+                            int32_t *visits, const int2 *__restrict__ xrng, int S1, int S2) {
   int tile = blockIdx.x * blockDim.x + threadIdx.x;
   if (tile >= t.ntiles) return;
   const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
@@ -122,7 +126,13 @@ __global__ void k_tile_nsub(const int32_t *__restrict__ bin_start, Tiles t, int3
       v += bin_start[b + bx1 + 1] - bin_start[b + bx0];  // bins of a row are contiguous
     }
   visits[tile] = v;
-  nsub[tile] = v > t.maxv ? (v + t.maxv - 1) / t.maxv : 1;
+  bool needed = v > 0 || !xrng;
+  for (int z = tz * t.T; !needed && z < min(tz * t.T + t.T, S2); z++)
+    for (int y = ty * t.T; !needed && y < min(ty * t.T + t.T, S1); y++) {
+      const int2 rr = xrng[(int64_t)z * S1 + y];
+      needed = rr.x <= tx * t.T + t.T - 1 && rr.y >= tx * t.T;
+    }
+  nsub[tile] = v > t.maxv ? (v + t.maxv - 1) / t.maxv : (needed ? 1 : 0);
 }
 
 __global__ void k_fill_subs(const int32_t *__restrict__ nsub, const int32_t *__restrict__ off,
@@ -1305,6 +1315,51 @@ int recompute_taper(mrn_op *op, mrn::FrameData &fr, hipStream_t st) {
   return MRN_OK;
 }
 
+// k-space support of a frame, per grid row (z, y): the x cells that any
+// sample's taps reach.  A sample binned at b on an axis (b = first tap / B)
+// reaches cells [b B, b B + B - 1 + W], W = floor(2 dwin) taps beyond the
+// first on a gridded axis, 0 otherwise.  Pass 1: one wavefront per bin row
+// (bz, by) finds its first and last nonempty bin in x.
+__global__ void k_binrow_x(const int32_t *__restrict__ bin_start, Tiles t, int2 *brow) {
+  const int row = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= t.nb[1] * t.nb[2]) return;  // whole wavefronts
+  const int32_t *bs = bin_start + (int64_t)row * t.nb[0];
+  int first = INT_MAX, last = -1;
+  for (int b0 = 0; b0 < t.nb[0]; b0 += 64) {
+    const int b = b0 + lane;
+    const bool ne = b < t.nb[0] && bs[b + 1] > bs[b];
+    const unsigned long long m = __ballot(ne);
+    if (m) {
+      if (first == INT_MAX) first = b0 + __builtin_ctzll(m);
+      last = b0 + 63 - __builtin_clzll(m);
+    }
+  }
+  if (lane == 0) brow[row] = make_int2(first, last);
+}
+
+// Pass 2: row (z, y) of cells takes the x extent of every bin row whose
+// reach covers it; {1, 0} (empty) when none does.
+__global__ void k_row_xrange(const int2 *__restrict__ brow, Tiles t, int S0, int S1, int S2, int W0, int W1,
+                             int W2, int2 *xrng) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (int64_t)S1 * S2) return;
+  const int y = (int)(r % S1), z = (int)(r / S1);
+  const int B = t.B;
+  // bins b with b B <= c <= b B + B - 1 + W: b in [ceil((c - B + 1 - W) / B), c / B]
+  const int z0 = max(z - B + 1 - W2, 0), y0 = max(y - B + 1 - W1, 0);
+  const int bz0 = (z0 + B - 1) / B, bz1 = min(z / B, t.nb[2] - 1);
+  const int by0 = (y0 + B - 1) / B, by1 = min(y / B, t.nb[1] - 1);
+  int first = INT_MAX, last = -1;
+  for (int bz = bz0; bz <= bz1; bz++)
+    for (int by = by0; by <= by1; by++) {
+      const int2 v = brow[bz * t.nb[1] + by];
+      first = min(first, v.x);
+      last = max(last, v.y);
+    }
+  xrng[r] = last < 0 ? make_int2(1, 0) : make_int2(first * B, min(last * B + B - 1 + W0, S0 - 1));
+}
+
 // Sort one frame's samples into tiles and build the scatter work list.
 int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const float *ky,
                const float *kz, const float *kw, hipStream_t st) {
@@ -1314,6 +1369,7 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   uint32_t *keys = nullptr, *keys2 = nullptr;
   int32_t *vals = nullptr, *counts = nullptr, *nsub = nullptr, *visits = nullptr, *off = nullptr,
           *hflag = nullptr, *hoff = nullptr;
+  int2 *brow = nullptr;
   void *tmp = nullptr;
   size_t tmp_bytes = 0, b2 = 0;
   int rc = MRN_OK;
@@ -1327,6 +1383,7 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
     hipFree(off);
     hipFree(hflag);
     hipFree(hoff);
+    hipFree(brow);
     hipFree(tmp);
   };
 #define MRN_BH(call)                                                                         \
@@ -1347,8 +1404,10 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipFree(fr.heavy);
   hipFree(fr.gsubs);
   hipFree(fr.sparse);
+  hipFree(fr.xrng);
   fr.gsubs = nullptr;
   fr.sparse = nullptr;
+  fr.xrng = nullptr;
   fr.ngsub = 0;
   fr.nsparse = 0;
   fr.coords = nullptr;
@@ -1404,6 +1463,19 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
     hipLaunchKernelGGL(k_traj_finish, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, kx, ky, kz, kw,
                        fr.perm, fr.Mv, g, fr.coords, fr.taper, fr.inv);
   }
+  // k-space support per grid row, for the x passes
+  {
+    const int nbr = t.nb[1] * t.nb[2];
+    const int64_t nrow = (int64_t)g.S[1] * g.S[2];
+    int W[3];
+    for (int a = 0; a < 3; a++) W[a] = g.grid_in[a] ? (int)floorf(2.0f * g.dwin[a]) : 0;
+    MRN_BH(hipMalloc(&brow, sizeof(int2) * nbr));
+    MRN_BH(hipMalloc(&fr.xrng, sizeof(int2) * nrow));
+    hipLaunchKernelGGL(k_binrow_x, dim3(nblk((int64_t)nbr * 64, 256)), dim3(256), 0, st, fr.bin_start, t, brow);
+    hipLaunchKernelGGL(k_row_xrange, dim3(nblk(nrow, 256)), dim3(256), 0, st, brow, t, g.S[0], g.S[1], g.S[2],
+                       W[0], W[1], W[2], fr.xrng);
+    MRN_BH(hipGetLastError());
+  }
   MRN_BH(hipMalloc(&nsub, sizeof(int32_t) * (nt + 1)));
   MRN_BH(hipMalloc(&visits, sizeof(int32_t) * (nt + 1)));
   MRN_BH(hipMalloc(&off, sizeof(int32_t) * (nt + 1)));
@@ -1411,7 +1483,8 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   MRN_BH(hipMalloc(&hoff, sizeof(int32_t) * (nt + 1)));
   MRN_BH(hipMemsetAsync(nsub, 0, sizeof(int32_t) * (nt + 1), st));
   MRN_BH(hipMemsetAsync(hflag, 0, sizeof(int32_t) * (nt + 1), st));
-  hipLaunchKernelGGL(k_tile_nsub, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, nsub, visits);
+  hipLaunchKernelGGL(k_tile_nsub, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, nsub, visits,
+                     fft_x_pruned(op) ? fr.xrng : nullptr, g.S[1], g.S[2]);
   MRN_BH(hipGetLastError());
   MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nsub, off, nt + 1, st));
   int32_t nsub_total = 0;

@@ -355,6 +355,11 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   if (batched) {
     if (amalloc((void **)&op->grid, sizeof(float2) * ncell * g.CC * op->nch) != hipSuccess)
       return cleanup_fail(MRN_ENOMEM);
+    // zeroed once: the forward x pass leaves the cells outside a frame's
+    // k-space support as they are, and readers of whole tile regions weight
+    // them by zero, so they must hold finite values
+    if (hipMemset(op->grid, 0, sizeof(float2) * ncell * g.CC * op->nch) != hipSuccess)
+      return cleanup_fail(MRN_EHIP);
     if (amalloc((void **)&op->smaps, sizeof(float2) * op->nvox * g.CC * op->nch) != hipSuccess)
       return cleanup_fail(MRN_ENOMEM);
   } else {
@@ -384,6 +389,8 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.subs);
     hipFree(fr.heavy);
     hipFree(fr.gsubs);
+    hipFree(fr.sparse);
+    hipFree(fr.xrng);
     hipFree(fr.kdata);
     hipFree(fr.kraw);
     hipFree(fr.traw);
