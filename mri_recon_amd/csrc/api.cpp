@@ -75,7 +75,7 @@ static PrePostArgs prepost(const mrn_op *op, int ch, int overwrite, int CC) {
 // Lines of the y and x passes of every coil block.  Forward passes load the
 // crop range (the rest is zero) and store everything; inverse passes load
 // everything and store the crop range (the rest is never read).
-static LinesArgs ypass_args(const mrn_op *op, bool inverse) {
+static LinesArgs ypass_args(const mrn_op *op, bool inverse, const FrameData *fr) {
   const Geom &g = op->g;
   const int CC = g.CC;
   LinesArgs y{};
@@ -92,6 +92,13 @@ static LinesArgs ypass_args(const mrn_op *op, bool inverse) {
   y.st1 = inverse ? g.og[1] + g.N[1] : g.S[1];
   y.scale = fft_scale(g.S[1]);
   y.xrun = op->xcd_run_fft;  // neighbouring 128-byte runs of a row per XCD
+  // the frame's rows of support in plane z: the forward pass stores, and the
+  // inverse pass loads, only those ky
+  y.rng = fr ? fr->yrng : nullptr;
+  y.rng_zs = 1;
+  y.rng_rs = 0;
+  y.rng_store = inverse ? 0 : 1;
+  y.rng_load = inverse ? 1 : 0;
   return y;
 }
 
@@ -114,7 +121,8 @@ static LinesArgs xpass_args(const mrn_op *op, bool inverse, const FrameData *fr)
   // the frame's k-space support: the forward pass stores, and the inverse
   // pass loads, only the x cells that some sample's taps reach
   x.rng = (fr && fft_x_pruned(op)) ? fr->xrng : nullptr;
-  x.rng_rows = g.S[1];
+  x.rng_zs = g.S[1];
+  x.rng_rs = 1;
   x.rng_store = inverse ? 0 : 1;
   x.rng_load = inverse ? 1 : 0;
   return x;
@@ -124,7 +132,7 @@ static LinesArgs xpass_args(const mrn_op *op, bool inverse, const FrameData *fr)
 static int fwd_stage(mrn_op *op, int q, const float2 *X, const FrameData *fr, hipStream_t st) {
   const Geom &g = op->g;
   if (q == 0) return fft_pre_z(op, op->grid, X, op->smaps, prepost(op, 0, 0, g.CC), st);
-  const LinesArgs a = (q == 1) ? ypass_args(op, false) : xpass_args(op, false, fr);
+  const LinesArgs a = (q == 1) ? ypass_args(op, false, fr) : xpass_args(op, false, fr);
   const int S = g.S[q == 1 ? 1 : 0];
   return fft_lines(op, S, -1, op->grid, a, dim3(cdiv(a.nlines, FFT_LINES), g.S[2], op->nch),
                    op->tw_d[q == 1 ? 1 : 0], st);
@@ -138,7 +146,7 @@ static int adj_stage(mrn_op *op, int q, float2 *X, bool overwrite, const FrameDa
       MRN_TRY(fft_post_z(op, op->grid, X, op->smaps, prepost(op, ch, overwrite && ch == 0, g.CC), st));
     return MRN_OK;
   }
-  const LinesArgs a = (q == 1) ? ypass_args(op, true) : xpass_args(op, true, fr);
+  const LinesArgs a = (q == 1) ? ypass_args(op, true, fr) : xpass_args(op, true, fr);
   const int S = g.S[q == 1 ? 1 : 0];
   return fft_lines(op, S, 1, op->grid, a, dim3(cdiv(a.nlines, FFT_LINES), g.S[2], op->nch),
                    op->tw_d[q == 1 ? 1 : 0], st);

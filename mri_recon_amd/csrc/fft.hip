@@ -13,6 +13,8 @@
 // zero pad, gridFFT.cpp:972-986) and the last inverse pass with accumulate
 // (deapodisation x conj(coil map), summed over the coils of a block,
 // gridFFT.cpp:926-941).
+#include <climits>
+
 #include "mrn_internal.hpp"
 
 namespace mrn {
@@ -569,15 +571,17 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
   const int r = ok ? gl / a.run : 0, cc = ok ? gl - r * a.run : 0;
   float2 *ln = grid + base + (int64_t)r * a.rstride + cc;
   int ld0 = a.ld0, ld1 = a.ld1, st0 = a.st0, st1 = a.st1;
-  if (a.rng && ok) {  // k-space support of this row (x passes)
-    const int2 rr = a.rng[(int64_t)blockIdx.y * a.rng_rows + r];
+  if (a.rng && ok) {  // k-space support of this line's row
+    const int2 rr = a.rng[(int64_t)blockIdx.y * a.rng_zs + (int64_t)r * a.rng_rs];
     if (a.rng_load) {
       ld0 = max(ld0, rr.x);
       ld1 = min(ld1, rr.y + 1);
+      if (rr.x == INT_MAX) st1 = st0;  // no later pass reads this row
     }
     if (a.rng_store) {
       st0 = max(st0, rr.x);
       st1 = min(st1, rr.y + 1);
+      if (rr.x > rr.y) ld1 = ld0;  // nothing to store
     }
   }
   float2 v[P];
@@ -808,6 +812,9 @@ bool fft_x_pruned(const mrn_op *op) {
   return op->p.form != MRN_FORM_PLAIN && op->g.CC >= FFT_LINES && !op->fft_stockham && fft4_ok(op->g.S[0]) &&
          op->g.fft_in[0];
 }
+
+// ... and the y passes with the frame's rows of support per z plane
+bool fft_xy_pruned(const mrn_op *op) { return fft_x_pruned(op) && fft4_ok(op->g.S[1]) && op->g.fft_in[1]; }
 
 bool fft_size_supported(int S) {
   switch (S) {

@@ -1360,6 +1360,24 @@ __global__ void k_row_xrange(const int2 *__restrict__ brow, Tiles t, int S0, int
   xrng[r] = last < 0 ? make_int2(1, 0) : make_int2(first * B, min(last * B + B - 1 + W0, S0 - 1));
 }
 
+// Pass 3 (y passes pruned too): plane z's rows with support span [y0, y1];
+// the rows outside it are marked {INT_MAX, INT_MIN} (no y pass reads them,
+// so the inverse x pass need not write them).
+__global__ void k_plane_yrange(int2 *xrng, int S1, int S2, int2 *yrng) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= S2) return;
+  int2 *row = xrng + (int64_t)z * S1;
+  int y0 = S1, y1 = -1;
+  for (int y = 0; y < S1; y++)
+    if (row[y].x <= row[y].y) {
+      y0 = min(y0, y);
+      y1 = y;
+    }
+  yrng[z] = y1 < 0 ? make_int2(1, 0) : make_int2(y0, y1);
+  for (int y = 0; y < S1; y++)
+    if (y < y0 || y > y1) row[y] = make_int2(INT_MAX, INT_MIN);
+}
+
 // Sort one frame's samples into tiles and build the scatter work list.
 int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const float *ky,
                const float *kz, const float *kw, hipStream_t st) {
@@ -1405,9 +1423,11 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipFree(fr.gsubs);
   hipFree(fr.sparse);
   hipFree(fr.xrng);
+  hipFree(fr.yrng);
   fr.gsubs = nullptr;
   fr.sparse = nullptr;
   fr.xrng = nullptr;
+  fr.yrng = nullptr;
   fr.ngsub = 0;
   fr.nsparse = 0;
   fr.coords = nullptr;
@@ -1475,6 +1495,11 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
     hipLaunchKernelGGL(k_row_xrange, dim3(nblk(nrow, 256)), dim3(256), 0, st, brow, t, g.S[0], g.S[1], g.S[2],
                        W[0], W[1], W[2], fr.xrng);
     MRN_BH(hipGetLastError());
+    if (fft_xy_pruned(op)) {
+      MRN_BH(hipMalloc(&fr.yrng, sizeof(int2) * g.S[2]));
+      hipLaunchKernelGGL(k_plane_yrange, dim3(nblk(g.S[2], 64)), dim3(64), 0, st, fr.xrng, g.S[1], g.S[2], fr.yrng);
+      MRN_BH(hipGetLastError());
+    }
   }
   MRN_BH(hipMalloc(&nsub, sizeof(int32_t) * (nt + 1)));
   MRN_BH(hipMalloc(&visits, sizeof(int32_t) * (nt + 1)));

@@ -76,6 +76,8 @@ struct FrameData {
   int32_t *sparse = nullptr;   // [nsparse] samples of tiles below the dense limit (direct gather)
   int64_t nsparse = 0;
   int2 *xrng = nullptr;        // [Sz][Sy] x cells any tap reaches in grid row (z, y); x > y: none
+                               // ({INT_MAX, INT_MIN}: row outside yrng[z], when yrng is set)
+  int2 *yrng = nullptr;        // [Sz] rows y of plane z with support (y passes pruned), else null
   float2 *kdata = nullptr;     // measured k-space, internal layout [nch][Mv][CC]
   float2 *kraw = nullptr;      // measured k-space, caller layout [C][M] (plain form)
   float *traw = nullptr;       // trajectory kx|ky|kz|kw, caller order [4][M]
@@ -252,11 +254,13 @@ struct LinesArgs {
   int ld0, ld1, st0, st1;          // loaded / stored point ranges
   float scale;
   int xrun;                        // workgroup groups per XCD run (xcd_block; 0/1 = hardware order)
-  // x passes of the batched form: k-space support of row (blockIdx.y, line / run),
-  // rng[blockIdx.y * rng_rows + line / run] = {x0, x1}; points outside are not
-  // stored (forward, rng_store) or are zero (inverse, rng_load)
+  // x and y passes of the batched form: k-space support of the line's row,
+  // rng[blockIdx.y * rng_zs + (line / run) * rng_rs] = {p0, p1}; points outside
+  // are not stored (forward, rng_store) or are zero (inverse, rng_load).  An
+  // empty range skips a forward line's loads; {INT_MAX, INT_MIN} also skips an
+  // inverse line's stores (no later pass reads that row).
   const int2 *rng;
-  int rng_rows, rng_store, rng_load;
+  int rng_zs, rng_rs, rng_store, rng_load;
 };
 
 struct PrePostArgs {
@@ -274,6 +278,7 @@ struct PrePostArgs {
 int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 grid3,
               const float2 *tw, hipStream_t st);
 bool fft_x_pruned(const mrn_op *op);
+bool fft_xy_pruned(const mrn_op *op);
 int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
               const PrePostArgs &a, hipStream_t st);
 int fft_post_z(mrn_op *op, const float2 *grid, float2 *X, const float2 *smaps,

@@ -391,6 +391,7 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.gsubs);
     hipFree(fr.sparse);
     hipFree(fr.xrng);
+    hipFree(fr.yrng);
     hipFree(fr.kdata);
     hipFree(fr.kraw);
     hipFree(fr.traw);
