@@ -1243,6 +1243,9 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   const size_t sh = sizeof(float) * g.ntab;  // kernel tables in LDS
   const int rows_axis = (t.T + t.NT - 2) / t.B + 2;  // bins per axis a tile can reach
   if (rows_axis * rows_axis > SG_MAXROWS) return fail(op, MRN_EUNSUPPORTED, "scatter neighbourhood too large");
+  // no work item: a frame without valid samples whose x passes are limited to
+  // its (empty) k-space support -- nothing reads the grid
+  if (fr.nsub == 0) return MRN_OK;
   if (g.CC == 16 && !op->scatter_valu) {
     hipLaunchKernelGGL((k_scatter_gd<16, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
                        fr.subs, fr.nsub, g, t, op->tables_d, cg, ck);

@@ -173,3 +173,42 @@ def test_kspace_support_pruning_across_frames():
             e_ref, e_unpruned = rel(got[0][k], ref[f][k]), rel(got[0][k], got[1][k])
             print("frame %d output %d: vs oracle %.2e, vs unpruned %.2e" % (f, k, e_ref, e_unpruned))
             assert e_ref <= 1e-5 and e_unpruned <= 1e-6
+
+
+def test_kspace_support_pruning_cartesian_kz_and_empty_frame():
+    """Support pruning with one tap in z (stack of stars, grid 80 x 80 x 32,
+    16-coil blocks) against the oracle, and a frame whose samples all have
+    kw = 0 (no valid sample: empty support, every x / y row skipped, no
+    scatter tile) between calls on the first frame: its A is zero, its A^H
+    and E'E are zero, and the first frame's results do not change."""
+    N, C = (48, 48, 32), 16
+    kw_ = dict(overgrid=1.5, dwin=2.5, grid_in=(1, 1, 0))
+    traj, X, smaps, d = problem(N, C, 91, cartesian_z=True)
+    o32 = orc.OracleOp(N, precision="f32", **kw_)
+    o64 = orc.OracleOp(N, precision="f64", **kw_)
+    assert tuple(o32.S) == (80, 80, 32)
+    A32, H32 = o32.image_to_kspace(X, smaps, traj), o32.kspace_to_image(d, smaps, traj)
+    A64 = o64.image_to_kspace(X.astype(np.complex128), smaps.astype(np.complex128), traj)
+    H64 = o64.kspace_to_image(d.astype(np.complex128), smaps.astype(np.complex128), traj)
+    bar_a, bar_h = max(1e-5, 1.5 * rel(A32, A64)), max(1e-5, 1.5 * rel(H32, H64))
+    op = GridFFT(N, ncoils=C, nframes=2, **kw_)
+    op.set_traj(*traj_dev(traj), frame=0)
+    dead = (traj[0], traj[1], traj[2], np.zeros_like(traj[3]))
+    op.set_traj(*traj_dev(dead), frame=1)
+    op.set_smaps(to_dev(smaps))
+    Xd, dd = to_dev(X), to_dev(d)
+    first = None
+    for rep in range(2):
+        a = op.backward(Xd, frame=0).cpu().numpy()
+        h = op.forward(dd, accumulate=False, frame=0).cpu().numpy()
+        n = op.normal(Xd, frame=0).cpu().numpy()
+        print("cartesian kz pruned: A %.2e AH %.2e" % (rel(a, A32), rel(h, H32)))
+        assert rel(a, A32) <= bar_a and rel(a, A64) <= 1e-4
+        assert rel(h, H32) <= bar_h and rel(h, H64) <= 1e-4
+        if first is None:
+            first = (a, h, n)
+        else:
+            assert all(np.array_equal(u, v) for u, v in zip(first, (a, h, n)))
+        assert not np.any(op.backward(Xd, frame=1).cpu().numpy())
+        assert not np.any(op.forward(dd, accumulate=False, frame=1).cpu().numpy())
+        assert not np.any(op.normal(Xd, frame=1).cpu().numpy())
