@@ -617,6 +617,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
   constexpr int P = S / 16, LD = fft4_ld(S);
   __shared__ float4 buf4[FFT_LINES * LD / 2];
   __shared__ float2 tws[S];
+  __shared__ float wzs[S];
   float2 *buf = reinterpret_cast<float2 *>(buf4);
   const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
   const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
@@ -630,7 +631,6 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
   const float wxv = lok ? wx[a.og[0] + xi] : 0.f;
   const float2 *sm = smaps + (int64_t)ch * a.chunk_smap;
   float2 xv[P], sv[P];
-  float wzv[P];
 #pragma unroll
   for (int j = 0; j < P; j++) {
     const int zi = t + 16 * j - ogz;
@@ -638,21 +638,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
       const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
       xv[j] = X[r + xi];
       sv[j] = sm[(((int64_t)y * a.N[0] + xi) * a.N[2] + zi) * CC + (gl & (CC - 1))];
-      wzv[j] = wz[ogz + zi];
     }
   }
+  // the z window in LDS with the twiddles, while the loads are in flight
+  for (int q = threadIdx.x; q < S; q += FFT_THREADS) {
+    tws[q] = tw[q];
+    wzs[q] = wz[q];
+  }
+  __syncthreads();
   float2 v[P];
 #pragma unroll
   for (int j = 0; j < P; j++) {
     const int zi = t + 16 * j - ogz;
     v[j] = make_float2(0.f, 0.f);
     if (lok && zi >= 0 && zi < a.N[2]) {
-      const float wt = (wzv[j] * wyv) * wxv;
+      const float wt = (wzs[ogz + zi] * wyv) * wxv;
       v[j] = cmul(make_float2(wt * xv[j].x, wt * xv[j].y), sv[j]);
     }
   }
-  load_twiddles<S>(tws, tw);
-  __syncthreads();
   fft4_rows<S, -1>(v, t, buf + l * LD, tws);
   __syncthreads();
   float2 *gch = grid + (int64_t)ch * a.chunk_grid + ((int64_t)(a.og[1] + y) * a.S[0] + a.og[0]) * CC + gl;
