@@ -123,13 +123,36 @@ __global__ __launch_bounds__(256) void k_wave_lines(float2 *X, int N0, int N1, i
 }
 
 // ------------------------------------------------------------ threshold
+#define THR_DEPTH 4                         // bisection steps per counting pass
+#define THR_NODES ((1 << THR_DEPTH) - 1)    // candidate values per pass
 struct ThrState {
   unsigned int umin, umax;  // bit patterns of min / max |c| (non-negative floats)
   float value, max_t, min_t, thr;
   int iter, done;
   unsigned long long counter;
   long long found, target, accuracy, total;
+  unsigned long long cnt[THR_NODES];  // count(|c| < candidate) of this pass
 };
+
+// The next THR_DEPTH values the bisection can query, breadth first: node k
+// (value v, bracket [lo, hi]) has children 2k + 1 (count > target: max = v)
+// and 2k + 2 (min = v), each at 0.5 * (max + min) as the sequential step
+// computes it, so that a pass counts exactly the values the sequence visits.
+__device__ __forceinline__ void thr_tree(const ThrState &s, float (&v)[THR_NODES]) {
+  float lo[THR_NODES], hi[THR_NODES];
+  v[0] = s.value;
+  lo[0] = s.min_t;
+  hi[0] = s.max_t;
+#pragma unroll
+  for (int k = 0; 2 * k + 2 < THR_NODES; k++) {
+    lo[2 * k + 1] = lo[k];
+    hi[2 * k + 1] = v[k];
+    v[2 * k + 1] = 0.5f * (hi[2 * k + 1] + lo[2 * k + 1]);
+    lo[2 * k + 2] = v[k];
+    hi[2 * k + 2] = hi[k];
+    v[2 * k + 2] = 0.5f * (hi[2 * k + 2] + lo[2 * k + 2]);
+  }
+}
 
 __device__ __forceinline__ float cabs_rn(float2 v) {
   return (float)sqrt((double)v.x * v.x + (double)v.y * v.y);
@@ -170,6 +193,7 @@ __global__ void k_thr_init(ThrState *s, long long total, float fraction) {
   s->iter = 0;
   s->found = total;
   s->counter = 0ull;
+  for (int k = 0; k < THR_NODES; k++) s->cnt[k] = 0ull;
   s->done = 0;
   if (s->target < 2) {
     s->value = 0.0f;
@@ -179,29 +203,60 @@ __global__ void k_thr_init(ThrState *s, long long total, float fraction) {
   }
 }
 
+// One counting pass: count(|c| < v) for every candidate of thr_tree.
 __global__ __launch_bounds__(256) void k_thr_count(const float2 *__restrict__ X, int64_t n, ThrState *s) {
-  if (s->done) return;
-  const float value = s->value;
-  unsigned int cnt = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    cnt += cabs_rn(X[i]) < value ? 1u : 0u;
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
-  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s->counter, (unsigned long long)cnt);
+  __shared__ unsigned int s_cnt[THR_NODES];
+  if (s->done) return;  // (uniform)
+  float v[THR_NODES];
+  thr_tree(*s, v);
+  if (threadIdx.x < THR_NODES) s_cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  unsigned int cnt[THR_NODES];
+#pragma unroll
+  for (int k = 0; k < THR_NODES; k++) cnt[k] = 0u;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float a = cabs_rn(X[i]);
+#pragma unroll
+    for (int k = 0; k < THR_NODES; k++) cnt[k] += a < v[k] ? 1u : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < THR_NODES; k++) {
+    unsigned int c = cnt[k];
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_cnt[k], c);
+  }
+  __syncthreads();
+  if (threadIdx.x < THR_NODES && s_cnt[threadIdx.x])
+    atomicAdd(&s->cnt[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
 }
 
 // one bisection step after a count (threshold.cpp:337-378)
+// Up to THR_DEPTH steps of threshold.cpp:250-380's bisection, from the
+// counts of one pass: the same values, counts and stopping rule as one step
+// per pass.
 __global__ void k_thr_step(ThrState *s) {
   if (s->done) return;
-  s->found = (long long)s->counter;
-  s->counter = 0ull;
-  s->iter++;
-  if (!(llabs(s->found - s->target) > s->accuracy && s->iter < 64)) {
-    s->done = 1;
-    return;
+  float v[THR_NODES];
+  thr_tree(*s, v);
+  int node = 0;
+  for (int lvl = 0; lvl < THR_DEPTH; lvl++) {
+    s->value = v[node];
+    s->found = (long long)s->cnt[node];
+    s->iter++;
+    if (!(llabs(s->found - s->target) > s->accuracy && s->iter < 64)) {
+      s->done = 1;
+      break;
+    }
+    if (s->found > s->target) {
+      s->max_t = v[node];
+      node = 2 * node + 1;
+    } else {
+      s->min_t = v[node];
+      node = 2 * node + 2;
+    }
+    if (lvl == THR_DEPTH - 1) s->value = 0.5f * (s->max_t + s->min_t);
   }
-  if (s->found > s->target) s->max_t = s->value;
-  else s->min_t = s->value;
-  s->value = 0.5f * (s->max_t + s->min_t);
+  for (int k = 0; k < THR_NODES; k++) s->cnt[k] = 0ull;
 }
 
 // global threshold = scale * value; optionally logged per FISTA iteration
@@ -318,7 +373,7 @@ static int fraction_threshold(mrn_op *op, const float2 *X, int64_t n, float frac
   hipLaunchKernelGGL(k_thr_reset, dim3(1), dim3(1), 0, st, s);
   hipLaunchKernelGGL(k_thr_minmax, dim3(grid_for(n)), dim3(256), 0, st, X, n, s);
   hipLaunchKernelGGL(k_thr_init, dim3(1), dim3(1), 0, st, s, (long long)n, fraction);
-  for (int r = 0; r < 64; r++) {
+  for (int r = 0; r < (64 + THR_DEPTH - 1) / THR_DEPTH; r++) {
     hipLaunchKernelGGL(k_thr_count, dim3(grid_for(n)), dim3(256), 0, st, X, n, s);
     hipLaunchKernelGGL(k_thr_step, dim3(1), dim3(1), 0, st, s);
   }
