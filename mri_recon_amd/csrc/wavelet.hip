@@ -13,10 +13,10 @@
 // (wavelet3D.cpp:143-190), computed from its closed form.  A line transform
 // stages FL lines in LDS and writes the result back in place; the inverse is
 // evaluated in gather form (each output sums its two contributing taps).
-// The fraction threshold runs entirely on the device: a min/max pass, then up
-// to 64 rounds of (count below value, one-thread bisection step), each round
-// a no-op once the bracket has converged, so the whole step can be captured
-// in a hipGraph.  |c| is evaluated as sqrt in double rounded to float, so the
+// The fraction threshold runs entirely on the device: a min/max pass, then 16
+// rounds of (count below the bisection's next 15 candidate values, one-thread
+// replay of up to 4 bisection steps), each round a no-op once the bracket has
+// converged, so the whole step can be captured in a hipGraph.  |c| is evaluated as sqrt in double rounded to float, so the
 // counts and the threshold do not depend on the order of operations.
 #include <cmath>
 #include <cstring>
@@ -175,7 +175,18 @@ __global__ __launch_bounds__(256) void k_thr_minmax(const float2 *__restrict__ X
     mn = fminf(mn, __shfl_xor(mn, o));
     mx = fmaxf(mx, __shfl_xor(mx, o));
   }
+  // one atomic pair per workgroup: atomics on one address serialise in L2
+  __shared__ float s_mn[256 / 64], s_mx[256 / 64];
   if ((threadIdx.x & 63) == 0) {
+    s_mn[threadIdx.x >> 6] = mn;
+    s_mx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      mn = fminf(mn, s_mn[w]);
+      mx = fmaxf(mx, s_mx[w]);
+    }
     atomicMin(&s->umin, __float_as_uint(mn));
     atomicMax(&s->umax, __float_as_uint(mx));
   }
@@ -371,7 +382,7 @@ static int fraction_threshold(mrn_op *op, const float2 *X, int64_t n, float frac
                               const int *log_counter, hipStream_t st) {
   ThrState *s = (ThrState *)op->dthr;
   hipLaunchKernelGGL(k_thr_reset, dim3(1), dim3(1), 0, st, s);
-  hipLaunchKernelGGL(k_thr_minmax, dim3(grid_for(n)), dim3(256), 0, st, X, n, s);
+  hipLaunchKernelGGL(k_thr_minmax, dim3(grid_for(n) < 1024 ? grid_for(n) : 1024), dim3(256), 0, st, X, n, s);
   hipLaunchKernelGGL(k_thr_init, dim3(1), dim3(1), 0, st, s, (long long)n, fraction);
   for (int r = 0; r < (64 + THR_DEPTH - 1) / THR_DEPTH; r++) {
     hipLaunchKernelGGL(k_thr_count, dim3(grid_for(n)), dim3(256), 0, st, X, n, s);
