@@ -212,3 +212,26 @@ def test_kspace_support_pruning_cartesian_kz_and_empty_frame():
         assert not np.any(op.backward(Xd, frame=1).cpu().numpy())
         assert not np.any(op.forward(dd, accumulate=False, frame=1).cpu().numpy())
         assert not np.any(op.normal(Xd, frame=1).cpu().numpy())
+
+
+def test_kspace_support_pruning_bin_edge_2_and_autotune():
+    """The support rows depend on the bin edge (a sample binned at b reaches
+    [2b, 2b + 1 + W] with 2-cell bins): with bin_edge 2 the pruned passes
+    still meet the oracle's bar, and the autotuner, which rebuilds the bins
+    (and the support) for each candidate, leaves every result unchanged."""
+    N, C = 48, 16
+    traj, X, smaps, d = problem(N, C, 101)
+    o32 = orc.OracleOp(N, overgrid=1.5, dwin=2.5, precision="f32")
+    A32, H32 = o32.image_to_kspace(X, smaps, traj), o32.kspace_to_image(d, smaps, traj)
+    for edge in (2, 1):
+        op = GridFFT(N, ncoils=C, overgrid=1.5, dwin=2.5, bin_edge=edge)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        a = op.backward(to_dev(X)).cpu().numpy()
+        h = op.forward(to_dev(d), accumulate=False).cpu().numpy()
+        print("bin edge %d: A %.2e AH %.2e" % (edge, rel(a, A32), rel(h, H32)))
+        assert rel(a, A32) <= 1e-5 and rel(h, H32) <= 1e-5
+    before = op.normal(to_dev(X)).cpu().numpy()
+    op.autotune()
+    after = op.normal(to_dev(X)).cpu().numpy()
+    assert rel(after, before) <= 1e-6
