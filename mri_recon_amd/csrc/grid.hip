@@ -804,7 +804,7 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 #define SG_BATCH 16
 #define SG_MAXROWS 144
 template <int CCT, bool MM>
-__global__ __launch_bounds__(256, 5) void k_scatter_gd(
+__global__ __launch_bounds__(256, 6) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int32_t *__restrict__ bin_start, const int4 *__restrict__ subs, int nsub, Geom g,
     Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
