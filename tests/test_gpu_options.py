@@ -133,6 +133,32 @@ def test_gather_coils_per_lane_same_bits(C):
         GridFFT(32, ncoils=C, gather_coils=3)
 
 
+def test_xcd_block_order_same_bits():
+    """Workgroups go to the 8 XCDs in runs of consecutive blocks: a
+    permutation of the block indices for any run length, with the blocks
+    that do not fill a whole group of 8 runs left in hardware order.  The
+    gather and the FFT passes write each output from one lane, so A gives the
+    same bits for every run length; the scatter's atomics add in another
+    order, so A^H agrees to FP32 rounding.  Here the gather has 225 blocks per
+    coil block: runs of 1, 3 and 7 leave partial groups at the end, runs of
+    64 leave every block in hardware order."""
+    N, C = 48, 16
+    traj, X, smaps, d = problem(N, C, 83)
+    outs = []
+    for run in (1, 3, 7, 64):
+        op = GridFFT(N, ncoils=C, overgrid=1.5, dwin=2.5, xcd_run=run, xcd_run_gather=run,
+                     xcd_run_fft=run)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        outs.append((op.backward(to_dev(X)).cpu().numpy(), op.forward(to_dev(d)).cpu().numpy()))
+    for a, h in outs[1:]:
+        assert np.array_equal(a, outs[0][0])
+        assert rel(h, outs[0][1]) <= 1e-6
+    o32 = orc.OracleOp(N, overgrid=1.5, dwin=2.5, precision="f32")
+    assert rel(outs[0][0], o32.image_to_kspace(X, smaps, traj)) <= 1e-5
+    assert rel(outs[0][1], o32.kspace_to_image(d, smaps, traj)) <= 1e-5
+
+
 def test_kspace_support_pruning_across_frames():
     """With 16-coil blocks and a four-step grid the forward x pass stores, and
     the inverse x pass loads, only the x cells that some sample of the frame
