@@ -62,7 +62,30 @@ def build(only=None, jobs=None, verbose=False):
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     build_facade_test()
+    build_io()
     return LIB
+
+
+IO_SRC = os.path.join(CSRC, "raw_io.cpp")
+IO_LIB = os.path.join(PKG, "libmrn_io.so")
+HDF5_PREFIX = os.environ.get("MRN_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io():
+    """libmrn_io.so: the MRI_Raw.h5 reader/writer (host C++ over the HDF5 C
+    library; no device code, so the operator library does not depend on HDF5)."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError("hdf5.h not found under %s (set MRN_HDF5_PREFIX)" % inc)
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-idirafter", inc, IO_SRC, "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libmrn_io build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_cg.cpp")
@@ -87,8 +110,9 @@ def main(argv=None):
     ap.add_argument("--only", nargs="*")
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("--io", action="store_true", help="build only libmrn_io.so")
     a = ap.parse_args(argv)
-    print(build(a.only, a.j, a.v))
+    print(build_io() if a.io else build(a.only, a.j, a.v))
 
 
 if __name__ == "__main__":

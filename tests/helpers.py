@@ -37,3 +37,33 @@ def to_dev(a):
 
 def traj_dev(traj):
     return [to_dev(np.ascontiguousarray(t, dtype=np.float32)) for t in traj]
+
+
+def stable_direction(d, k, h, N, overgrid, dwin=-1.0):
+    """Zero the rows of d whose tap range would change between k - h d and k + h d.
+
+    The truncated Kaiser-Bessel kernel (and the reference's table) is
+    discontinuous at |u| = dwin, so a finite difference across a support edge
+    measures the jump, not the derivative.  Tap ranges are computed as the
+    operator computes them (dk = k * g + S/2 in float32, taps
+    ceil(dk - dwin) .. floor(dk + dwin)) and as the FP64 model does.
+    Returns (d with those rows zeroed, number of rows kept)."""
+    from oracle import oracle as orc
+    geo = orc.OracleOp(N, overgrid=overgrid, dwin=dwin, precision="f64").geometry()
+    dw = np.float32(geo["dwin"][0])
+
+    def ranges(kk):
+        r = []
+        for a in range(3):
+            S2 = int(geo["S"][a]) // 2
+            dk = (kk[:, a].astype(np.float32) * np.float32(geo["grid"][a])).astype(np.float32) + np.float32(S2)
+            dk64 = kk[:, a].astype(np.float64) * float(geo["grid"][a]) + S2
+            r += [np.ceil((dk - dw).astype(np.float32)), np.floor((dk + dw).astype(np.float32)),
+                  np.ceil(dk64 - float(dw)), np.floor(dk64 + float(dw))]
+        return r
+
+    r0, rp, rm = ranges(k), ranges(k + h * d), ranges(k - h * d)
+    keep = np.ones(k.shape[0], bool)
+    for a, b, c in zip(r0, rp, rm):
+        keep &= (a == b) & (a == c)
+    return d * keep[:, None], int(keep.sum())

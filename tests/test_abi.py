@@ -80,3 +80,15 @@ def test_cpp_facade_compiles_against_header(tmp_path):
     r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_io_library_loads_and_exports_everything():
+    """libmrn_io.so (the MRI_Raw.h5 reader/writer) exports every symbol
+    include/mrn_io.h declares."""
+    from mri_recon_amd import raw_io
+    txt = open(os.path.join(ROOT, "include", "mrn_io.h")).read()
+    syms = sorted(set(re.findall(r"\b(mrn_raw_[a-z_0-9]+)\s*\(", txt)))
+    assert len(syms) == 16
+    L = raw_io.lib()
+    for s in syms:
+        assert hasattr(L, s), s
