@@ -63,7 +63,7 @@ typedef struct mrn_params {
   int32_t kernel_mode;   /* MRN_KMODE_* */
   int32_t form;          /* MRN_FORM_* */
   int32_t coil_chunk;    /* coils per grid cell block (1,2,4,8,16); 0 = auto */
-  int32_t tile[3];       /* scatter tile edge (x,y,z); 0 = auto */
+  int32_t tile[3];       /* scatter tile edge (x,y,z): 0 (auto) or 4; other edges are MRN_EUNSUPPORTED */
   int32_t max_visits;    /* samples per scatter work item; 0 = auto */
   int32_t bin_edge;      /* sample sort bin edge in cells (1 or 2); 0 = auto */
   int32_t xcd_run;       /* scatter work items per XCD run (1 = hardware order); 0 = auto */
@@ -205,15 +205,35 @@ int mrn_get_fista_thresholds(mrn_op *op, double *out, int n, void *stream);
 int mrn_traj_grad(mrn_op *op, int frame, const void *X, const void *g, float *grad_k,
                   void *stream);
 
-/* Time the tunable launch shapes of one frame and keep the fastest
- * (scatter tile, samples per work item, gather block).  No counterpart in
- * the reference. */
-int mrn_autotune(mrn_op *op, int frame, void *stream);
+/* Launch shapes of the batched form (mrn_get_tuning). */
+typedef struct mrn_tuning {
+  int32_t bin_edge;       /* sample bin edge in cells */
+  int32_t max_visits;     /* samples per scatter work item */
+  int32_t xcd_run;        /* scatter work items per XCD run */
+  int32_t xcd_run_gather; /* gather workgroups per XCD run */
+  int32_t gather_coils;   /* coils per lane in the gather */
+  int32_t xcd_run_fft;    /* y and z FFT pass workgroups per XCD run */
+  int32_t fft_stockham;   /* 1: LDS Stockham passes; 0: four-step passes with support pruning */
+  int32_t scatter_valu;   /* 1: vector-ALU tile accumulate; 0: matrix cores */
+  int32_t tile[3];        /* scatter tile edge (4, 4, 4) */
+  int32_t from_cache;     /* 1: the last mrn_autotune applied a cached choice without timing */
+  float normal_ms;        /* E'E time of the tuned frame with the choice (timed or cached) */
+  int32_t reserved[3];
+} mrn_tuning;
 
-/* The scatter launch shape in use (after mrn_autotune, the tuned one): the
- * bin edge in cells and the samples per work item.  No counterpart in the
- * reference. */
-int mrn_get_tuning(const mrn_op *op, int32_t *bin_edge, int32_t *max_visits);
+/* Time the batched form's launch shapes on one frame's trajectory and keep
+ * the fastest: scatter bin edge x samples per work item x accumulate form,
+ * gather coils per lane x XCD run length, the FFT passes' XCD run length,
+ * and the FFT pass form (four-step with pruning or Stockham).  The choice
+ * is looked up in, or else added to, the JSON file cache_path (NULL: no
+ * cache), keyed by sizes, kernel, sample and coil counts and device; a hit
+ * applies it without timing.  On failure the previous shapes are restored.
+ * Every candidate computes the same operator up to FP32 summation order.
+ * No counterpart in the reference. */
+int mrn_autotune(mrn_op *op, int frame, const char *cache_path, void *stream);
+
+/* The launch shapes in use.  No counterpart in the reference. */
+int mrn_get_tuning(const mrn_op *op, mrn_tuning *out);
 
 /* Average device time (ms, HIP events on `stream`) of each batched stage of
  * one frame over `reps` runs: [0] set_image+z FFT, [1] y FFT, [2] x FFT,
@@ -226,6 +246,26 @@ int mrn_get_tuning(const mrn_op *op, int32_t *bin_edge, int32_t *max_visits);
 #define MRN_NSTAGES 9
 int mrn_stage_times(mrn_op *op, int frame, const void *X, int reps, float ms[MRN_NSTAGES],
                     void *stream);
+
+/* The op's oversampled k-space grids (the reference's public member
+ * gridFFT::k3d_grid, gridFFT.h:30, which RECON::kspace_residual zeroes and
+ * fills through chop_grid_forward / chop_grid_backward, recon_lib.cpp:
+ * 2244-2252): *grid receives the device pointer of complex64
+ * [nch][Sz][Sy][Sx][CC], coil c at block c / CC, lane c % CC; dims =
+ * {nch, CC, Sx, Sy, Sz}.  The plain form has one grid of one coil.  The
+ * contents between calls are the last call's working state; callers may
+ * read or write them between chop_grid calls. */
+int mrn_get_grid(mrn_op *op, void **grid, int64_t dims[5]);
+/* k3d_grid = 0 for every coil (recon_lib.cpp:2244). */
+int mrn_grid_zero(mrn_op *op, void *stream);
+/* gridFFT::chop_grid_forward (gridFFT.cpp:1009-1164) for every coil of
+ * frame f: grid_c += the scatter of kw * taper * data_c (data [C][M]), added
+ * to the grids' contents; no FFT.  Plain form: C = 1. */
+int mrn_chop_grid_forward(mrn_op *op, int frame, const void *data, void *stream);
+/* gridFFT::chop_grid_backward (gridFFT.cpp:1166-1322) for every coil of
+ * frame f: data_c = the gather of grid_c at the samples (minus sub when not
+ * NULL); kw == 0 and out-of-grid samples give 0; no FFT.  Plain form: C = 1. */
+int mrn_chop_grid_backward(mrn_op *op, int frame, void *data, const void *sub, void *stream);
 
 /* Wait for the work queued on `stream` (NULL = the null stream). */
 int mrn_sync(mrn_op *op, void *stream);

@@ -251,7 +251,9 @@ class SenseOperator {
     r.resize((size_t)(iters > 0 ? iters : 0));
     return r;
   }
-  void autotune(int frame = 0) { check(mrn_autotune(op_, frame, stream), op_, "autotune"); }
+  void autotune(int frame = 0, const char *cache_path = nullptr) {
+    check(mrn_autotune(op_, frame, cache_path, stream), op_, "autotune");
+  }
   void sync() { check(mrn_sync(op_, stream), op_, "sync"); }
   mrn_op *handle() const { return op_; }
 

@@ -27,8 +27,27 @@ EXPORTS = [
     "mrn_dcf_calc", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
     "mrn_get_fista_llr_smax", "mrn_gate_params_default", "mrn_gate_parse_args", "mrn_gate_scale_times",
     "mrn_pregate",
+    "mrn_get_grid", "mrn_grid_zero", "mrn_chop_grid_forward", "mrn_chop_grid_backward",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
+
+
+class MrnTuning(ctypes.Structure):
+    """mrn_tuning (include/mrn.h)."""
+    _fields_ = [
+        ("bin_edge", ctypes.c_int32),
+        ("max_visits", ctypes.c_int32),
+        ("xcd_run", ctypes.c_int32),
+        ("xcd_run_gather", ctypes.c_int32),
+        ("gather_coils", ctypes.c_int32),
+        ("xcd_run_fft", ctypes.c_int32),
+        ("fft_stockham", ctypes.c_int32),
+        ("scatter_valu", ctypes.c_int32),
+        ("tile", ctypes.c_int32 * 3),
+        ("from_cache", ctypes.c_int32),
+        ("normal_ms", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
 
 
 class GateParams(ctypes.Structure):
@@ -152,8 +171,8 @@ def lib():
             "mrn_power_method": (i32, [vp, i32, ctypes.POINTER(dbl), vp]),
             "mrn_fista": (i32, [vp, vp, i32, dbl, i32, vp, vp]),
             "mrn_traj_grad": (i32, [vp, i32, vp, vp, vp, vp]),
-            "mrn_autotune": (i32, [vp, i32, vp]),
-            "mrn_get_tuning": (i32, [vp, vp, vp]),
+            "mrn_autotune": (i32, [vp, i32, ctypes.c_char_p, vp]),
+            "mrn_get_tuning": (i32, [vp, ctypes.POINTER(MrnTuning)]),
             "mrn_dcf_params_default": (None, [ctypes.POINTER(DcfParams)]),
             "mrn_dcf_parse_args": (i32, [ctypes.POINTER(DcfParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
             "mrn_dcf_grid_size": (i32, [ctypes.POINTER(DcfParams), vp]),
@@ -168,6 +187,10 @@ def lib():
             "mrn_gate_scale_times": (i32, [ctypes.POINTER(GateParams), i64, vp, vp]),
             "mrn_pregate": (i32, [ctypes.POINTER(GateParams), i64, i64, vp, vp, vp, vp, vp, vp, i32, vp, vp,
                                   vp, vp, vp, vp, vp, vp]),
+            "mrn_get_grid": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(i64)]),
+            "mrn_grid_zero": (i32, [vp, vp]),
+            "mrn_chop_grid_forward": (i32, [vp, i32, vp, vp]),
+            "mrn_chop_grid_backward": (i32, [vp, i32, vp, vp, vp]),
             "mrn_sync": (i32, [vp, vp]),
             "mrn_status_string": (ctypes.c_char_p, [i32]),
             "mrn_last_error": (ctypes.c_char_p, [vp]),

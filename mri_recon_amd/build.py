@@ -15,7 +15,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmrn.so")
-SOURCES = ["fft.hip", "grid.hip", "ops.hip", "grad_tune.hip", "wavelet.hip", "dcf.hip", "llr.hip", "gate.hip", "setup.cpp", "api.cpp"]
+SOURCES = ["fft.hip", "grid.hip", "ops.hip", "grad_tune.hip", "wavelet.hip", "dcf.hip", "llr.hip", "gate.hip", "setup.cpp", "api.cpp", "tune.cpp"]
 ARCH = os.environ.get("MRN_OFFLOAD_ARCH", "gfx950")
 
 

@@ -17,12 +17,7 @@
 #include "mrn_internal.hpp"
 
 namespace mrn {
-int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, bool, hipStream_t);
-int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
-int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
-int launch_scatter_zero(mrn_op *, float2 *, const FrameData &, hipStream_t);
 int recompute_taper(mrn_op *, FrameData &, hipStream_t);
-int launch_scatter_tiles(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int launch_gather_plain(mrn_op *, const float2 *, float2 *, int64_t, const float *, const float *,
                         const float *, const float *, const float2 *, hipStream_t);
 int launch_scatter_plain(mrn_op *, float2 *, const float2 *, int64_t, const float *, const float *,
@@ -44,7 +39,7 @@ int op_power_step(mrn_op *, float2 *, const float2 *, int64_t, double *, hipStre
 int op_fista_momentum(mrn_op *, float2 *, float2 *, int64_t, hipStream_t);
 int op_fista_finish(mrn_op *, float2 *, float2 *, int64_t, float, double *, hipStream_t);
 int traj_grad_impl(mrn_op *, int, const float2 *, const float2 *, float *, hipStream_t);
-int autotune_impl(mrn_op *, int, hipStream_t);
+int autotune_impl(mrn_op *, int, const char *, hipStream_t);
 
 static inline bool batched(const mrn_op *op) { return op->p.form == MRN_FORM_BATCHED; }
 static inline int64_t ncell(const mrn_op *op) {
@@ -168,6 +163,10 @@ int fwd_grid_pub(mrn_op *op, const float2 *X, const FrameData *fr, hipStream_t s
 static int adj_grid(mrn_op *op, float2 *X, bool overwrite, const FrameData *fr, hipStream_t st) {
   for (int q = 0; q < 3; q++) MRN_TRY(adj_stage(op, q, X, overwrite, fr, st));
   return MRN_OK;
+}
+
+int adj_grid_pub(mrn_op *op, float2 *X, bool overwrite, const FrameData *fr, hipStream_t st) {
+  return adj_grid(op, X, overwrite, fr, st);
 }
 
 // ---------------------------------------------------- plain transforms
@@ -359,30 +358,25 @@ static int run_iters(mrn_op *op, int iters, bool graph, hipStream_t st, Body bod
     for (int it = 0; it < iters; it++) MRN_TRY(body(st));
     return MRN_OK;
   }
-  hipEvent_t ev0, ev1;
-  MRN_HIP(op, hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-  MRN_HIP(op, hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-  MRN_HIP(op, hipEventRecord(ev0, st));
-  MRN_HIP(op, hipStreamWaitEvent(op->stream, ev0, 0));
-  hipGraph_t graph_h = nullptr;
-  hipGraphExec_t exec = nullptr;
+  Events ev;
+  MRN_HIP(op, ev.create(2, hipEventDisableTiming));
+  MRN_HIP(op, hipEventRecord(ev[0], st));
+  MRN_HIP(op, hipStreamWaitEvent(op->stream, ev[0], 0));
+  GraphGuard gg;
   MRN_HIP(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
-  int rc = body(op->stream);
-  hipError_t ec = hipStreamEndCapture(op->stream, &graph_h);
-  if (rc != MRN_OK) {
-    if (graph_h) hipGraphDestroy(graph_h);
-    return rc;
-  }
+  const int rc = body(op->stream);
+  const hipError_t ec = hipStreamEndCapture(op->stream, &gg.g);
+  if (rc != MRN_OK) return rc;
   MRN_HIP(op, ec);
-  MRN_HIP(op, hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0));
-  for (int it = 0; it < iters; it++) MRN_HIP(op, hipGraphLaunch(exec, op->stream));
-  MRN_HIP(op, hipEventRecord(ev1, op->stream));
-  MRN_HIP(op, hipStreamWaitEvent(st, ev1, 0));
+  MRN_HIP(op, hipGraphInstantiate(&gg.x, gg.g, nullptr, nullptr, 0));
+  gg.launched = op->stream;
+  for (int it = 0; it < iters; it++) {
+    gg.was_launched = true;
+    MRN_HIP(op, hipGraphLaunch(gg.x, op->stream));
+  }
+  MRN_HIP(op, hipEventRecord(ev[1], op->stream));
+  MRN_HIP(op, hipStreamWaitEvent(st, ev[1], 0));
   MRN_HIP(op, hipStreamSynchronize(op->stream));
-  hipGraphExecDestroy(exec);
-  hipGraphDestroy(graph_h);
-  hipEventDestroy(ev0);
-  hipEventDestroy(ev1);
   return MRN_OK;
 }
 
@@ -596,25 +590,25 @@ int mrn_cg_iterate(mrn_op *op, void *Xv, int iters, int use_graph, double *trace
       MRN_HIP(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
       int rc = cg_step(op, X, true, op->stream);
       hipError_t ec = hipStreamEndCapture(op->stream, &op->cg_graph);
-      if (rc != MRN_OK) {
+      if (rc != MRN_OK || ec != hipSuccess) {
         drop_cg_graph(op);
-        return rc;
+        return rc != MRN_OK ? rc : fail(op, MRN_EHIP, hipGetErrorString(ec));
       }
-      MRN_HIP(op, ec);
-      MRN_HIP(op, hipGraphInstantiate(&op->cg_exec, op->cg_graph, nullptr, nullptr, 0));
+      ec = hipGraphInstantiate(&op->cg_exec, op->cg_graph, nullptr, nullptr, 0);
+      if (ec != hipSuccess) {
+        drop_cg_graph(op);
+        return fail(op, MRN_EHIP, hipGetErrorString(ec));
+      }
       op->cg_X = Xv;
       op->cg_version = op->version;
     }
-    hipEvent_t ev0, ev1;
-    MRN_HIP(op, hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-    MRN_HIP(op, hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-    MRN_HIP(op, hipEventRecord(ev0, st));
-    MRN_HIP(op, hipStreamWaitEvent(op->stream, ev0, 0));
+    Events ev;
+    MRN_HIP(op, ev.create(2, hipEventDisableTiming));
+    MRN_HIP(op, hipEventRecord(ev[0], st));
+    MRN_HIP(op, hipStreamWaitEvent(op->stream, ev[0], 0));
     for (int it = 0; it < iters; it++) MRN_HIP(op, hipGraphLaunch(op->cg_exec, op->stream));
-    MRN_HIP(op, hipEventRecord(ev1, op->stream));
-    MRN_HIP(op, hipStreamWaitEvent(st, ev1, 0));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    MRN_HIP(op, hipEventRecord(ev[1], op->stream));
+    MRN_HIP(op, hipStreamWaitEvent(st, ev[1], 0));
   } else {
     for (int it = 0; it < iters; it++) MRN_TRY(cg_step(op, X, true, st));
   }
@@ -731,18 +725,29 @@ int mrn_traj_grad(mrn_op *op, int f, const void *X, const void *g, float *grad_k
   return traj_grad_impl(op, f, (const float2 *)X, (const float2 *)g, grad_k, pick(op, stream));
 }
 
-int mrn_autotune(mrn_op *op, int f, void *stream) {
+int mrn_autotune(mrn_op *op, int f, const char *cache_path, void *stream) {
   if (!op) return MRN_EINVAL;
   MRN_TRY(check_frame(op, f, false));
   if (!batched(op)) return MRN_OK;
+  MRN_TRY(ensure_solver(op));
   op->version++;
-  return autotune_impl(op, f, pick(op, stream));
+  return autotune_impl(op, f, cache_path, pick(op, stream));
 }
 
-int mrn_get_tuning(const mrn_op *op, int32_t *bin_edge, int32_t *max_visits) {
-  if (!op || !bin_edge || !max_visits) return MRN_EINVAL;
-  *bin_edge = op->t.B;
-  *max_visits = op->t.maxv;
+int mrn_get_tuning(const mrn_op *op, mrn_tuning *out) {
+  if (!op || !out) return MRN_EINVAL;
+  memset(out, 0, sizeof(*out));
+  out->bin_edge = op->t.B;
+  out->max_visits = op->t.maxv;
+  out->xcd_run = op->t.xrun;
+  out->xcd_run_gather = op->t.xrun_gather;
+  out->gather_coils = op->gather_coils > 0 ? std::min(op->gather_coils, op->g.CC) : std::min(4, op->g.CC);
+  out->xcd_run_fft = op->xcd_run_fft;
+  out->fft_stockham = op->fft_stockham ? 1 : 0;
+  out->scatter_valu = (op->scatter_valu || op->g.CC != 16) ? 1 : 0;
+  for (int a = 0; a < 3; a++) out->tile[a] = op->t.T;
+  out->from_cache = op->tune_from_cache;
+  out->normal_ms = op->tune_ms;
   return MRN_OK;
 }
 
@@ -754,8 +759,8 @@ int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[MRN_NS
   FrameData &fr = op->frames[f];
   const float2 *X = (const float2 *)Xv;
   MRN_TRY(ensure_solver(op));
-  hipEvent_t ev[MRN_NSTAGES + 1];
-  for (int q = 0; q <= MRN_NSTAGES; q++) MRN_HIP(op, hipEventCreate(&ev[q]));
+  Events ev;
+  MRN_HIP(op, ev.create(MRN_NSTAGES + 1));
   double acc[MRN_NSTAGES] = {0};
   for (int r = 0; r < reps; r++) {
     int e = 0;
@@ -781,9 +786,55 @@ int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[MRN_NS
       acc[q] += t;
     }
   }
-  for (int q = 0; q <= MRN_NSTAGES; q++) (void)hipEventDestroy(ev[q]);
   for (int q = 0; q < MRN_NSTAGES; q++) ms[q] = (float)(acc[q] / reps);
   return MRN_OK;
+}
+
+int mrn_get_grid(mrn_op *op, void **grid, int64_t dims[5]) {
+  if (!op || !grid || !dims) return MRN_EINVAL;
+  const bool b = batched(op);
+  *grid = b ? (void *)op->grid : (void *)op->cgrid;
+  dims[0] = b ? op->nch : 1;
+  dims[1] = b ? op->g.CC : 1;
+  for (int a = 0; a < 3; a++) dims[2 + a] = op->g.S[a];
+  return MRN_OK;
+}
+
+int mrn_grid_zero(mrn_op *op, void *stream) {
+  if (!op) return MRN_EINVAL;
+  const int64_t n = batched(op) ? ncell(op) * op->g.CC * op->nch : ncell(op);
+  return op_zero(op, batched(op) ? op->grid : op->cgrid, n, pick(op, stream));
+}
+
+int mrn_chop_grid_forward(mrn_op *op, int f, const void *data, void *stream) {
+  if (!op) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  FrameData &fr = op->frames[f];
+  if (!data && fr.M > 0) return fail(op, MRN_EINVAL, "null k-space pointer");
+  hipStream_t st = pick(op, stream);
+  if (!batched(op)) {
+    if (op->C != 1) return fail(op, MRN_EUNSUPPORTED, "the plain form's grid holds one coil: chop_grid needs C = 1");
+    return launch_scatter_plain(op, op->cgrid, (const float2 *)data, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2),
+                                traw(fr, 3), st);
+  }
+  MRN_TRY(op_permute_in(op, op->kwork, (const float2 *)data, fr, st));
+  MRN_TRY(launch_weight(op, op->kwork, op->kwork, fr, st));
+  return launch_scatter(op, op->grid, op->kwork, fr, st, true);
+}
+
+int mrn_chop_grid_backward(mrn_op *op, int f, void *data, const void *sub, void *stream) {
+  if (!op) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  FrameData &fr = op->frames[f];
+  if (!data && fr.M > 0) return fail(op, MRN_EINVAL, "null k-space pointer");
+  hipStream_t st = pick(op, stream);
+  if (!batched(op)) {
+    if (op->C != 1) return fail(op, MRN_EUNSUPPORTED, "the plain form's grid holds one coil: chop_grid needs C = 1");
+    return launch_gather_plain(op, op->cgrid, (float2 *)data, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2),
+                               traw(fr, 3), (const float2 *)sub, st);
+  }
+  MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, false, st));
+  return op_permute_out(op, (float2 *)data, op->kwork, fr, (const float2 *)sub, st);
 }
 
 int mrn_sync(mrn_op *op, void *stream) {

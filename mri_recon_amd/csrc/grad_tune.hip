@@ -16,9 +16,6 @@
 
 namespace mrn {
 int fwd_grid_pub(mrn_op *op, const float2 *X, const FrameData *fr, hipStream_t st);
-int launch_gather(mrn_op *, const float2 *, float2 *, const FrameData &, const float2 *, bool, hipStream_t);
-int launch_weight(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
-int launch_scatter(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int build_bins(mrn_op *, FrameData &, int64_t, const float *, const float *, const float *,
                const float *, hipStream_t);
 int op_permute_in(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
@@ -137,59 +134,6 @@ int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float 
                      fr.coords, fr.perm, fr.Mv, g, op->tables_d,
                      (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, fr.Mv * g.CC, grad_k);
   MRN_HIP(op, hipGetLastError());
-  return MRN_OK;
-}
-
-// ------------------------------------------------------------ autotuner
-// The scatter's work-item size (samples per wavefront before a tile is split
-// into partial sums) is timed on the frame's own trajectory and the fastest is
-// kept; every candidate computes the same result.
-int autotune_impl(mrn_op *op, int f, hipStream_t st) {
-  FrameData &fr = op->frames[f];
-  const int keep = op->t.maxv;
-  // non-zero samples so that the timing covers the arithmetic
-  MRN_TRY(op_fill(op, op->kwork, fr.Mv * op->g.CC * op->nch, make_float2(1.f, 0.f), st));
-  hipEvent_t e0, e1;
-  MRN_HIP(op, hipEventCreate(&e0));
-  MRN_HIP(op, hipEventCreate(&e1));
-  float best = 1e30f;
-  int bestv = keep, bestB = op->t.B;
-  auto set_bins = [&](int B) {
-    op->t.B = B;
-    for (int a = 0; a < 3; a++) op->t.nb[a] = (op->g.S[a] + B - 1) / B;
-    op->t.nbins = op->t.nb[0] * op->t.nb[1] * op->t.nb[2];
-  };
-  for (int B : {1, 2})
-    for (int maxv : {1024, 4096, 16384}) {
-        set_bins(B);
-        op->t.maxv = maxv;
-        MRN_TRY(build_bins(op, fr, fr.M, fr.traw, fr.traw + fr.M, fr.traw + 2 * fr.M, fr.traw + 3 * fr.M, st));
-        MRN_TRY(op_fill(op, op->kwork, fr.Mv * op->g.CC * op->nch, make_float2(1.f, 0.f), st));
-        MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));  // warm-up
-        MRN_HIP(op, hipEventRecord(e0, st));
-        for (int r = 0; r < 3; r++) MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
-        MRN_HIP(op, hipEventRecord(e1, st));
-        MRN_HIP(op, hipEventSynchronize(e1));
-        float ms = 0.f;
-        MRN_HIP(op, hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          bestv = maxv;
-          bestB = B;
-        }
-    }
-  set_bins(bestB);
-  op->t.maxv = bestv;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  // rebuild every frame's work list for the chosen size (sample order is unchanged)
-  for (int q = 0; q < op->F; q++) {
-    FrameData &fq = op->frames[q];
-    if (!fq.has_traj) continue;
-    MRN_TRY(build_bins(op, fq, fq.M, fq.traw, fq.traw + fq.M, fq.traw + 2 * fq.M, fq.traw + 3 * fq.M, st));
-    if (fq.has_kdata) MRN_TRY(op_permute_in(op, fq.kdata, fq.kraw, fq, st));
-  }
-  MRN_HIP(op, hipStreamSynchronize(st));
   return MRN_OK;
 }
 

@@ -363,6 +363,16 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
   }
 }
 
+// Whether the forward passes stored cell (x, y, z) for this frame: with its
+// k-space support (xrng, pruned x passes) only the cells some tap reaches
+// were stored, and the staged regions below hold zeros elsewhere instead of
+// whatever an earlier call left in the grid.
+__device__ __forceinline__ bool in_support(const int2 *__restrict__ xrng, const Geom &g, int x, int y, int z) {
+  if (!xrng) return true;
+  const int2 rr = xrng[(int64_t)z * g.S[1] + y];
+  return x >= rr.x && x <= rr.y;
+}
+
 // Tile-staged gather.  Work item gsubs[blockIdx.x] = {tile, first, end} over
 // the tile's home samples (first tap inside the 4x4x4 tile).  The workgroup
 // loads the R^3 cells (R = 4 + NT - 1) that those samples' taps can reach, for
@@ -381,7 +391,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float2 *__restrict__ 
                                                      const int4 *__restrict__ gsubs, Geom g, Tiles t,
                                                      const float *__restrict__ tables, int64_t chunk_grid,
                                                      int64_t chunk_k, const float *__restrict__ taper,
-                                                     int weighted) {
+                                                     int weighted, const int2 *__restrict__ xrng) {
   constexpr int R = NT + 3, HC = CCT / 2;  // region edge, float4 per cell
   constexpr int LPS = HC;                  // lanes per sample
   constexpr int SPP = 256 / LPS;           // samples per pass
@@ -404,7 +414,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float2 *__restrict__ 
       const int rx = cell % R, ry = (cell / R) % R, rz = cell / (R * R);
       const int gx = t0[0] + rx, gy = t0[1] + ry, gz = t0[2] + rz;
       v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (e < NE && gx < g.S[0] && gy < g.S[1] && gz < g.S[2])
+      if (e < NE && gx < g.S[0] && gy < g.S[1] && gz < g.S[2] && in_support(xrng, g, gx, gy, gz))
         v[j] = G4[(((int64_t)gz * g.S[1] + gy) * g.S[0] + gx) * HC + q];
     }
 #pragma unroll
@@ -560,7 +570,7 @@ __global__ __launch_bounds__(256) void k_gather_mm(const float2 *__restrict__ gr
                                                    const int4 *__restrict__ gsubs, Geom g, Tiles t,
                                                    const float *__restrict__ tables, int64_t chunk_grid,
                                                    int64_t chunk_k, const float *__restrict__ taper,
-                                                   int weighted) {
+                                                   int weighted, const int2 *__restrict__ xrng) {
   constexpr int CCT = 16, HC = 8, R = NT + 3, RX = (R + 1) & ~1, XP = RX / 2;
   extern __shared__ float4 reg[];  // [R][R][RX][HC]: cell c holds reals 32c .. 32c+31
   __shared__ int s_rs[32], s_rp[33];
@@ -584,7 +594,7 @@ __global__ __launch_bounds__(256) void k_gather_mm(const float2 *__restrict__ gr
       const int rx = cell % RX, ry = (cell / RX) % R, rz = cell / (RX * R);
       const int gx = t0[0] + rx, gy = t0[1] + ry, gz = t0[2] + rz;
       v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (e < NE && gx < g.S[0] && gy < g.S[1] && gz < g.S[2])
+      if (e < NE && gx < g.S[0] && gy < g.S[1] && gz < g.S[2] && in_support(xrng, g, gx, gy, gz))
         v[j] = G4[(((int64_t)gz * g.S[1] + gy) * g.S[0] + gx) * HC + q];
     }
 #pragma unroll
@@ -807,7 +817,7 @@ template <int CCT, bool MM>
 __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int32_t *__restrict__ bin_start, const int4 *__restrict__ subs, int nsub, Geom g,
-    Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
+    Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum) {
   __shared__ int s_rs[4][SG_MAXROWS];        // first sample of each bin row
   __shared__ int s_rp[4][SG_MAXROWS + 1];    // visits before each row
   __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
@@ -980,6 +990,8 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
         const float v = h ? acc1[r] : acc0[r];
         if (sp.w) {
           if (v != 0.f) atomicAdd(dst, v);
+        } else if (accum) {  // the tile's only work item: a plain read-modify-write
+          *dst += v;
         } else {
           *dst = v;
         }
@@ -994,6 +1006,12 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     for (int c = 0; c < CCT; c++) {
       if (ar[c] != 0.f) atomicAdd(&dst[c].x, ar[c]);
       if (ai[c] != 0.f) atomicAdd(&dst[c].y, ai[c]);
+    }
+  } else if (accum) {
+#pragma unroll
+    for (int c = 0; c < CCT; c++) {
+      const float2 o = dst[c];
+      dst[c] = make_float2(o.x + ar[c], o.y + ai[c]);
     }
   } else {
 #pragma unroll
@@ -1137,7 +1155,8 @@ static int launch_gather_tile(mrn_op *op, const float2 *grid, float2 *out, const
   const Geom &g = op->g;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   hipLaunchKernelGGL((k_gather_tile<NT, CCT>), dim3(fr.ngsub, op->nch), dim3(256), lds, st, grid, out, fr.coords,
-                     sub, fr.bin_start, fr.gsubs, g, op->t, op->tables_d, cg, ck, fr.taper, wflag);
+                     sub, fr.bin_start, fr.gsubs, g, op->t, op->tables_d, cg, ck, fr.taper, wflag,
+                     fft_x_pruned(op) ? fr.xrng : nullptr);
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
 }
@@ -1152,7 +1171,8 @@ static int launch_gather_mm(mrn_op *op, const float2 *grid, float2 *out, const m
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   const size_t lds = gather_mm_lds(NT);
   hipLaunchKernelGGL((k_gather_mm<NT>), dim3(fr.ngsub, op->nch), dim3(256), lds, st, grid, out, fr.coords, sub,
-                     fr.bin_start, fr.gsubs, g, op->t, op->tables_d, cg, ck, fr.taper, wflag);
+                     fr.bin_start, fr.gsubs, g, op->t, op->tables_d, cg, ck, fr.taper, wflag,
+                     fft_x_pruned(op) ? fr.xrng : nullptr);
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
 }
@@ -1233,9 +1253,12 @@ int launch_scatter_zero(mrn_op *op, float2 *grid, const mrn::FrameData &fr, hipS
   return MRN_OK;
 }
 
-// The tile scatter proper; launch_scatter_zero must precede it.
+// The tile scatter proper; launch_scatter_zero must precede it unless
+// accum (add into the grid's contents: tiles split over several work items
+// add with atomics, the others read, add and store).
 int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn::FrameData &fr,
-                         hipStream_t st) {
+                         hipStream_t st, bool accum) {
+  const int af = accum ? 1 : 0;
   const Geom &g = op->g;
   const Tiles &t = op->t;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
@@ -1248,7 +1271,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   if (fr.nsub == 0) return MRN_OK;
   if (g.CC == 16 && !op->scatter_valu) {
     hipLaunchKernelGGL((k_scatter_gd<16, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
-                       fr.subs, fr.nsub, g, t, op->tables_d, cg, ck);
+                       fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);
     MRN_HIP(op, hipGetLastError());
     return MRN_OK;
   }
@@ -1256,7 +1279,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
     hipLaunchKernelGGL((k_scatter_gd<n, false>), g3, dim3(256), sh, st, grid, kin, fr.coords,    \
-                       fr.bin_start, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck);              \
+                       fr.bin_start, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);          \
     break;
     MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)
 #undef MRN_S
@@ -1268,9 +1291,9 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
 }
 
 int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::FrameData &fr,
-                   hipStream_t st) {
-  MRN_TRY(launch_scatter_zero(op, grid, fr, st));
-  return launch_scatter_tiles(op, grid, kin, fr, st);
+                   hipStream_t st, bool accum) {
+  if (!accum) MRN_TRY(launch_scatter_zero(op, grid, fr, st));
+  return launch_scatter_tiles(op, grid, kin, fr, st, accum);
 }
 
 int launch_weight(mrn_op *op, float2 *kout, const float2 *kin, const mrn::FrameData &fr,

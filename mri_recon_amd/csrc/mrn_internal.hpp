@@ -128,6 +128,8 @@ struct mrn_op {
   bool fft_stockham = false;   // LDS Stockham FFT passes (else four-step where the size allows)
   int xcd_run_fft = 8;         // y and z FFT pass workgroups per XCD run
   int gather_coils = 0;        // coils per lane in the direct gather (0 = auto)
+  int tune_from_cache = 0;     // the last mrn_autotune applied a cached choice
+  float tune_ms = 0.f;         // E'E of the tuned frame with the choice (ms)
   float l1_fraction = 0.0f;
   int l1_levels[3] = {4, 4, 4};
   int l1_spins = 4;
@@ -161,6 +163,57 @@ int fail(mrn_op *op, int code, const char *msg);
   } while (0)
 
 inline hipStream_t pick(mrn_op *, void *s) { return (hipStream_t)s; }  // NULL = null stream
+
+// Owning wrappers, released on every path out of a scope (errors included).
+struct Events {
+  std::vector<hipEvent_t> e;
+  Events() = default;
+  Events(const Events &) = delete;
+  Events &operator=(const Events &) = delete;
+  hipError_t create(int n, unsigned flags = hipEventDefault) {
+    for (int i = 0; i < n; i++) {
+      hipEvent_t x = nullptr;
+      const hipError_t r = hipEventCreateWithFlags(&x, flags);
+      if (r != hipSuccess) return r;
+      e.push_back(x);
+    }
+    return hipSuccess;
+  }
+  hipEvent_t operator[](size_t i) const { return e[i]; }
+  ~Events() {
+    for (hipEvent_t x : e) (void)hipEventDestroy(x);
+  }
+};
+
+// A captured graph and its executable; a launched executable is destroyed
+// only after the stream it runs on has finished it.
+struct GraphGuard {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  hipStream_t launched = nullptr;
+  bool was_launched = false;
+  GraphGuard() = default;
+  GraphGuard(const GraphGuard &) = delete;
+  GraphGuard &operator=(const GraphGuard &) = delete;
+  ~GraphGuard() {
+    if (was_launched) (void)hipStreamSynchronize(launched);
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+// Runs f on scope exit unless released.
+template <class F>
+struct ScopeExit {
+  F f;
+  bool on = true;
+  explicit ScopeExit(F fn) : f(fn) {}
+  ScopeExit(const ScopeExit &) = delete;
+  ~ScopeExit() {
+    if (on) f();
+  }
+  void release() { on = false; }
+};
 
 // ------------------------------------------------------ device helpers
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -275,6 +328,16 @@ struct PrePostArgs {
 };
 
 // host-side launchers (defined in the .hip files)
+int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const FrameData &fr, const float2 *sub,
+                  bool weighted, hipStream_t st);
+int launch_weight(mrn_op *op, float2 *kout, const float2 *kin, const FrameData &fr, hipStream_t st);
+// the scatter: overwrite the tiles it reaches (zeroing the split tiles first),
+// or with accum add into the grid's contents
+int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const FrameData &fr, hipStream_t st,
+                   bool accum = false);
+int launch_scatter_zero(mrn_op *op, float2 *grid, const FrameData &fr, hipStream_t st);
+int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const FrameData &fr, hipStream_t st,
+                         bool accum = false);
 int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 grid3,
               const float2 *tw, hipStream_t st);
 bool fft_x_pruned(const mrn_op *op);

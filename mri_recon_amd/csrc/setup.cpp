@@ -289,6 +289,13 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
     return MRN_EINVAL;
   }
   op->t.B = p.bin_edge > 0 ? p.bin_edge : 1;
+  // scatter tiles are 4 x 4 x 4 cells: one wavefront of cells, and the two
+  // 32-row halves of the matrix-core product
+  for (int a = 0; a < 3; a++)
+    if (p.tile[a] != 0 && p.tile[a] != 4) {
+      delete op;
+      return mrn::free_fail(MRN_EUNSUPPORTED, "scatter tile edge must be 0 (auto) or 4");
+    }
   op->t.T = 4;
   for (int a = 0; a < 3; a++) {
     op->t.nb[a] = (g.S[a] + op->t.B - 1) / op->t.B;

@@ -13,11 +13,17 @@ frame (C, M); trajectory kx, ky, kz, kw float32 (M,) in the reference's units
 (cycles/FOV, |k| <= N/2).
 """
 import ctypes
+import os
 
 import torch
 
 from . import _ffi
 from ._ffi import MrnParams, check, lib
+
+
+# the autotuner's choices, keyed by geometry, sample and coil counts and device
+DEFAULT_TUNE_CACHE = os.environ.get(
+    "MRN_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "mri_recon_amd", "tune.json"))
 
 
 def _ptr(t):
@@ -45,6 +51,19 @@ def _need(t, dtype, shape=None, name="tensor"):
     if shape is not None and tuple(t.shape) != tuple(shape):
         raise ValueError("%s must have shape %s, got %s" % (name, tuple(shape), tuple(t.shape)))
     return t
+
+
+class _DevArray:
+    """A device allocation owned by the op, exposed to torch through the
+    array interface torch reads for HIP memory (__cuda_array_interface__)."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<c8", "data": (ptr, False), "version": 3,
+                                         "strides": None}
+
+
+def _device_buffer(ptr, n):
+    return torch.as_tensor(_DevArray(ptr, n), device="cuda")
 
 
 def _trip(v):
@@ -160,14 +179,23 @@ class GridFFT:
         _need(data, torch.complex64, (self.C, self.M[frame]), "data")
         self._c(lib().mrn_set_kdata(self._h, frame, _ptr(data), _stream()))
 
-    def autotune(self, frame=0):
-        self._c(lib().mrn_autotune(self._h, frame, _stream()))
+    def autotune(self, frame=0, cache=DEFAULT_TUNE_CACHE):
+        """Time the launch shapes on this frame and keep the fastest; the choice
+        is cached in the JSON file `cache` (None: no cache).  Returns tuning()."""
+        path = None
+        if cache:
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            path = os.fsencode(cache)
+        self._c(lib().mrn_autotune(self._h, frame, path, _stream()))
+        return self.tuning()
 
     def tuning(self):
-        """Scatter launch shape in use: {"bin_edge": cells, "max_visits": samples per work item}."""
-        b, v = ctypes.c_int32(), ctypes.c_int32()
-        self._c(lib().mrn_get_tuning(self._h, ctypes.byref(b), ctypes.byref(v)))
-        return {"bin_edge": b.value, "max_visits": v.value}
+        """Launch shapes in use (mrn_tuning as a dict)."""
+        t = _ffi.MrnTuning()
+        self._c(lib().mrn_get_tuning(self._h, ctypes.byref(t)))
+        out = {name: getattr(t, name) for name, _ in t._fields_ if name not in ("reserved", "tile")}
+        out["tile"] = tuple(t.tile)
+        return out
 
     # ------------------------------------------------------------- operator
     def backward(self, X, data=None, frame=0, sub=None):
@@ -336,6 +364,45 @@ class GridFFT:
         out = torch.empty((self.M[frame], 3), dtype=torch.float32, device=X.device)
         self._c(lib().mrn_traj_grad(self._h, frame, _ptr(X), _ptr(g), _ptr(out), _stream()))
         return out
+
+    # ------------------------------------------------ grid-level stages
+    def grid(self):
+        """The op's oversampled grids as a torch view (no copy), shaped
+        (C, Sz, Sy, Sx) for the caller: the reference's k3d_grid per coil.
+        Internally they are stored coil-interleaved; this returns a strided
+        view of that memory, so writes through it reach the op."""
+        ptr, dims = ctypes.c_void_p(), (ctypes.c_int64 * 5)()
+        self._c(lib().mrn_get_grid(self._h, ctypes.byref(ptr), dims))
+        nch, CC, Sx, Sy, Sz = [int(v) for v in dims]
+        n = nch * CC * Sx * Sy * Sz
+        mem = _device_buffer(ptr.value, n)
+        g = mem.view(nch, Sz, Sy, Sx, CC).permute(0, 4, 1, 2, 3).reshape(nch * CC, Sz, Sy, Sx)
+        return g[: self.C] if nch * CC >= self.C else g
+
+    def grid_raw(self):
+        """(nch, Sz, Sy, Sx, CC) view of the grid memory."""
+        ptr, dims = ctypes.c_void_p(), (ctypes.c_int64 * 5)()
+        self._c(lib().mrn_get_grid(self._h, ctypes.byref(ptr), dims))
+        nch, CC, Sx, Sy, Sz = [int(v) for v in dims]
+        return _device_buffer(ptr.value, nch * CC * Sx * Sy * Sz).view(nch, Sz, Sy, Sx, CC)
+
+    def grid_zero(self):
+        self._c(lib().mrn_grid_zero(self._h, _stream()))
+
+    def chop_grid_forward(self, data, frame=0):
+        """grid_c += scatter(kw taper data_c) (gridFFT::chop_grid_forward)."""
+        _need(data, torch.complex64, (self.C, self.M[frame]), "data")
+        self._c(lib().mrn_chop_grid_forward(self._h, frame, _ptr(data), _stream()))
+
+    def chop_grid_backward(self, data=None, frame=0, sub=None):
+        """data_c = gather(grid_c) (gridFFT::chop_grid_backward); returns data."""
+        if data is None:
+            data = torch.empty((self.C, self.M[frame]), dtype=torch.complex64, device="cuda")
+        _need(data, torch.complex64, (self.C, self.M[frame]), "data")
+        if sub is not None:
+            _need(sub, torch.complex64, (self.C, self.M[frame]), "sub")
+        self._c(lib().mrn_chop_grid_backward(self._h, frame, _ptr(data), _ptr(sub), _stream()))
+        return data
 
     def sync(self):
         self._c(lib().mrn_sync(self._h, _stream()))

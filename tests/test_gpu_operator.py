@@ -262,9 +262,11 @@ def test_autotune_keeps_results():
     case = CASES[3]
     op, o32, _, traj, X, smaps, d = make(case)
     before = op.forward(to_dev(d), accumulate=False).cpu().numpy()
-    op.autotune()
+    op.autotune(cache=None)
     after = op.forward(to_dev(d), accumulate=False).cpu().numpy()
-    assert rel(after, before) <= 1e-6
+    # the tuned shapes sum in another order (bins, FFT pass form)
+    assert rel(after, before) <= 1e-5
+    assert rel(after, o32.kspace_to_image(d, smaps, traj)) <= TOL32
 
 
 @pytest.mark.gpu
