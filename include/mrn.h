@@ -192,6 +192,20 @@ int mrn_l1_wavelet(mrn_op *op, void *X, int frames, double fraction, const int32
  * stream from `seed`, each (value >> 33) % 4 (the reference uses rand() % 4,
  * whose sequence depends on every other rand() call of the program). */
 int mrn_set_fista_l1(mrn_op *op, double fraction, const int32_t levels[3], int spins, uint64_t seed);
+/* FISTA / IST step rule (-iterative_step_type, recon_lib.cpp:346-360;
+ * enum values of RECON::IterativeStepType, recon_lib.h:129-130).  MAXEIG:
+ * the fixed step mrn_fista is given (1/max_eig from mrn_power_method).
+ * CAUCHY: at iterations below cauchy_update_number (-cauchy_update_number,
+ * default 30) the step is recomputed as complex<float>(sum|R|^2 / sum
+ * conj(R) E'E R) with R the gradient of all frames, both sums complex
+ * double (recon_lib.cpp:2013-2041, 2093-2100); later iterations keep the
+ * last one; mrn_fista's step argument is used only before the first update. */
+enum { MRN_STEP_CAUCHY = 0, MRN_STEP_MAXEIG = 1 };
+int mrn_set_fista_step(mrn_op *op, int step_type, int cauchy_update_number);
+/* Host copy of the complex step (re, im; 2 doubles per iteration) of the
+ * first n iterations of the last mrn_fista with the Cauchy rule. */
+int mrn_get_fista_steps(mrn_op *op, double *out, int n, void *stream);
+
 /* Host copy of the threshold the last mrn_fista applied at each of its first
  * n iterations. */
 int mrn_get_fista_thresholds(mrn_op *op, double *out, int n, void *stream);

@@ -23,7 +23,7 @@ EXPORTS = [
     "mrn_coil_forward", "mrn_set_kdata", "mrn_cg", "mrn_cg_init", "mrn_cg_iterate",
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
     "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
-    "mrn_get_fista_thresholds", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
+    "mrn_get_fista_thresholds", "mrn_set_fista_step", "mrn_get_fista_steps", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
     "mrn_dcf_calc", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
     "mrn_get_fista_llr_smax", "mrn_gate_params_default", "mrn_gate_parse_args", "mrn_gate_scale_times",
     "mrn_pregate",
@@ -163,6 +163,8 @@ def lib():
             "mrn_l1_wavelet": (i32, [vp, vp, i32, ctypes.c_double, vp, i32, vp, vp, vp]),
             "mrn_set_fista_l1": (i32, [vp, ctypes.c_double, vp, i32, ctypes.c_uint64]),
             "mrn_get_fista_thresholds": (i32, [vp, vp, i32, vp]),
+            "mrn_set_fista_step": (i32, [vp, i32, i32]),
+            "mrn_get_fista_steps": (i32, [vp, vp, i32, vp]),
             "mrn_set_kdata": (i32, [vp, i32, vp, vp]),
             "mrn_cg": (i32, [vp, vp, i32, i32, vp, vp]),
             "mrn_cg_init": (i32, [vp, vp, vp]),

@@ -37,7 +37,9 @@ int op_cg_alpha(mrn_op *, const float2 *, const float2 *, const float2 *, int64_
 int op_cg_update(mrn_op *, float2 *, float2 *, float2 *, const float2 *, int64_t, double *, hipStream_t);
 int op_power_step(mrn_op *, float2 *, const float2 *, int64_t, double *, hipStream_t);
 int op_fista_momentum(mrn_op *, float2 *, float2 *, int64_t, hipStream_t);
-int op_fista_finish(mrn_op *, float2 *, float2 *, int64_t, float, double *, hipStream_t);
+int op_fista_finish(mrn_op *, float2 *, float2 *, int64_t, float, double *, hipStream_t, bool);
+int op_cauchy_step(mrn_op *, const float2 *, const float2 *, int64_t, hipStream_t);
+int op_set_step(mrn_op *, float, float, hipStream_t);
 int traj_grad_impl(mrn_op *, int, const float2 *, const float2 *, float *, hipStream_t);
 int autotune_impl(mrn_op *, int, const char *, hipStream_t);
 
@@ -326,6 +328,7 @@ static int ensure_solver(mrn_op *op) {
     MRN_HIP(op, hipMalloc(&op->dtrace, sizeof(double) * 4 * MRN_TRACE_CAP));
     op->dtrace_cap = MRN_TRACE_CAP;
   }
+  if (!op->dstep_log) MRN_HIP(op, hipMalloc(&op->dstep_log, sizeof(double) * 2 * MRN_TRACE_CAP));
   return MRN_OK;
 }
 
@@ -654,13 +657,20 @@ int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, doubl
   MRN_TRY(op_zero(op, op->vP, n, st));  // X_old
   const float stf = (float)step;
   const bool l1 = op->l1_fraction > 0.0f;
+  const bool cauchy = op->step_type == MRN_STEP_CAUCHY;
   if (l1) MRN_TRY(ensure_l1_state(op));  // before any capture
   MRN_TRY(llr_fista_prepare(op));
+  if (cauchy) MRN_TRY(op_set_step(op, stf, 0.0f, st));  // the step until the first Cauchy update
   auto body = [&](hipStream_t s) -> int {
     MRN_TRY(op_fista_momentum(op, X, op->vP, n, s));
     for (int f = 0; f < op->F; f++)
       MRN_TRY(grad_frame(op, f, X + f * op->nvox, op->vR + f * op->nvox, s));
-    MRN_TRY(op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s));
+    if (cauchy) {  // P = E'E R per frame, step = sum|R|^2 / sum conj(R) P (recon_lib.cpp:2013-2041)
+      for (int f = 0; f < op->F; f++)
+        MRN_TRY(normal_frame(op, f, op->vR + f * op->nvox, op->vL + f * op->nvox, s));
+      MRN_TRY(op_cauchy_step(op, op->vR, op->vL, n, s));
+    }
+    MRN_TRY(op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s, cauchy));
     // locally low-rank threshold over the frames (recon_lib.cpp:2117-2121)
     if (op->llr_on) MRN_TRY(llr_fista_step(op, X, s));
     // L1 wavelet soft threshold after the gradient step (recon_lib.cpp:2145-2150)
@@ -706,6 +716,26 @@ int mrn_set_fista_l1(mrn_op *op, double fraction, const int32_t levels[3], int s
   op->l1_spins = spins;
   op->l1_seed = seed;
   op->version++;
+  return MRN_OK;
+}
+
+int mrn_set_fista_step(mrn_op *op, int step_type, int cauchy_update_number) {
+  if (!op) return MRN_EINVAL;
+  if (step_type != MRN_STEP_CAUCHY && step_type != MRN_STEP_MAXEIG)
+    return fail(op, MRN_EINVAL, "step type is MRN_STEP_CAUCHY or MRN_STEP_MAXEIG");
+  if (cauchy_update_number < 0) return fail(op, MRN_EINVAL, "cauchy_update_number < 0");
+  op->step_type = step_type;
+  op->cauchy_updates = cauchy_update_number;
+  op->version++;
+  return MRN_OK;
+}
+
+int mrn_get_fista_steps(mrn_op *op, double *out, int n, void *stream) {
+  if (!op || !out || n < 0 || n > MRN_TRACE_CAP) return MRN_EINVAL;
+  if (!op->dstep_log) return fail(op, MRN_ESTATE, "no FISTA run yet");
+  hipStream_t st = pick(op, stream);
+  MRN_HIP(op, hipMemcpyAsync(out, op->dstep_log, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+  MRN_HIP(op, hipStreamSynchronize(st));
   return MRN_OK;
 }
 

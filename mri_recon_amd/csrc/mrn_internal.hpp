@@ -128,6 +128,9 @@ struct mrn_op {
   bool fft_stockham = false;   // LDS Stockham FFT passes (else four-step where the size allows)
   int xcd_run_fft = 8;         // y and z FFT pass workgroups per XCD run
   int gather_coils = 0;        // coils per lane in the direct gather (0 = auto)
+  int step_type = MRN_STEP_MAXEIG;  // FISTA step: fixed (power method) or Cauchy
+  int cauchy_updates = 30;     // iterations that recompute the Cauchy step
+  double *dstep_log = nullptr; // Cauchy step per iteration (re, im)
   int tune_from_cache = 0;     // the last mrn_autotune applied a cached choice
   float tune_ms = 0.f;         // E'E of the tuned frame with the choice (ms)
   float l1_fraction = 0.0f;

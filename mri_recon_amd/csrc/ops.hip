@@ -16,7 +16,7 @@ namespace mrn {
 // device scalar slots (double dscal[...])
 enum { D_RR = 0, D_PR = 1, D_PI = 2, D_RN = 3, D_EIG = 4 };
 // float slots after the doubles: float* f = (float*)(dscal + 8)
-enum { F_AR = 0, F_AI = 1, F_BETA = 2, F_SCALE = 3, F_STEP = 4 };
+enum { F_AR = 0, F_AI = 1, F_BETA = 2, F_SCALE = 3, F_STEP = 4, F_SR = 5, F_SI = 6 };
 
 __global__ void k_zero2(float2 *p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -322,6 +322,57 @@ __global__ void k_fista_step(float2 *X, float2 *R, int64_t n, float step) {
   }
 }
 
+// Cauchy step of the gradient iteration (recon_lib.cpp:2013-2041,
+// 2093-2100): while the iteration is below update_number, step =
+// complex<float>(sum|R|^2 / sum conj(R) E'E R) with both sums in complex
+// double; afterwards the last step stays.  Logged per iteration.
+__global__ __launch_bounds__(RED_THREADS) void k_cauchy_step(const double *part, int nb, double *dscal,
+                                                             const int *counter, int update_number,
+                                                             double *log) {
+  __shared__ double sh[RED_THREADS];
+  double rr = 0, pr = 0, pi = 0;
+  for (int b = threadIdx.x; b < nb; b += RED_THREADS) {
+    rr += part[3 * b];
+    pr += part[3 * b + 1];
+    pi += part[3 * b + 2];
+  }
+  rr = block_sum(rr, sh);
+  pr = block_sum(pr, sh);
+  pi = block_sum(pi, sh);
+  if (threadIdx.x == 0) {
+    const int it = *counter;
+    float *f = (float *)(dscal + 8);
+    if (it < update_number) {
+      const double den = pr * pr + pi * pi;
+      f[F_SR] = (float)(rr * pr / den);
+      f[F_SI] = (float)(-rr * pi / den);
+    }
+    if (log) {
+      log[2 * (it % MRN_TRACE_CAP)] = f[F_SR];
+      log[2 * (it % MRN_TRACE_CAP) + 1] = f[F_SI];
+    }
+  }
+}
+
+// R *= step (complex<float>, from the device) ; X -= R
+__global__ void k_fista_step_c(float2 *X, float2 *R, int64_t n, const double *dscal) {
+  const float *f = (const float *)(dscal + 8);
+  const float sr = f[F_SR], si = f[F_SI];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float2 r = R[i], x = X[i];
+    r = make_float2(r.x * sr - r.y * si, r.x * si + r.y * sr);
+    R[i] = r;
+    X[i] = make_float2(x.x - r.x, x.y - r.y);
+  }
+}
+
+__global__ void k_set_step(double *dscal, float sr, float si) {
+  float *f = (float *)(dscal + 8);
+  f[F_SR] = sr;
+  f[F_SI] = si;
+}
+
 // ------------------------------------------------------------ launchers
 static inline unsigned gs(int64_t n) {
   int64_t b = (n + 255) / 256;
@@ -425,13 +476,31 @@ int op_fista_momentum(mrn_op *op, float2 *X, float2 *Xold, int64_t n, hipStream_
   LAUNCH_CHECK(op);
   return MRN_OK;
 }
+// device_step: the complex step in the device scalars (Cauchy), else `step`
 int op_fista_finish(mrn_op *op, float2 *X, float2 *R, int64_t n, float step, double *trace,
-                    hipStream_t st) {
+                    hipStream_t st, bool device_step) {
   const unsigned nb = rb(n);
   hipLaunchKernelGGL(k_energy, dim3(nb), dim3(RED_THREADS), 0, st, R, n, op->dpart);
   hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(RED_THREADS), 0, st, op->dpart, (int)nb, trace,
                      op->dcounter);
-  hipLaunchKernelGGL(k_fista_step, dim3(gs(n)), dim3(256), 0, st, X, R, n, step);
+  if (device_step)
+    hipLaunchKernelGGL(k_fista_step_c, dim3(gs(n)), dim3(256), 0, st, X, R, n, op->dscal);
+  else
+    hipLaunchKernelGGL(k_fista_step, dim3(gs(n)), dim3(256), 0, st, X, R, n, step);
+  LAUNCH_CHECK(op);
+  return MRN_OK;
+}
+// step from sum|R|^2 and sum conj(R) L, L = E'E R (before op_fista_finish)
+int op_cauchy_step(mrn_op *op, const float2 *R, const float2 *L, int64_t n, hipStream_t st) {
+  const unsigned nb = rb(n);
+  hipLaunchKernelGGL(k_dot_rr_pl, dim3(nb), dim3(RED_THREADS), 0, st, R, R, L, n, op->dpart);
+  hipLaunchKernelGGL(k_cauchy_step, dim3(1), dim3(RED_THREADS), 0, st, op->dpart, (int)nb, op->dscal,
+                     op->dcounter, op->cauchy_updates, op->dstep_log);
+  LAUNCH_CHECK(op);
+  return MRN_OK;
+}
+int op_set_step(mrn_op *op, float sr, float si, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, st, op->dscal, sr, si);
   LAUNCH_CHECK(op);
   return MRN_OK;
 }

@@ -422,6 +422,7 @@ int mrn_op_destroy(mrn_op *op) {
   hipFree(op->dpart);
   hipFree(op->dscal);
   hipFree(op->dtrace);
+  hipFree(op->dstep_log);
   hipFree(op->dcounter);
   hipFree(op->dthr);
   hipFree(op->dsh);

@@ -351,6 +351,20 @@ class GridFFT:
         self._c(lib().mrn_get_fista_llr_smax(self._h, ctypes.byref(out), _stream()))
         return out.value
 
+    def set_fista_step(self, step_type="maxeig", cauchy_update_number=30):
+        """FISTA step rule: "maxeig" (the step fista() is given) or "cauchy"
+        (recomputed from the gradient for the first cauchy_update_number
+        iterations, recon_lib.cpp:2013-2041)."""
+        t = {"cauchy": 0, "maxeig": 1}[step_type]
+        self._c(lib().mrn_set_fista_step(self._h, t, int(cauchy_update_number)))
+
+    def fista_steps(self, n):
+        """Complex step of each of the first n iterations of the last Cauchy FISTA."""
+        out = (ctypes.c_double * max(2 * n, 2))()
+        self._c(lib().mrn_get_fista_steps(self._h, out, int(n), _stream()))
+        v = list(out)[: 2 * n]
+        return torch.tensor([complex(v[2 * i], v[2 * i + 1]) for i in range(n)], dtype=torch.complex128)
+
     def fista_thresholds(self, n):
         out = (ctypes.c_double * max(n, 1))()
         self._c(lib().mrn_get_fista_thresholds(self._h, out, int(n), _stream()))

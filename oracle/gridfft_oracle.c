@@ -866,6 +866,70 @@ void orc_fista(orc_op *op, real *Xo, const real *smaps, int C, const real *data,
   free(diff);
 }
 
+/* FISTA with the Cauchy step (recon_lib.cpp:2013-2041, 2093-2100): for
+ * it < update_number, P = sum_c A_c^H W A_c R, step = complex<float>(RhR /
+ * RhP), RhR = sum |R|^2, RhP = sum conj(R) P (complex double); then R *=
+ * step (complex float), X -= R.  `step` is the value before the first
+ * update; steps (nullable) receives re, im per iteration. */
+void orc_fista_cauchy(orc_op *op, real *Xo, const real *smaps, int C, const real *data, const float *kx,
+                      const float *ky, const float *kz, const float *kw, long M, int iters, double step,
+                      int update_number, double *resid, double *steps) {
+  long nv = nvox(op);
+  cpx *X = (cpx *)Xo;
+  cpx *Xold = (cpx *)calloc((size_t)nv, sizeof(cpx));
+  cpx *R = (cpx *)malloc(sizeof(cpx) * (size_t)nv);
+  cpx *P = (cpx *)malloc(sizeof(cpx) * (size_t)nv);
+  real *diff = (real *)malloc(sizeof(cpx) * (size_t)M);
+  float sr = (float)step, si = 0.0f;
+  memset(X, 0, sizeof(cpx) * (size_t)nv);
+  for (int it = 0; it < iters; it++) {
+    memset(R, 0, sizeof(cpx) * (size_t)nv);
+    real A = (real)(1.0 + (it - 1.0) / (it + 1.0));
+    real B = (real)(-(it - 1.0) / (it + 1.0));
+    for (long q = 0; q < nv; q++) {
+      cpx x0 = Xold[q], x1 = X[q];
+      X[q].re = A * x1.re + B * x0.re;
+      X[q].im = A * x1.im + B * x0.im;
+      Xold[q] = x1;
+    }
+    for (int c = 0; c < C; c++) {
+      orc_backward(op, (const real *)X, smaps ? smaps + 2 * nv * c : NULL, diff, kx, ky, kz, kw, M,
+                   data + 2 * M * c);
+      orc_forward(op, (real *)R, smaps ? smaps + 2 * nv * c : NULL, diff, kx, ky, kz, kw, M);
+    }
+    double rr = energy(R, nv);
+    if (it < update_number) {
+      memset(P, 0, sizeof(cpx) * (size_t)nv);
+      for (int c = 0; c < C; c++) {
+        orc_backward(op, (const real *)R, smaps ? smaps + 2 * nv * c : NULL, diff, kx, ky, kz, kw, M, NULL);
+        orc_forward(op, (real *)P, smaps ? smaps + 2 * nv * c : NULL, diff, kx, ky, kz, kw, M);
+      }
+      double pr = 0, pi = 0;
+      for (long q = 0; q < nv; q++) {  /* P *= conj(R), summed */
+        pr += (double)(P[q].re * R[q].re + P[q].im * R[q].im);
+        pi += (double)(P[q].im * R[q].re - P[q].re * R[q].im);
+      }
+      const double den = pr * pr + pi * pi;
+      sr = (float)(rr * pr / den);
+      si = (float)(-rr * pi / den);
+    }
+    if (resid) resid[it] = rr;
+    if (steps) {
+      steps[2 * it] = sr;
+      steps[2 * it + 1] = si;
+    }
+    for (long q = 0; q < nv; q++) {
+      real r0 = R[q].re * (real)sr - R[q].im * (real)si, r1 = R[q].re * (real)si + R[q].im * (real)sr;
+      X[q].re -= r0;
+      X[q].im -= r1;
+    }
+  }
+  free(Xold);
+  free(R);
+  free(P);
+  free(diff);
+}
+
 int orc_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -163,6 +163,8 @@ def _lib(precision):
         lib.orc_power_method.restype = dbl
         lib.orc_fista.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int,
                                   dbl, vp]
+        lib.orc_fista_cauchy.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int,
+                                         dbl, ctypes.c_int, vp, vp]
         lib.orc_threads.restype = ctypes.c_int
         lib.orc_dcf_table_len.argtypes = [ctypes.c_float]
         lib.orc_dcf_table_len.restype = ctypes.c_int
@@ -638,6 +640,19 @@ class OracleOp:
                               kx.shape[0], iters, float(step), float(fraction), lv.ctypes.data,
                               int(spins), sh.ctypes.data, resid.ctypes.data, thr.ctypes.data)
         return X, resid, thr
+
+    def fista_cauchy(self, smaps, data, traj, iters, step=0.0, update_number=30):
+        """FISTA with the Cauchy step; returns (X, resid, steps (complex))."""
+        kx, ky, kz, kw = self._traj(traj)
+        data, smaps = self._c(data), self._c(smaps)
+        X = np.zeros((self.N[2], self.N[1], self.N[0]), self.ctype)
+        resid = np.zeros(iters)
+        steps = np.zeros(2 * iters)
+        self.lib.orc_fista_cauchy(self.h, X.ctypes.data, _p(smaps), data.shape[0], data.ctypes.data,
+                                  kx.ctypes.data, ky.ctypes.data, kz.ctypes.data, kw.ctypes.data,
+                                  kx.shape[0], iters, float(step), int(update_number), resid.ctypes.data,
+                                  steps.ctypes.data)
+        return X, resid, steps[0::2] + 1j * steps[1::2]
 
     def fista(self, smaps, data, traj, iters, step):
         kx, ky, kz, kw = self._traj(traj)

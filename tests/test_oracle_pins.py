@@ -428,3 +428,33 @@ def test_forward_sos_is_squared_no_map_forward():
     X0 = crandn(N, N, N, rng=rng)
     Xs = op.forward_sos(X0.copy(), d, traj)
     assert np.allclose(Xs - X0, np.abs(Xf) ** 2, rtol=1e-12, atol=1e-12 * np.max(np.abs(Xf) ** 2))
+
+
+def test_cauchy_step_is_the_exact_line_search():
+    """The Cauchy step of the gradient iteration (recon_lib.cpp:2013-2041)
+    minimises f(x - a R) = 1/2 sum kw |A(x - a R) - d|^2 along the gradient
+    R = A^H kw (A x - d) (taper off): f at the oracle's step lies below f at
+    the step scaled by 1 +- 1e-3 and the parabola through the three values
+    has its vertex there; the first iteration starts from x = 0."""
+    import math
+    from tests.helpers import radial_problem
+    N, C = 12, 2
+    traj, X, smaps, d = radial_problem(N, C, 60, 12, seed=3)
+    o = orc.OracleOp(N, overgrid=1.5, precision="f64", k_rad=math.inf)
+    sm, dd = smaps.astype(np.complex128), d.astype(np.complex128)
+    X1, _, steps = o.fista_cauchy(sm, dd, traj, 1, 0.0, 5)
+    a = steps[0]
+    kw = traj[3].astype(np.float64)
+    R = -X1 / a  # x1 = 0 - a R
+
+    def f(alpha):
+        r = o.image_to_kspace(-alpha * R, sm, traj) - dd
+        return 0.5 * float(np.sum(kw * np.abs(r) ** 2))
+
+    f0, fp, fm = f(a), f(a * (1 + 1e-3)), f(a * (1 - 1e-3))
+    assert f0 < fp and f0 < fm
+    # vertex of the parabola through (1-e, 1, 1+e): at 1 within rounding
+    e = 1e-3
+    vertex = 1 - e * (fp - fm) / (2 * (fp - 2 * f0 + fm))
+    assert abs(vertex - 1) < 1e-6
+    assert abs(a.imag) < 1e-6 * abs(a.real)  # R^H A^H W A R is real for a Hermitian E'WE
