@@ -774,7 +774,7 @@ int mrn_get_tuning(const mrn_op *op, mrn_tuning *out) {
   out->gather_coils = op->gather_coils > 0 ? std::min(op->gather_coils, op->g.CC) : std::min(4, op->g.CC);
   out->xcd_run_fft = op->xcd_run_fft;
   out->fft_stockham = op->fft_stockham ? 1 : 0;
-  out->scatter_valu = (op->scatter_valu || op->g.CC != 16) ? 1 : 0;
+  out->scatter_valu = (op->scatter_valu || (op->g.CC != 16 && op->g.CC != 8)) ? 1 : 0;
   for (int a = 0; a < 3; a++) out->tile[a] = op->t.T;
   out->from_cache = op->tune_from_cache;
   out->normal_ms = op->tune_ms;

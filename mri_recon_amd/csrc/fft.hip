@@ -812,7 +812,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_dft_lines(float2 *__restrict__ 
 // The batched x passes run the four-step kernel with the frame's k-space
 // support (forward: stores, inverse: loads limited to the support).
 bool fft_x_pruned(const mrn_op *op) {
-  return op->p.form != MRN_FORM_PLAIN && op->g.CC >= FFT_LINES && !op->fft_stockham && fft4_ok(op->g.S[0]) &&
+  return op->p.form != MRN_FORM_PLAIN && op->g.CC >= FFT_LINES / 2 && !op->fft_stockham && fft4_ok(op->g.S[0]) &&
          op->g.fft_in[0];
 }
 
@@ -833,10 +833,10 @@ bool fft_size_supported(int S) {
 int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 g3,
               const float2 *tw, hipStream_t st) {
   if (g3.x == 0 || g3.y == 0 || g3.z == 0) return MRN_OK;
-  // four-step kernel where the 16 lines of a workgroup share each point's
-  // 128-byte row (runs of at least 16 lines); the others keep k_fft_lines'
-  // layouts, which coalesce short runs
-  const bool four = a.run >= FFT_LINES && !op->fft_stockham;
+  // four-step kernel where the 16 lines of a workgroup read each point as
+  // 64-byte or longer runs (8 coils of two rows or 16 of one); the others
+  // keep k_fft_lines' layouts, which coalesce short runs
+  const bool four = a.run >= FFT_LINES / 2 && !op->fft_stockham;
   switch (S) {
 #define MRN_CASE_LINES(n)                                                                    \
   case n:                                                                                    \

@@ -98,9 +98,12 @@ __global__ void k_taper(const float *__restrict__ traw, int64_t M, const int32_t
 }
 
 // Bins (per axis) whose samples can have a tap in tile `ti`: first taps in
-// [ti*T - NT + 1, ti*T + T - 1].
+// [ti*T - NTb + 1, ti*T + T - 1].  NTb is the usual tap count (2 dwin when
+// that is an integer); a sample with more taps on some axis (its coordinate
+// exactly on the window edge) can reach a tile outside this range and is
+// scattered on its own by k_scatter_wide.
 __host__ __device__ __forceinline__ void tile_bins(const Tiles &t, int a, int ti, int &b0, int &b1) {
-  int lo = ti * t.T - t.NT + 1;
+  int lo = ti * t.T - t.NTb + 1;
   lo = lo < 0 ? 0 : lo;
   const int hi = ti * t.T + t.T - 1;
   b0 = lo / t.B;
@@ -541,6 +544,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 typedef float mrn_v16f __attribute__((ext_vector_type(16)));
+typedef float mrn_v4f __attribute__((ext_vector_type(4)));
 
 // Tile-staged gather on the matrix cores (16-coil blocks).  A workgroup
 // stages the cells that the home samples of work item gsubs[blockIdx.x] can
@@ -867,11 +871,12 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
   if (lane == 0) s_rp[wave][0] = 0;
   wave_sync();
   const int vbeg = max(sp.y, 0), vend = min(sp.z, carry);
-  static_assert(!MM || CCT == 16, "matrix-core scatter needs 16 coils per cell");
+  static_assert(!MM || CCT == 16 || CCT == 8, "matrix-core scatter needs 8 or 16 coils per cell");
   float ar[MM ? 1 : CCT], ai[MM ? 1 : CCT];
 #pragma unroll
   for (int c = 0; c < (MM ? 1 : CCT); c++) ar[c] = ai[c] = 0.f;
-  mrn_v16f acc0 = {}, acc1 = {};  // MM: cells 0-31 and 32-63
+  mrn_v16f acc0 = {}, acc1 = {};  // MM, 16 coils: cells 0-31 and 32-63
+  mrn_v4f a8[4] = {};             // MM, 8 coils: cells 16 b .. 16 b + 15
   const int sl = lane >> 2, q = lane & 3;
   constexpr int CPL = CCT >= 4 ? CCT / 4 : 1;  // coil values a lane stages
   // internal index of visit v of this work item (0 past the end: a valid
@@ -907,12 +912,20 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     const float dk[3] = {k.x, k.y, k.z};
     unsigned long long hit = ~0ull;
     float wa[3];
+    int sta[3], ena[3];
+    bool oka[3], wide = false;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      int st = 0, en = -1;
+      sta[a] = 0;
+      ena[a] = -1;
+      oka[a] = ok && axis_range(g, a, dk[a], sta[a], ena[a]);
+      // more taps than the tiles' bin reach assumes: scattered by k_scatter_wide
+      wide = wide || ena[a] - sta[a] + 1 > t.NTb;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
       const int l = t0[a] + q;
-      bool in = ok && axis_range(g, a, dk[a], st, en);
-      in = in && l >= st && l <= en;
+      const bool in = oka[a] && !wide && l >= sta[a] && l <= ena[a];
       wa[a] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
       s_w[wave][sl * 12 + a * 4 + q] = wa[a];
       unsigned long long m = __ballot(in);
@@ -937,7 +950,26 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     if (MM && base + SG_BATCH < vend) load_batch(visit_index(base + SG_BATCH + sl));
     wave_sync();
     // ---- B: accumulate the samples that reach the tile
-    if constexpr (MM) {
+    if constexpr (MM && CCT == 8) {
+      // T[64 cells][16 reals] += W[64][16] D[16][16] as four
+      // v_mfma_f32_16x16x4f32 per 4 samples (cells 16 b .. 16 b + 15): lane
+      // i + 16 k holds A = W[cell 16 b + i][sample k] and B = D[sample k][real i]
+      const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
+      const int col = lane & 15, kq = lane >> 4;
+      if (hit)
+#pragma unroll
+        for (int kt = 0; kt < SG_BATCH / 4; kt++) {
+          const int smp = 4 * kt + kq;
+          const float b = dflat[smp * 2 * CCT + col];
+          const float wx = s_w[wave][smp * 12 + (col & 3)];
+#pragma unroll
+          for (int cb = 0; cb < 4; cb++) {
+            const float a = s_yz[wave][smp * 16 + 4 * cb + (col >> 2)] * wx;
+            a8[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, a8[cb], 0, 0, 0);
+          }
+        }
+      hit = 0;
+    } else if constexpr (MM) {
       const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
       const int c0 = lane & 31, c1 = c0 + 32;
       // no per-pair skip: straight-line code (a sample off the tile has
@@ -974,7 +1006,30 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     }
     wave_sync();
   }
-  if constexpr (MM) {
+  if constexpr (MM && CCT == 8) {
+    // a8[b] register r of lane l: cell 16 b + 4 (l / 16) + r, real number l % 16
+    // of the cell's 8 coils
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int cell = 16 * cb + 4 * (lane >> 4) + r;
+        const int cx = t0[0] + (cell & 3), cy = t0[1] + ((cell >> 2) & 3), cz = t0[2] + (cell >> 4);
+        if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) continue;
+        float *dst = reinterpret_cast<float *>(grid + ch * chunk_grid +
+                                               (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT) +
+                     (lane & 15);
+        const float v = a8[cb][r];
+        if (sp.w) {
+          if (v != 0.f) atomicAdd(dst, v);
+        } else if (accum) {
+          *dst += v;
+        } else {
+          *dst = v;
+        }
+      }
+    return;
+  } else if constexpr (MM) {
     // accumulator register r of lane l: cell 8*(r/4) + 4*(l/32) + r%4 (+32 for
     // acc1), real number l%32 of the cell's 16 coils
 #pragma unroll
@@ -1017,6 +1072,60 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
 #pragma unroll
     for (int c = 0; c < CCT; c++) dst[c] = make_float2(ar[c], ai[c]);
   }
+}
+
+// The samples with more than NTb taps on some axis (fr.wide), which the tile
+// scatter leaves out: every tap added with float atomics after the tiles are
+// written.  Thread = (listed sample, coil) of coil block blockIdx.y.  Such
+// samples sit exactly on a window edge, so the list is short (often empty).
+__global__ void k_scatter_wide(float2 *grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
+                               const int32_t *__restrict__ wide, int nwide, Geom g,
+                               const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nwide * g.CC) return;
+  const int c = (int)(idx & (g.CC - 1));
+  const int64_t i = wide[idx >> g.lgCC];
+  const int ch = blockIdx.y;
+  const float4 k = coords[i];
+  const float dk[3] = {k.x, k.y, k.z};
+  int s[3], e[3];
+  for (int a = 0; a < 3; a++)
+    if (!axis_range(g, a, dk[a], s[a], e[a])) return;
+  const float2 v = kin[ch * chunk_k + i * g.CC + c];
+  float2 *G = grid + ch * chunk_grid + c;
+  for (int lz = s[2]; lz <= e[2]; lz++) {
+    const float wz = tap_weight(g, 2, tables + g.toff[2], dk[2], lz);
+    for (int ly = s[1]; ly <= e[1]; ly++) {
+      const float wzy = wz * tap_weight(g, 1, tables + g.toff[1], dk[1], ly);
+      float2 *row = G + ((int64_t)lz * g.S[1] + ly) * g.S[0] * g.CC;
+      for (int lx = s[0]; lx <= e[0]; lx++) {
+        const float w = wzy * tap_weight(g, 0, tables + g.toff[0], dk[0], lx);
+        atomicAdd(&row[(int64_t)lx * g.CC].x, w * v.x);
+        atomicAdd(&row[(int64_t)lx * g.CC].y, w * v.y);
+      }
+    }
+  }
+}
+
+// wide[i] = 1 when valid sample i has more than NTb taps on some axis
+__global__ void k_wide_flag(const float4 *__restrict__ coords, int64_t Mv, Geom g, int NTb, int32_t *flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Mv) return;
+  const float4 k = coords[i];
+  const float dk[3] = {k.x, k.y, k.z};
+  bool w = false;
+  for (int a = 0; a < 3; a++) {
+    int s, e;
+    axis_range(g, a, dk[a], s, e);
+    w = w || e - s + 1 > NTb;
+  }
+  flag[i] = w ? 1 : 0;
+}
+
+__global__ void k_compact64(const int32_t *__restrict__ flag, const int32_t *__restrict__ off, int64_t n,
+                            int32_t *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) out[off[i]] = (int32_t)i;
 }
 
 // kout = (kin * kw) * taper for the valid samples of one frame, every block
@@ -1264,7 +1373,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   dim3 g3((fr.nsub + 3) / 4, op->nch);
   const size_t sh = sizeof(float) * g.ntab;  // kernel tables in LDS
-  const int rows_axis = (t.T + t.NT - 2) / t.B + 2;  // bins per axis a tile can reach
+  const int rows_axis = (t.T + t.NTb - 2) / t.B + 2;  // bins per axis a tile can reach
   if (rows_axis * rows_axis > SG_MAXROWS) return fail(op, MRN_EUNSUPPORTED, "scatter neighbourhood too large");
   // no work item: a frame without valid samples whose x passes are limited to
   // its (empty) k-space support -- nothing reads the grid
@@ -1272,10 +1381,11 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   if (g.CC == 16 && !op->scatter_valu) {
     hipLaunchKernelGGL((k_scatter_gd<16, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
                        fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);
-    MRN_HIP(op, hipGetLastError());
-    return MRN_OK;
-  }
-  switch (g.CC) {
+  } else if (g.CC == 8 && !op->scatter_valu) {
+    hipLaunchKernelGGL((k_scatter_gd<8, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
+                       fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);
+  } else {
+    switch (g.CC) {
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
     hipLaunchKernelGGL((k_scatter_gd<n, false>), g3, dim3(256), sh, st, grid, kin, fr.coords,    \
@@ -1283,10 +1393,16 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
     break;
     MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)
 #undef MRN_S
-    default:
-      return fail(op, MRN_EUNSUPPORTED, "coil block size");
+      default:
+        return fail(op, MRN_EUNSUPPORTED, "coil block size");
+    }
   }
   MRN_HIP(op, hipGetLastError());
+  if (fr.nwide > 0) {  // after the tiles' plain stores
+    hipLaunchKernelGGL(k_scatter_wide, dim3(nblk((int64_t)fr.nwide * g.CC, 256), op->nch), dim3(256), 0, st, grid,
+                       kin, fr.coords, fr.wide, fr.nwide, g, op->tables_d, cg, ck);
+    MRN_HIP(op, hipGetLastError());
+  }
   return MRN_OK;
 }
 
@@ -1414,10 +1530,14 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   int32_t *vals = nullptr, *counts = nullptr, *nsub = nullptr, *visits = nullptr, *off = nullptr,
           *hflag = nullptr, *hoff = nullptr;
   int2 *brow = nullptr;
-  void *tmp = nullptr;
+  int32_t *wflag = nullptr, *woff = nullptr;
+  void *tmp = nullptr, *wtmp = nullptr;
   size_t tmp_bytes = 0, b2 = 0;
   int rc = MRN_OK;
   auto cleanup = [&]() {
+    hipFree(wflag);
+    hipFree(woff);
+    hipFree(wtmp);
     hipFree(keys);
     hipFree(keys2);
     hipFree(vals);
@@ -1450,6 +1570,9 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipFree(fr.sparse);
   hipFree(fr.xrng);
   hipFree(fr.yrng);
+  hipFree(fr.wide);
+  fr.wide = nullptr;
+  fr.nwide = 0;
   fr.gsubs = nullptr;
   fr.sparse = nullptr;
   fr.xrng = nullptr;
@@ -1508,6 +1631,29 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   if (fr.Mv > 0) {
     hipLaunchKernelGGL(k_traj_finish, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, kx, ky, kz, kw,
                        fr.perm, fr.Mv, g, fr.coords, fr.taper, fr.inv);
+  }
+  // samples with more taps than the tiles' bin reach assumes (only possible
+  // when 2 dwin is an integer on some gridded axis)
+  if (t.NTb < t.NT && fr.Mv > 0) {
+    const int64_t n = fr.Mv + 1;
+    MRN_BH(hipMalloc(&wflag, sizeof(int32_t) * n));
+    MRN_BH(hipMalloc(&woff, sizeof(int32_t) * n));
+    MRN_BH(hipMemsetAsync(wflag, 0, sizeof(int32_t) * n, st));
+    hipLaunchKernelGGL(k_wide_flag, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, fr.coords, fr.Mv, g, t.NTb, wflag);
+    MRN_BH(hipGetLastError());
+    size_t wb = 0;
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, wb, wflag, woff, (int)n, st));
+    MRN_BH(hipMalloc(&wtmp, wb > 0 ? wb : 16));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(wtmp, wb, wflag, woff, (int)n, st));
+    int32_t nw = 0;
+    MRN_BH(hipMemcpyAsync(&nw, woff + fr.Mv, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    MRN_BH(hipStreamSynchronize(st));
+    fr.nwide = nw;
+    if (nw > 0) {
+      MRN_BH(hipMalloc(&fr.wide, sizeof(int32_t) * nw));
+      hipLaunchKernelGGL(k_compact64, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, wflag, woff, fr.Mv, fr.wide);
+      MRN_BH(hipGetLastError());
+    }
   }
   // k-space support per grid row, for the x passes
   {

@@ -55,6 +55,7 @@ struct Tiles {
   int nt[3];    // tiles per axis
   int ntiles;
   int NT;       // max taps per axis
+  int NTb;      // taps per axis that set a tile's bin reach (samples with more: FrameData::wide)
   int maxv;     // samples per scatter work item
   int xrun;     // scatter work items per XCD run (1 = hardware order)
   int xrun_gather;  // gather workgroups per XCD run
@@ -73,6 +74,8 @@ struct FrameData {
   int nsub = 0, nheavy = 0;
   int4 *gsubs = nullptr;       // [ngsub] gather work items {tile, first, end, 0} (home samples)
   int ngsub = 0;
+  int32_t *wide = nullptr;     // [nwide] samples with more than Tiles::NTb taps on some axis
+  int nwide = 0;
   int32_t *sparse = nullptr;   // [nsparse] samples of tiles below the dense limit (direct gather)
   int64_t nsparse = 0;
   int2 *xrng = nullptr;        // [Sz][Sy] x cells any tap reaches in grid row (z, y); x > y: none
@@ -329,6 +332,11 @@ struct PrePostArgs {
   float scale;
   int xrun;             // x-column workgroups per XCD run (xcd_block)
 };
+
+// taps per axis of the unrolled gather and of the tiles' bin reach: 2 dwin
+// when that is an integer (a sample exactly on the window edge has one more),
+// floor(2 dwin) + 1 otherwise
+int taps_main(const Geom &g);
 
 // host-side launchers (defined in the .hip files)
 int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const FrameData &fr, const float2 *sub,

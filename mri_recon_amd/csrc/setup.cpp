@@ -284,6 +284,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   for (int a = 0; a < 3; a++)
     if (g.grid_in[a]) ntap = std::max(ntap, (int)floorf(2 * g.dwin[a]) + 1);
   op->t.NT = ntap;
+  op->t.NTb = taps_main(g);
   if (p.bin_edge != 0 && p.bin_edge != 1 && p.bin_edge != 2) {
     delete op;
     return MRN_EINVAL;
@@ -397,6 +398,7 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.heavy);
     hipFree(fr.gsubs);
     hipFree(fr.sparse);
+    hipFree(fr.wide);
     hipFree(fr.xrng);
     hipFree(fr.yrng);
     hipFree(fr.kdata);

@@ -3,8 +3,8 @@
 //
 // Timed on the frame's own trajectory, one stage at a time, in this order:
 //   1. scatter: sample bin edge B in {1, 2} x samples per work item
-//      {1024, 4096, 16384} (the bins are rebuilt for each), and for 16-coil
-//      blocks the matrix-core or vector-ALU accumulate of the 4^3 tiles;
+//      {1024, 4096, 16384} (the bins are rebuilt for each), and for 8- and
+//      16-coil blocks the matrix-core or vector-ALU accumulate of the 4^3 tiles;
 //   2. gather: coils per lane {1, 2, 4, 8} (<= coils per block) x XCD run
 //      length {16, 64, 256} workgroups;
 //   3. FFT: XCD run length of the y and z passes {1, 4, 8, 16};
@@ -255,7 +255,7 @@ int autotune_impl(mrn_op *op, int f, const char *cache_path, hipStream_t st) {
     Knobs k;
     float ms = 0.f;
     if (it != cache.end() && from_map(it->second, k, ms)) {
-      if (k.scatter_valu && op->g.CC != 16) k.scatter_valu = 0;
+      if (k.scatter_valu && op->g.CC != 16 && op->g.CC != 8) k.scatter_valu = 0;
       if (k.gather_coils > op->g.CC) k.gather_coils = op->g.CC;
       apply(op, k);
       MRN_TRY(rebuild_all(op, st));
@@ -289,7 +289,7 @@ int autotune_impl(mrn_op *op, int f, const char *cache_path, hipStream_t st) {
   float best_ms = 1e30f;
   for (int B : {1, 2})
     for (int maxv : {1024, 4096, 16384})
-      for (int valu = 0; valu <= (op->g.CC == 16 ? 1 : 0); valu++) {
+      for (int valu = 0; valu <= ((op->g.CC == 16 || op->g.CC == 8) ? 1 : 0); valu++) {
         Knobs k = best;
         k.B = B;
         k.maxv = maxv;

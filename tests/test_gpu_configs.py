@@ -36,7 +36,7 @@ TOL32, TOL64 = 1e-5, 1e-4
 
 
 def _log(msg):
-    _log(msg, flush=True)
+    print(msg, flush=True)
 
 
 def _host(t):
