@@ -261,6 +261,32 @@ int mrn_get_tuning(const mrn_op *op, mrn_tuning *out);
 int mrn_stage_times(mrn_op *op, int frame, const void *X, int reps, float ms[MRN_NSTAGES],
                     void *stream);
 
+/* Work lists of one frame's batched gridding (diagnostic; no counterpart
+ * in the reference). */
+typedef struct mrn_frame_stats {
+  int64_t samples;         /* M */
+  int64_t valid;           /* samples with kw != 0 inside the grid */
+  int64_t tile_visits;     /* sample visits over the scatter's 4^3 tiles */
+  int64_t scatter_atomics; /* float atomics of one scatter over all coils, upper bound
+                              (split tiles' flushes and window-edge samples; zero sums skip theirs) */
+  int32_t scatter_items;   /* scatter work items */
+  int32_t split_items;     /* work items of tiles split over several items (flushed with atomics) */
+  int32_t split_tiles;     /* tiles split over several work items */
+  int32_t wide;            /* samples with one tap more than usual on some axis */
+  int32_t gather_items;    /* tile-staged gather work items */
+  int32_t reserved[7];
+} mrn_frame_stats;
+int mrn_get_frame_stats(mrn_op *op, int frame, mrn_frame_stats *out);
+
+/* k-space support of one frame (host copies): xrng [Sz][Sy][2] receives,
+ * per grid row (z, y), the first and last x cell that some sample's taps
+ * reach (first > last: none; INT_MAX, INT_MIN: the row is outside plane z's
+ * rows of support); yrng (nullable) [Sz][2] the first and last row y of
+ * plane z with support.  The forward FFT passes store, and the inverse ones
+ * load, only these cells; without pruning every cell is listed.
+ * Diagnostic; no counterpart in the reference. */
+int mrn_get_support(mrn_op *op, int frame, int32_t *xrng, int32_t *yrng);
+
 /* The op's oversampled k-space grids (the reference's public member
  * gridFFT::k3d_grid, gridFFT.h:30, which RECON::kspace_residual zeroes and
  * fills through chop_grid_forward / chop_grid_backward, recon_lib.cpp:
@@ -316,6 +342,10 @@ int mrn_dcf_grid_size(const mrn_dcf_params *p, int32_t S[3]);
  * and returns when the weights are on the device.  Errors: mrn_last_error(NULL). */
 int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const float *ky, const float *kz,
                  float *kw, double *sums, void *stream);
+/* Float atomics the grid scatters of the last mrn_dcf_calc on the current
+ * device issued (all iterations).  Diagnostic; no counterpart in the
+ * reference. */
+int mrn_dcf_last_atomics(int64_t *count);
 
 /* ---- Locally low-rank threshold over frames ------------------------------
  * LOWRANKCOIL::update_threshold + ::thresh (clear.cpp:233-556) as recon_lib

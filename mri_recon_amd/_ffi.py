@@ -24,10 +24,10 @@ EXPORTS = [
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
     "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
     "mrn_get_fista_thresholds", "mrn_set_fista_step", "mrn_get_fista_steps", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
-    "mrn_dcf_calc", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
+    "mrn_dcf_calc", "mrn_dcf_last_atomics", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
     "mrn_get_fista_llr_smax", "mrn_gate_params_default", "mrn_gate_parse_args", "mrn_gate_scale_times",
     "mrn_pregate",
-    "mrn_get_grid", "mrn_grid_zero", "mrn_chop_grid_forward", "mrn_chop_grid_backward",
+    "mrn_get_frame_stats", "mrn_get_support", "mrn_get_grid", "mrn_grid_zero", "mrn_chop_grid_forward", "mrn_chop_grid_backward",
     "mrn_sync", "mrn_status_string", "mrn_last_error",
 ]
 
@@ -47,6 +47,22 @@ class MrnTuning(ctypes.Structure):
         ("from_cache", ctypes.c_int32),
         ("normal_ms", ctypes.c_float),
         ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
+class MrnFrameStats(ctypes.Structure):
+    """mrn_frame_stats (include/mrn.h)."""
+    _fields_ = [
+        ("samples", ctypes.c_int64),
+        ("valid", ctypes.c_int64),
+        ("tile_visits", ctypes.c_int64),
+        ("scatter_atomics", ctypes.c_int64),
+        ("scatter_items", ctypes.c_int32),
+        ("split_items", ctypes.c_int32),
+        ("split_tiles", ctypes.c_int32),
+        ("wide", ctypes.c_int32),
+        ("gather_items", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -179,6 +195,7 @@ def lib():
             "mrn_dcf_parse_args": (i32, [ctypes.POINTER(DcfParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
             "mrn_dcf_grid_size": (i32, [ctypes.POINTER(DcfParams), vp]),
             "mrn_dcf_calc": (i32, [ctypes.POINTER(DcfParams), i64, vp, vp, vp, vp, vp, vp]),
+            "mrn_dcf_last_atomics": (i32, [ctypes.POINTER(i64)]),
             "mrn_llr_params_default": (None, [ctypes.POINTER(LlrParams)]),
             "mrn_llr_parse_args": (i32, [ctypes.POINTER(LlrParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
             "mrn_llr": (i32, [ctypes.POINTER(LlrParams), vp, i32, i32, vp, i32, i32, i64, vp, vp]),
@@ -189,6 +206,8 @@ def lib():
             "mrn_gate_scale_times": (i32, [ctypes.POINTER(GateParams), i64, vp, vp]),
             "mrn_pregate": (i32, [ctypes.POINTER(GateParams), i64, i64, vp, vp, vp, vp, vp, vp, i32, vp, vp,
                                   vp, vp, vp, vp, vp, vp]),
+            "mrn_get_frame_stats": (i32, [vp, i32, ctypes.POINTER(MrnFrameStats)]),
+            "mrn_get_support": (i32, [vp, i32, vp, vp]),
             "mrn_get_grid": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(i64)]),
             "mrn_grid_zero": (i32, [vp, vp]),
             "mrn_chop_grid_forward": (i32, [vp, i32, vp, vp]),

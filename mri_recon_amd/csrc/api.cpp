@@ -820,6 +820,66 @@ int mrn_stage_times(mrn_op *op, int f, const void *Xv, int reps, float ms[MRN_NS
   return MRN_OK;
 }
 
+int mrn_get_frame_stats(mrn_op *op, int f, mrn_frame_stats *out) {
+  if (!op || !out) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  memset(out, 0, sizeof(*out));
+  const FrameData &fr = op->frames[f];
+  out->samples = fr.M;
+  out->valid = fr.Mv;
+  if (!batched(op)) return MRN_OK;
+  out->scatter_items = fr.nsub;
+  out->split_tiles = fr.nheavy;
+  out->wide = fr.nwide;
+  out->gather_items = fr.ngsub;
+  if (fr.nsub > 0) {
+    std::vector<int4> subs((size_t)fr.nsub);
+    MRN_HIP(op, hipMemcpy(subs.data(), fr.subs, sizeof(int4) * fr.nsub, hipMemcpyDeviceToHost));
+    for (const int4 &w : subs) {
+      out->tile_visits += w.z - w.y;
+      if (w.w) out->split_items++;
+    }
+  }
+  // a split tile's items each add 64 cells x 2 CC reals; a wide sample adds at
+  // most NT^3 taps x 2 CC reals; per coil block
+  const int64_t per_cell = 2 * (int64_t)op->g.CC;
+  const int64_t nt3 = (int64_t)op->t.NT * op->t.NT * op->t.NT;
+  out->scatter_atomics = (out->split_items * 64 * per_cell + (int64_t)fr.nwide * nt3 * per_cell) * op->nch;
+  return MRN_OK;
+}
+
+int mrn_get_support(mrn_op *op, int f, int32_t *xrng, int32_t *yrng) {
+  if (!op || !xrng) return MRN_EINVAL;
+  MRN_TRY(check_frame(op, f, false));
+  const FrameData &fr = op->frames[f];
+  const int64_t nrow = (int64_t)op->g.S[1] * op->g.S[2];
+  if (!batched(op) || !fr.xrng) {  // every cell of every row
+    for (int64_t r = 0; r < nrow; r++) {
+      xrng[2 * r] = 0;
+      xrng[2 * r + 1] = op->g.S[0] - 1;
+    }
+  } else {
+    MRN_HIP(op, hipMemcpy(xrng, fr.xrng, sizeof(int2) * nrow, hipMemcpyDeviceToHost));
+  }
+  if (yrng) {
+    if (fr.yrng && fft_xy_pruned(op)) {
+      MRN_HIP(op, hipMemcpy(yrng, fr.yrng, sizeof(int2) * op->g.S[2], hipMemcpyDeviceToHost));
+    } else {
+      for (int z = 0; z < op->g.S[2]; z++) {
+        yrng[2 * z] = 0;
+        yrng[2 * z + 1] = op->g.S[1] - 1;
+      }
+    }
+  }
+  // the x passes prune only where the four-step form applies
+  if (!fft_x_pruned(op))
+    for (int64_t r = 0; r < nrow; r++) {
+      xrng[2 * r] = 0;
+      xrng[2 * r + 1] = op->g.S[0] - 1;
+    }
+  return MRN_OK;
+}
+
 int mrn_get_grid(mrn_op *op, void **grid, int64_t dims[5]) {
   if (!op || !grid || !dims) return MRN_EINVAL;
   const bool b = batched(op);

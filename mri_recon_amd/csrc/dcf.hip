@@ -160,6 +160,16 @@ __global__ void k_dcf_fill(float *v, int64_t n, float value) {
 // path.  The grid sums were already unordered (float atomics); a box only
 // sums its part in double before rounding once.
 constexpr int DCF_BOX = 8192;  // FP64 LDS box cells (64 KB): 221 ms vs 241 with 4096
+
+// float atomics the scatters of the last mrn_dcf_calc on this device issued
+// to the grid (diagnostic: one 64-bit atomic per wavefront and launch)
+static __device__ unsigned long long g_dcf_atomics;
+__global__ void k_dcf_count_reset() { g_dcf_atomics = 0ull; }
+__device__ __forceinline__ void count_atomics(unsigned long long n) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(&g_dcf_atomics, n);
+}
 __global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, const float *__restrict__ kw,
                                                          const float *__restrict__ kx, const float *__restrict__ ky,
                                                          const float *__restrict__ kz, int64_t M, DcfGeom g,
@@ -214,6 +224,7 @@ __global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, 
   const int64_t vol = (int64_t)ext[0] * ext[1] * ext[2];  // 0: no live sample
   if (vol == 0) return;                                     // (uniform)
   const bool boxed = vol <= DCF_BOX;
+  unsigned long long nat = 0;  // grid atomics of this thread
   if (boxed) {
     for (int q = threadIdx.x; q < vol; q += blockDim.x) s_acc[q] = 0.0;
     __syncthreads();
@@ -228,10 +239,12 @@ __global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, 
         for (int lx = s[0]; lx <= e[0]; lx++) {
           const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
           const float v = __fmul_rn(wtx, temp);
-          if (boxed)
+          if (boxed) {
             atomicAdd(brow + lx, (double)v);
-          else
+          } else {
             atomicAdd(row + lx, v);
+            nat++;
+          }
         }
       }
     }
@@ -244,8 +257,10 @@ __global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, 
       const int x = q % ext[0], yz = q / ext[0];
       const int y = yz % ext[1], z = yz / ext[1];
       atomicAdd(X + ((int64_t)(lo[2] + z) * g.S[1] + (lo[1] + y)) * g.S[0] + lo[0] + x, (float)v);
+      nat++;
     }
   }
+  count_atomics(nat);
 }
 
 // DCFgridFFT::backward (DCFgridFFT.cpp:314-440) and the ratio update of
@@ -398,6 +413,15 @@ int mrn_dcf_grid_size(const mrn_dcf_params *p, int32_t S[3]) {
   return MRN_OK;
 }
 
+int mrn_dcf_last_atomics(int64_t *count) {
+  if (!count) return MRN_EINVAL;
+  unsigned long long v = 0;
+  const hipError_t e = hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_dcf_atomics), sizeof(v), 0, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return dcf_fail(MRN_EHIP, hipGetErrorString(e));
+  *count = (int64_t)v;
+  return MRN_OK;
+}
+
 int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const float *ky, const float *kz,
                  float *kw, double *sums, void *stream) {
   g_free_err.clear();
@@ -464,6 +488,7 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
     hipLaunchKernelGGL(k_dcf_gather3, gs, dim3(256), 0, st, order, kx, ky, kz, M, sx, sy, sz);
     hipLaunchKernelGGL(k_dcf_fill, dim3(fillb), dim3(256), 0, st, sw, M, 1.0f);
     DCF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_dcf_count_reset, dim3(1), dim3(1), 0, st);
     if (p->iters > 0) {
       // zero the per-iteration sums (a double 0 is two float 0s)
       hipLaunchKernelGGL(k_dcf_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(dsum),

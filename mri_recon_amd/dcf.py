@@ -59,3 +59,10 @@ def dcf_calc(kx, ky, kz, N=None, params=None, out=None, return_sums=False, **kw)
     if return_sums:
         return out, [sums[i] for i in range(params.iters)]
     return out
+
+
+def dcf_last_atomics():
+    """Float atomics the grid scatters of the last dcf_calc issued (all iterations)."""
+    n = ctypes.c_int64()
+    check(lib().mrn_dcf_last_atomics(ctypes.byref(n)))
+    return n.value

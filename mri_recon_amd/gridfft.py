@@ -189,6 +189,22 @@ class GridFFT:
         self._c(lib().mrn_autotune(self._h, frame, path, _stream()))
         return self.tuning()
 
+    def support(self, frame=0):
+        """k-space support of a frame: (xrng (Sz, Sy, 2), yrng (Sz, 2)) int32 numpy
+        arrays of the first and last x cell per row and row y per plane."""
+        import numpy as np
+        Sx, Sy, Sz = self.S
+        x = np.zeros((Sz, Sy, 2), np.int32)
+        y = np.zeros((Sz, 2), np.int32)
+        self._c(lib().mrn_get_support(self._h, frame, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data)))
+        return x, y
+
+    def frame_stats(self, frame=0):
+        """Work lists of one frame's gridding (mrn_frame_stats as a dict)."""
+        t = _ffi.MrnFrameStats()
+        self._c(lib().mrn_get_frame_stats(self._h, frame, ctypes.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_ if name != "reserved"}
+
     def tuning(self):
         """Launch shapes in use (mrn_tuning as a dict)."""
         t = _ffi.MrnTuning()

@@ -275,7 +275,7 @@ def test_tile_staged_gather_equals_direct_and_oracle(C, dwin):
     """The opt-in tile-staged gather (coil blocks of 8 or 16, LDS region per
     dense tile, direct gather for the samples of sparse tiles; this radial
     set has both kinds) matches the direct gather and the FP32 oracle.  With
-    8-coil blocks it sums the same products in the same order (the same
+    8-coil blocks A sums the same products in the same order (the same
     bits); with 16-coil blocks the matrix cores sum them in another order
     (FP32 rounding, <= 1e-6).  dwin 2.0 puts the k = 0 samples exactly on
     the window edge (5 taps on an axis), the tap-by-tap path."""
@@ -294,9 +294,11 @@ def test_tile_staged_gather_equals_direct_and_oracle(C, dwin):
     n_dir = ops[1].normal(Xd).cpu().numpy()
     print("tile gather C=%d dwin=%.1f: A %.2e  AHA %.2e" % (C, dwin, rel(a_tile, a_dir), rel(n_tile, n_dir)))
     if C == 8:
-        assert np.array_equal(a_tile, a_dir) and np.array_equal(n_tile, n_dir)
+        assert np.array_equal(a_tile, a_dir)
     else:
-        assert rel(a_tile, a_dir) <= 1e-6 and rel(n_tile, n_dir) <= 1e-6
+        assert rel(a_tile, a_dir) <= 1e-6
+    # E'E also runs the scatter, whose window-edge samples add with atomics
+    assert rel(n_tile, n_dir) <= 1e-6
     o = orc.OracleOp(N, overgrid=1.25, dwin=dwin, precision="f32")
     ref = o.image_to_kspace(X, smaps, traj)
     assert rel(a_tile, ref) < 1e-5

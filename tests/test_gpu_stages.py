@@ -80,3 +80,21 @@ def test_grid_contents_do_not_reach_results(kw):
     op.grid_raw().fill_(complex(math.inf, -math.inf))
     torch.cuda.synchronize()
     assert rel(op.normal(Xd).cpu().numpy(), e0.cpu().numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("kw", [{}, {"bin_edge": 2}, {"coil_chunk": 8}])
+def test_every_tap_lies_in_the_frames_support(kw):
+    """The CPU check behind the grid-contents test: with the support ranges the
+    passes use (copied from the device), every tap of every sample, computed as
+    the gather computes it, lies in its row's support, and the support rows of
+    a plane lie in that plane's y range."""
+    from tests.test_gpu_configs import _taps_in_support
+    N, C = 32, 16
+    op, traj, X, smaps, d = _op(N, C, **kw)
+    bad, n = _taps_in_support(op, traj, 2.5)
+    assert n > 0 and bad == 0
+    xr, yr = op.support()
+    for z in range(op.S[2]):
+        rows = np.nonzero(xr[z, :, 0] <= xr[z, :, 1])[0]
+        if rows.size:
+            assert yr[z, 0] <= rows.min() and rows.max() <= yr[z, 1]
