@@ -26,6 +26,25 @@ sys.path.insert(0, ROOT)
 METRIC = ("batched-graph vs plain SENSE-CG iter/s, 256³×32ch; gridding GB/s; "
           "rel err≤1e-4")
 HBM_PEAK_GBPS = 8000.0
+# The CPU leg's FFT, in the order of preference of the reference's build:
+# FFTW3f (absent from the image), MKL's FFTW3 interface (libmkl_rt, its
+# sequential layer per line under the oracle's OpenMP loop over lines), else
+# the oracle's own mixed-radix Stockham passes
+CPU_FFT_BACKEND = "oracle Stockham (radix 2/3/4/5, OpenMP over lines)"
+
+
+def cpu_fft_setup(o32):
+    """Switch the oracle's line transforms to MKL's FFTW3 interface when it is
+    present and warm it up at the grid size (its first call initialises)."""
+    global CPU_FFT_BACKEND
+    import numpy as np
+
+    from oracle import oracle as orc
+    if orc.use_mkl_fft(True, "f32"):
+        CPU_FFT_BACKEND = ("MKL FFTW3 interface (libmkl_rt, sequential per line, OpenMP over lines); "
+                           "FFTW3f absent from the image")
+        o32.fft3(np.zeros(o32.grid_shape(), np.complex64), -1)
+    return CPU_FFT_BACKEND
 # device kernel behind each stage of GridFFT.stage_times (as rocprofv3 names it)
 KERNEL_NAMES = {
     "pre_z": "k_fft_pre_z", "fft_y": "k_fft_lines (y)", "fft_x": "k_fft_lines (x)",
@@ -62,6 +81,32 @@ def rocprof_summary(root):
     print("\n".join(out))
 
 
+def stage_bytes(N, C, M, S):
+    """Algorithmic bytes per launch of each batched stage over all C coils
+    (compulsory reads + writes; 16*M for the kx, ky, kz, kw trajectory)."""
+    return {
+        "pre_z": 8 * N**3 + 8 * N**3 * C + 8 * N * N * S * C,
+        "fft_y": 8 * N * N * S * C + 8 * N * S * S * C,
+        "fft_x": 8 * N * S * S * C + 8 * S**3 * C,
+        "gather": 8 * S**3 * C + 8 * M * C + 16 * M,
+        "scatter_zero": 0,  # tiles shared by several work items; its bytes count in the scatter's
+        "scatter": 8 * M * C + 16 * M + 8 * S**3 * C,
+        "ifft_x": 8 * S**3 * C + 8 * N * S * S * C,
+        "ifft_y": 8 * N * S * S * C + 8 * N * N * S * C,
+        "post_z": 8 * N * N * S * C + 8 * N**3 * C + 16 * N**3,
+    }
+
+
+def roofline_of(st, b):
+    """Dominant stage of `st` (stage -> ms) with its achieved HBM rate on the
+    algorithmic bytes `b`."""
+    dom = max(st, key=lambda k: st[k])
+    ach = b[dom] / (st[dom] * 1e-3) / 1e9
+    return {"kernel": KERNEL_NAMES[dom], "stage": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS, "traffic": None, "algorithmic_bytes": b[dom],
+            "avg_ms": st[dom]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +118,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the DCF / LLR / gating / L1-FISTA timings")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 128^3 x 8 CG and 160^3 x 24 x 10-frame FISTA+LLR lines")
     ap.add_argument("--next-only", action="store_true",
                     help="only the DCF / LLR / gating / L1-FISTA timings (no CPU leg; for rocprofv3)")
     ap.add_argument("--profile", action="store_true", help="batched iterations only (for rocprofv3)")
@@ -166,17 +213,7 @@ def main():
     st = op.stage_times(xtrue, reps=5)
     S = op.S[0]
     CC = C  # all coils per launch
-    b = {
-        "pre_z": 8 * N**3 + 8 * N**3 * CC + 8 * N * N * S * CC,
-        "fft_y": 8 * N * N * S * CC + 8 * N * S * S * CC,
-        "fft_x": 8 * N * S * S * CC + 8 * S**3 * CC,
-        "gather": 8 * S**3 * CC + 8 * M * CC + 16 * M,
-        "scatter_zero": 0,  # tiles shared by several work items; its bytes count in the scatter's
-        "scatter": 8 * M * CC + 16 * M + 8 * S**3 * CC,
-        "ifft_x": 8 * S**3 * CC + 8 * N * S * S * CC,
-        "ifft_y": 8 * N * S * S * CC + 8 * N * N * S * CC,
-        "post_z": 8 * N * N * S * CC + 8 * N**3 * CC + 16 * N**3,
-    }
+    b = stage_bytes(N, C, M, S)
     stages = {k: {"ms": st[k], "GBps": b[k] / (st[k] * 1e-3) / 1e9 if b[k] else None} for k in st}
     dom = max(st, key=lambda k: st[k])
     fft_ms = sum(st[k] for k in ("pre_z", "fft_y", "fft_x"))
@@ -194,6 +231,17 @@ def main():
         "tuning": op.tuning(),
     }
     log("stages ms: " + ", ".join("%s %.2f" % (k, v) for k, v in st.items()))
+    # float atomics of the scatter: split tiles' flushes and window-edge samples
+    fs = op.frame_stats()
+    res["atomics"] = {
+        "scatter_atomics_per_launch": fs["scatter_atomics"], "scatter_ms": scat_ms,
+        "scatter_atomics_per_s": fs["scatter_atomics"] / (scat_ms * 1e-3),
+        "split_items": fs["split_items"], "split_tiles": fs["split_tiles"], "scatter_items": fs["scatter_items"],
+        "window_edge_samples": fs["wide"], "tile_visits": fs["tile_visits"],
+        "note": "upper bound: zero sums skip their atomics; the rest of the scatter writes each tile once"}
+    log("scatter: %d work items (%d split, %d window-edge samples), <= %.3g atomics -> %.3g /s"
+        % (fs["scatter_items"], fs["split_items"], fs["wide"], fs["scatter_atomics"],
+           res["atomics"]["scatter_atomics_per_s"]))
 
     # ---- weighted adjoint identity on the same operator -------------------
     y = synth.cn_image((C, M), seed=seed + 3, device=dev)
@@ -240,6 +288,7 @@ def main():
     if not a.no_cpu:
         from oracle import oracle as orc
         o32 = orc.OracleOp(N, overgrid=OS, dwin=DW, precision="f32")
+        cpu_fft_setup(o32)
         traj_h = [t.cpu().numpy() for t in (kx, ky, kz, kw)]
         P = xtrue.cpu().numpy()
         sm_h = smaps.cpu().numpy()
@@ -264,6 +313,7 @@ def main():
         err = float(np.linalg.norm(Lg - L1) / np.linalg.norm(L1))
         del op_k
         cpu = {"value": 1.0 / t_iter, "unit": "iter/s", "cores": orc.threads(), "kind": "port",
+               "fft_backend": CPU_FFT_BACKEND,
                "sample": "E'E (A then A^H) of %d of %d coils at the full 256^3 / 5.12M-sample size "
                          "on the FP32 oracle, %.2f s per coil, scaled x%d to one CG iteration "
                          "(vector updates excluded)" % (k, C, t_coil, C),
@@ -275,6 +325,14 @@ def main():
     if not a.no_next:
         res["next_rows"] = next_rows(a, op, N, kx, ky, kz, M, log)
 
+    grid_S = list(op.S)
+    if not a.no_configs:
+        op.close()
+        del data, smaps
+        torch.cuda.empty_cache()
+        res["configs"] = {"sense_cg_128cube_8coil": small_cg(a, log),
+                          "gated_llr_fista_160cube_24coil_10frame": gated_fista(a, log)}
+
     dom_b = b[dom]
     achieved = dom_b / (st[dom] * 1e-3) / 1e9
     traffic = profiled_traffic(KERNEL_NAMES[dom])
@@ -282,7 +340,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "iter/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32 (fp64 CG scalars)", "data": "synthetic",
-        "config": {"workload": workload, "N": N, "coils": C, "samples_per_coil": M, "grid": list(op.S),
+        "config": {"workload": workload, "N": N, "coils": C, "samples_per_coil": M, "grid": grid_S,
                    "overgrid": OS, "kb_half_width": DW, "form": "batched, hipGraph-captured",
                    "parallelism": "single GPU"},
         "roofline": {"kernel": KERNEL_NAMES[dom], "stage": dom, "bound": "hbm", "achieved": achieved,
@@ -294,6 +352,147 @@ def main():
     }
     out.update(res)
     print(json.dumps(out))
+
+
+def small_cg(a, log):
+    """SENSE-CG 128^3 x 8 coils, 4096 x 64 radial, os 1.5, KB half width 2
+    (grid 192^3): batched + graph iter/s, the plain per-coil form, the
+    dominant stage's roofline and the CPU oracle."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import GridFFT, synth
+    N, C, NSP, NPT, OS, DW, seed = 128, 8, 4096, 64, 1.5, 2.0, 1001
+    dev = torch.device("cuda")
+    kx, ky, kz, kw = synth.radial_traj(NSP, NPT, N, device=dev)
+    M = kx.numel()
+    smaps = synth.coil_maps(C, N, seed=seed + 1, device=dev)
+    xtrue = synth.ellipsoid_phantom(N, seed=seed, device=dev)
+    op = GridFFT(N, ncoils=C, overgrid=OS, dwin=DW)
+    op.set_traj(kx, ky, kz, kw)
+    op.set_smaps(smaps)
+    data = synth.add_noise(op.backward(xtrue), 0.01, seed=seed + 2)
+    op.set_kdata(data)
+    if not a.no_autotune:
+        op.autotune()
+    steps = 20
+    X = op.cg_init()
+    op.cg_iterate(X, 3, graph=True, trace=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.cg_iterate(X, steps, graph=True, trace=False)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = op.stage_times(xtrue, reps=5)
+    out = {"workload": "SENSE-CG 128^3 x 8 coils, 3-D radial 4096x64, os 1.5, KB W=4 (grid 192^3)",
+           "value": 1.0 / dt, "unit": "iter/s", "ms_per_step": dt * 1e3, "steps": steps,
+           "roofline": roofline_of(st, stage_bytes(N, C, M, op.S[0])), "stage_ms": st, "tuning": op.tuning()}
+    pl = GridFFT(N, ncoils=C, overgrid=OS, dwin=DW, form="plain")
+    pl.set_traj(kx, ky, kz, kw)
+    pl.set_smaps(smaps)
+    pl.set_kdata(data)
+    Xp = pl.cg_init()
+    pl.cg_iterate(Xp, 1, graph=False, trace=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pl.cg_iterate(Xp, 3, graph=False, trace=False)
+    torch.cuda.synchronize()
+    out["plain_iter_s"] = 3 / (time.perf_counter() - t0)
+    out["speedup_vs_plain"] = out["value"] / out["plain_iter_s"]
+    pl.close()
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        o32 = orc.OracleOp(N, overgrid=OS, dwin=DW, precision="f32")
+        cpu_fft_setup(o32)
+        th = [t.cpu().numpy() for t in (kx, ky, kz, kw)]
+        sm = smaps.cpu().numpy()
+        P = xtrue.cpu().numpy()
+        t0 = time.perf_counter()
+        o32.normal(P, sm[:2], th)
+        tc = (time.perf_counter() - t0) / 2
+        out["cpu_baseline"] = {"value": 1.0 / (tc * C), "unit": "iter/s", "cores": orc.threads(), "kind": "port",
+                               "fft_backend": CPU_FFT_BACKEND,
+                               "sample": "E'E of 2 of 8 coils on the FP32 oracle, %.3f s per coil, x8 to one "
+                                         "iteration (vector updates excluded)" % tc}
+    log("128^3 x 8 CG: %.1f iter/s (plain %.2f, x%.1f), dominant %s %.2f ms (%.1f%% of HBM)"
+        % (out["value"], out["plain_iter_s"], out["speedup_vs_plain"], out["roofline"]["stage"],
+           out["roofline"]["avg_ms"], 100 * out["roofline"]["frac"]))
+    op.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def gated_fista(a, log):
+    """Gated low-rank FISTA 160^3 x 24 coils x 10 frames (Poisson spoke counts,
+    mean 3000 x 256), grid 240^3, local low rank on 8^3 blocks over the
+    frames: ms per FISTA+LLR iteration of the graph-captured form, the
+    dominant stage's roofline on frame 0, and the CPU oracle."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import GridFFT, synth
+    from mri_recon_amd.llr import llr_params
+    N, C, F, seed = 160, 24, 10, 1003
+    dev = torch.device("cuda")
+    frames = synth.gated_radial_frames(F, 3000, 256, N, seed=seed, device=dev)
+    smaps = synth.coil_maps(C, N, seed=seed + 1, device=dev)
+    base = synth.ellipsoid_phantom(N, seed=seed + 2, device=dev)
+    blob = synth.ellipsoid_phantom(N, n_ellipsoids=2, seed=seed + 3, device=dev)
+    op = GridFFT(N, ncoils=C, nframes=F)
+    for f in range(F):
+        op.set_traj(*frames[f], frame=f)
+    op.set_smaps(smaps)
+    for f in range(F):
+        truth = base + (0.5 + 0.5 * math.sin(2 * math.pi * f / F)) * blob
+        op.set_kdata(synth.add_noise(op.backward(truth.contiguous(), frame=f), 0.01, seed=seed + 10 + f), frame=f)
+    if not a.no_autotune:
+        op.autotune()
+    step = 0.5 * op.power_method(8)
+    op.set_fista_llr(llr_params(alpha_time=0.02, block=(8, 8, 8), block_iter=4, seed=23))
+    op.fista(1, step, graph=True)
+    torch.cuda.synchronize()
+    tt = []
+    for k in (1, 4):
+        t0 = time.perf_counter()
+        op.fista(k, step, graph=True)
+        torch.cuda.synchronize()
+        tt.append(time.perf_counter() - t0)
+    per = (tt[1] - tt[0]) / 3
+    Ms = [fr[0].numel() for fr in frames]
+    st = op.stage_times(base, reps=3, frame=0)
+    out = {"workload": "FISTA + local low rank (8^3 blocks, 4 shifted passes), 160^3 x 24 coils x 10 frames, "
+                       "3-D radial Poisson(3000) x 256 spokes per frame, os 1.5, KB W=5 (grid 240^3)",
+           "value": 1.0 / per, "unit": "iter/s", "ms_per_step": per * 1e3, "samples_per_frame": Ms,
+           "roofline": roofline_of(st, stage_bytes(N, C, Ms[0], op.S[0])), "stage_ms_frame0": st,
+           "tuning": op.tuning()}
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        o32 = orc.OracleOp(N, overgrid=1.5, precision="f32")
+        cpu_fft_setup(o32)
+        th = [t.cpu().numpy() for t in frames[0]]
+        sm = smaps[:1].cpu().numpy()
+        P = base.cpu().numpy()
+        d0 = np.zeros((1, Ms[0]), np.complex64)
+        t0 = time.perf_counter()
+        r = o32.image_to_kspace(P, sm, th, sub=d0)
+        o32.kspace_to_image(r, sm, th)
+        tg = time.perf_counter() - t0
+        n = 32
+        Xs = (np.random.default_rng(6).standard_normal((F, 1, n, n, n)) + 0j).astype(np.complex64)
+        t0 = time.perf_counter()
+        orc.llr(orc.llr_params(alpha_time=0.02, block=(8, 8, 8), block_iter=4), Xs, 0, 0, 0, "f32")
+        tl = (time.perf_counter() - t0) * (N / n) ** 3
+        t_it = tg * C * F + tl
+        out["cpu_baseline"] = {"value": 1.0 / t_it, "unit": "iter/s", "cores": orc.threads(), "kind": "port",
+                               "fft_backend": CPU_FFT_BACKEND,
+                               "sample": "FP32 oracle gradient (A - d, then A^H) of 1 coil of frame 0, %.2f s, "
+                                         "x%d coil-frames, plus the oracle LLR step on a %d^3 x %d-frame image "
+                                         "scaled by voxels (%.1f s)" % (tg, C * F, n, F, tl)}
+    log("160^3 x 24 x 10 FISTA+LLR: %.1f ms/iter, dominant %s %.2f ms (%.1f%% of HBM)"
+        % (per * 1e3, out["roofline"]["stage"], out["roofline"]["avg_ms"], 100 * out["roofline"]["frac"]))
+    op.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def next_rows(a, op, N, kx, ky, kz, M, log):
@@ -313,8 +512,12 @@ def next_rows(a, op, N, kx, ky, kz, M, log):
     w = dcf_calc(kx, ky, kz, params=p)
     torch.cuda.synchronize()
     t_dcf = time.perf_counter() - t0
+    from mri_recon_amd.dcf import dcf_last_atomics
+    nat = dcf_last_atomics()
     dcf = {"ms": t_dcf * 1e3, "iters": p.iters, "samples": M, "grid": list(dcf_grid_size(p)),
-           "ms_per_iter": t_dcf * 1e3 / p.iters, "nonzero_weights": int(torch.count_nonzero(w).item())}
+           "ms_per_iter": t_dcf * 1e3 / p.iters, "nonzero_weights": int(torch.count_nonzero(w).item()),
+           "grid_atomics_per_iter": nat / p.iters,
+           "grid_atomics_per_s_whole_iteration": nat / t_dcf}
     log("Pipe DCF (%d iterations, grid %s): %.1f ms" % (p.iters, dcf["grid"], t_dcf * 1e3))
     if not a.no_cpu:
         from oracle import oracle as orc

@@ -42,6 +42,7 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+#include <dlfcn.h>
 #endif
 
 #ifndef ORC_REAL
@@ -351,6 +352,44 @@ void orc_fft1d(int n, int sign, real *data, int howmany) {
   free(pl.tw);
 }
 
+/* Optional line transforms through MKL's FFTW3 interface (libmkl_rt with its
+ * sequential threading layer, one line per OpenMP thread at a time, as the
+ * reference runs FFTW: ArrayTemplates.cpp:276-427).  Only the CPU timing
+ * baseline turns it on; the checker keeps the textbook transform above. */
+typedef void *(*mkl_plan_fn)(int, void *, void *, int, unsigned);
+typedef void (*mkl_exec_fn)(const void *, void *, void *);
+typedef void (*mkl_destroy_fn)(void *);
+static struct {
+  int on;
+  mkl_plan_fn plan;
+  mkl_exec_fn exec;
+  mkl_destroy_fn destroy;
+} g_mkl;
+
+int orc_use_mkl_fft(int on) {
+  if (!on) {
+    g_mkl.on = 0;
+    return 0;
+  }
+  if (!g_mkl.plan) {
+    setenv("MKL_THREADING_LAYER", "SEQUENTIAL", 0);
+    static const char *paths[] = {"libmkl_rt.so", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so"};
+    void *h = NULL;
+    for (int i = 0; i < 3 && !h; i++) h = dlopen(paths[i], RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 0;
+    const int f = sizeof(real) == 4;
+    g_mkl.plan = (mkl_plan_fn)dlsym(h, f ? "fftwf_plan_dft_1d" : "fftw_plan_dft_1d");
+    g_mkl.exec = (mkl_exec_fn)dlsym(h, f ? "fftwf_execute_dft" : "fftw_execute_dft");
+    g_mkl.destroy = (mkl_destroy_fn)dlsym(h, f ? "fftwf_destroy_plan" : "fftw_destroy_plan");
+    if (!g_mkl.plan || !g_mkl.exec || !g_mkl.destroy) {
+      g_mkl.plan = NULL;
+      return 0;
+    }
+  }
+  g_mkl.on = 1;
+  return 1;
+}
+
 /* fft3 on a box [lo,hi) of a column-major grid (x fastest) along `axis`:
  * every line of the box is transformed over the full extent S[axis]. */
 static void fft_axis(cpx *g, const int S[3], int axis, int sign, const int lo[3],
@@ -362,22 +401,37 @@ static void fft_axis(cpx *g, const int S[3], int axis, int sign, const int lo[3]
   long st[3] = {1, S[0], (long)S[0] * S[1]};
   int a1 = (axis == 0) ? 1 : 0, a2 = (axis == 2) ? 1 : 2;
   long nl1 = hi[a1] - lo[a1], nl2 = hi[a2] - lo[a2];
+  void *mp = NULL;
+  const size_t lb = ((sizeof(cpx) * (size_t)n + 63) / 64) * 64;  /* 64-byte aligned line buffers */
+  if (g_mkl.on) {
+    cpx *pa = (cpx *)aligned_alloc(64, 2 * lb);
+    mp = g_mkl.plan(n, pa, (char *)pa + lb, sign, 1u << 6 /* FFTW_ESTIMATE */);
+    free(pa);
+  }
 #pragma omp parallel
   {
-    cpx *buf = (cpx *)malloc(sizeof(cpx) * (size_t)n * 2);
+    cpx *buf = (cpx *)aligned_alloc(64, 2 * lb);
+    cpx *out = (cpx *)((char *)buf + lb);
 #pragma omp for schedule(static)
     for (long l = 0; l < nl1 * nl2; l++) {
       long i1 = lo[a1] + l % nl1, i2 = lo[a2] + l / nl1;
       cpx *base = g + i1 * st[a1] + i2 * st[a2];
       for (int p = 0; p < n; p++) buf[p] = base[p * st[axis]];
-      line_fft(&pl, buf, buf + n);
+      const cpx *res = buf;
+      if (mp) {
+        g_mkl.exec(mp, buf, out);
+        res = out;
+      } else {
+        line_fft(&pl, buf, out);
+      }
       for (int p = 0; p < n; p++) {
-        base[p * st[axis]].re = buf[p].re * scale;
-        base[p * st[axis]].im = buf[p].im * scale;
+        base[p * st[axis]].re = res[p].re * scale;
+        base[p * st[axis]].im = res[p].im * scale;
       }
     }
     free(buf);
   }
+  if (mp) g_mkl.destroy(mp);
   free(pl.tw);
 }
 

@@ -166,6 +166,8 @@ def _lib(precision):
         lib.orc_fista_cauchy.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, i64, ctypes.c_int,
                                          dbl, ctypes.c_int, vp, vp]
         lib.orc_threads.restype = ctypes.c_int
+        lib.orc_use_mkl_fft.argtypes = [ctypes.c_int]
+        lib.orc_use_mkl_fft.restype = ctypes.c_int
         lib.orc_dcf_table_len.argtypes = [ctypes.c_float]
         lib.orc_dcf_table_len.restype = ctypes.c_int
         lib.orc_dcf_table.argtypes = [ctypes.c_float, ctypes.c_int, vp]
@@ -195,6 +197,12 @@ def _lib(precision):
         lib.orc_pregate.argtypes = [GP, i64, i64, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
         _LIBS[precision] = lib
     return _LIBS[precision]
+
+
+def use_mkl_fft(on=True, precision="f32"):
+    """Route the oracle's 3-D FFT line transforms through MKL's FFTW3 interface
+    (the CPU timing baseline only); returns whether MKL was found."""
+    return bool(_lib(precision).orc_use_mkl_fft(int(bool(on))))
 
 
 def threads():

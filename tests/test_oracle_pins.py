@@ -458,3 +458,21 @@ def test_cauchy_step_is_the_exact_line_search():
     vertex = 1 - e * (fp - fm) / (2 * (fp - 2 * f0 + fm))
     assert abs(vertex - 1) < 1e-6
     assert abs(a.imag) < 1e-6 * abs(a.real)  # R^H A^H W A R is real for a Hermitian E'WE
+
+
+def test_mkl_line_transforms_match_the_textbook_fft():
+    """The CPU timing baseline may route the oracle's line transforms through
+    MKL's FFTW3 interface; it computes the same unitary 3-D transform as the
+    oracle's textbook passes (pinned against numpy above) to FP32 rounding."""
+    o = orc.OracleOp(24, overgrid=1.5, precision="f32")
+    rng = np.random.default_rng(2)
+    g = (rng.standard_normal(o.grid_shape()) + 1j * rng.standard_normal(o.grid_shape())).astype(np.complex64)
+    ref = [o.fft3(g, -1), o.fft3(g, 1)]
+    if not orc.use_mkl_fft(True, "f32"):
+        pytest.skip("libmkl_rt not found")
+    try:
+        got = [o.fft3(g, -1), o.fft3(g, 1)]
+    finally:
+        orc.use_mkl_fft(False, "f32")
+    for a, b in zip(got, ref):
+        assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
