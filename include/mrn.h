@@ -377,6 +377,16 @@ int mrn_llr_parse_args(mrn_llr_params *p, int argc, const char **argv);
  * receives the threshold scale.  Returns when done; errors: mrn_last_error(NULL). */
 int mrn_llr(const mrn_llr_params *p, void *X, int Nt, int Ne, const int32_t N[3], int dim, int iter,
             int64_t call, float *smax, void *stream);
+/* mrn_llr without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as mrn_llr; its scratch (8 bytes per block
+ * plus about 2.5 KB) lives in a fixed 8 MiB device array, so calls with more
+ * than about 1,048,000 blocks return MRN_EINVAL with the limit in
+ * mrn_last_error(NULL).  Every call on one device shares that array: calls
+ * that overlap on one device race. */
+int mrn_llr_noalloc(const mrn_llr_params *p, void *X, int Nt, int Ne, const int32_t N[3], int dim, int iter,
+                    int64_t call, float *smax, void *stream);
 /* Time LLR (dim 0 over the op's frames) inside each FISTA iteration, after the
  * gradient step and before the L1 step; NULL or alpha_time = 0 turns it off.
  * The iteration index drives amp's decay and the shift stream's call index. */

@@ -24,7 +24,7 @@ EXPORTS = [
     "mrn_power_method", "mrn_fista", "mrn_traj_grad", "mrn_autotune", "mrn_get_tuning", "mrn_stage_times",
     "mrn_coil_forward_sos", "mrn_set_k_rad", "mrn_wavelet", "mrn_l1_wavelet", "mrn_set_fista_l1",
     "mrn_get_fista_thresholds", "mrn_set_fista_step", "mrn_get_fista_steps", "mrn_dcf_params_default", "mrn_dcf_parse_args", "mrn_dcf_grid_size",
-    "mrn_dcf_calc", "mrn_dcf_last_atomics", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_set_fista_llr",
+    "mrn_dcf_calc", "mrn_dcf_last_atomics", "mrn_llr_params_default", "mrn_llr_parse_args", "mrn_llr", "mrn_llr_noalloc", "mrn_set_fista_llr",
     "mrn_get_fista_llr_smax", "mrn_gate_params_default", "mrn_gate_parse_args", "mrn_gate_scale_times",
     "mrn_pregate",
     "mrn_get_frame_stats", "mrn_get_support", "mrn_get_grid", "mrn_grid_zero", "mrn_chop_grid_forward", "mrn_chop_grid_backward",
@@ -199,6 +199,7 @@ def lib():
             "mrn_llr_params_default": (None, [ctypes.POINTER(LlrParams)]),
             "mrn_llr_parse_args": (i32, [ctypes.POINTER(LlrParams), i32, ctypes.POINTER(ctypes.c_char_p)]),
             "mrn_llr": (i32, [ctypes.POINTER(LlrParams), vp, i32, i32, vp, i32, i32, i64, vp, vp]),
+            "mrn_llr_noalloc": (i32, [ctypes.POINTER(LlrParams), vp, i32, i32, vp, i32, i32, i64, vp, vp]),
             "mrn_set_fista_llr": (i32, [vp, ctypes.POINTER(LlrParams)]),
             "mrn_get_fista_llr_smax": (i32, [vp, vp, vp]),
             "mrn_gate_params_default": (None, [ctypes.POINTER(GateParams)]),

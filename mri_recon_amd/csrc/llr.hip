@@ -20,8 +20,10 @@
 // Shifts: pass `pass` of call `call` shifts block axis q by
 // splitmix64(seed, 3 * (call * block_iter + pass) + q) % b - b/2 (the
 // reference draws rand() % b - b/2), 0 when block_iter == 1.
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "mrn_internal.hpp"
 
@@ -500,6 +502,53 @@ int llr_fista_step(mrn_op *op, float2 *X, hipStream_t st) {
 
 }  // namespace mrn
 
+// ---- fixed scratch of mrn_llr_noalloc ------------------------------------
+// The block maxima (8 bytes per block), the median selection state and the
+// threshold scalars of one call live in this array instead of hipMalloc'd
+// memory, for callers that cannot allocate.  Each device holds its own copy
+// until the process ends; every call on one device shares it, so calls that
+// overlap on one device race.  Every part is written by the call before it
+// is read (the selection state is cleared by k_sel_init).
+#define MRN_LLR_SCRATCH_BYTES (8u << 20)
+static __device__ __align__(256) unsigned char scratch[MRN_LLR_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+size_t scratch_bytes(long long nblk) {
+  return align256(sizeof(double) * (size_t)(nblk > 0 ? nblk : 1)) + align256(sizeof(mrn::SelState)) +
+         align256(sizeof(double) + sizeof(float));
+}
+long long scratch_max_blocks() {
+  return (long long)((MRN_LLR_SCRATCH_BYTES - align256(sizeof(mrn::SelState)) - 256) / sizeof(double));
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current (host code cannot use the symbol's name
+// as a device pointer).
+hipError_t scratch_address(unsigned char **out, const char *&err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = "mrn_llr_noalloc: device index beyond its per-device table";
+    return hipErrorInvalidDevice;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+
 using namespace mrn;
 
 extern "C" {
@@ -570,6 +619,46 @@ int mrn_llr(const mrn_llr_params *p, void *X, int Nt, int Ne, const int32_t N[3]
   hipFree(sel);
   hipFree(tau);
   return out;
+}
+
+int mrn_llr_noalloc(const mrn_llr_params *p, void *X, int Nt, int Ne, const int32_t N[3], int dim, int iter,
+                    int64_t call, float *smax, void *stream) {
+  free_err_clear();
+  if (!p || !X || !N) return free_fail(MRN_EINVAL, "null argument");
+  LlrGeom g;
+  const char *err;
+  const int rc = llr_geom(p, Nt, Ne, N, dim, g, err);
+  if (rc != MRN_OK) return free_fail(rc, err);
+  const long long nblk = llr_blocks(g);
+  const size_t need = scratch_bytes(nblk);
+  if (need > sizeof(scratch)) {
+    char msg[200];
+    snprintf(msg, sizeof msg,
+             "mrn_llr_noalloc: %lld blocks need %zu bytes of scratch; its fixed scratch holds %zu bytes (%lld blocks)",
+             nblk, need, sizeof(scratch), scratch_max_blocks());
+    return free_fail(MRN_EINVAL, msg);
+  }
+  unsigned char *base = nullptr;
+  const char *serr = nullptr;
+  const hipError_t se = scratch_address(&base, serr);
+  if (se != hipSuccess) return free_fail(MRN_EHIP, serr ? serr : hipGetErrorString(se));
+  // layout: block maxima | selection state | tau, smax (256-byte aligned parts)
+  double *sb = reinterpret_cast<double *>(base);
+  void *sel = base + align256(sizeof(double) * (size_t)nblk);
+  double *tau = reinterpret_cast<double *>(base + align256(sizeof(double) * (size_t)nblk) + align256(sizeof(SelState)));
+  float *sm = reinterpret_cast<float *>(tau + 1);
+  hipStream_t st = (hipStream_t)stream;
+  const float alpha = dim == 0 ? p->alpha_time : p->alpha_encode;
+  hipError_t e = llr_launch(g, (float2 *)X, alpha, p->amp, p->t2, iter, (long long)call, nullptr, sb, sel, tau, sm, st);
+  float h = 0.0f;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, sm, sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    return free_fail(MRN_EHIP, hipGetErrorString(e));
+  }
+  if (smax) *smax = h;
+  return MRN_OK;
 }
 
 int mrn_set_fista_llr(mrn_op *op, const mrn_llr_params *p) {
