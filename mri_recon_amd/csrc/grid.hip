@@ -148,6 +148,36 @@ __global__ void k_fill_subs(const int32_t *__restrict__ nsub, const int32_t *__r
   heavy_flag[tile] = n > 1;
 }
 
+// Visit counts widened for the offsets' scan (entry ntiles: 0).
+__global__ void k_widen(const int32_t *__restrict__ v, int n, int64_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) out[i] = i < n ? v[i] : 0;
+}
+
+// Visit list: tile t's visits (the samples of its bin rows, rows in (bz, by)
+// order, each row one contiguous run of internal indices) at
+// vlist[vofs[t] ...], so that the scatter reads visit v of a tile with one
+// load.  One wavefront per tile.
+__global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, const int64_t *__restrict__ vofs,
+                              int32_t *vlist) {
+  const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (tile >= t.ntiles) return;
+  const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
+  int bx0, bx1, by0, by1, bz0, bz1;
+  tile_bins(t, 0, tx, bx0, bx1);
+  tile_bins(t, 1, ty, by0, by1);
+  tile_bins(t, 2, tz, bz0, bz1);
+  int64_t o = vofs[tile];
+  for (int bz = bz0; bz <= bz1; bz++)
+    for (int by = by0; by <= by1; by++) {
+      const int b = (bz * t.nb[1] + by) * t.nb[0];
+      const int s0 = bin_start[b + bx0], n = bin_start[b + bx1 + 1] - s0;
+      for (int q = lane; q < n; q += 64) vlist[o + q] = s0 + q;
+      o += n;
+    }
+}
+
 // Gather work list: the samples whose first tap lies in tile t (its "home"
 // samples: bins [t*T/B, (t*T+T-1)/B] per axis, one contiguous run per (bz, by)
 // row), in items of at most `per` samples.  Tiles without samples get none.
@@ -783,53 +813,203 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
   }
 }
 
-// Grid-driven scatter.  Wavefront w of the workgroup takes work item
-// subs[4*b + w] = {tile, first visit, end visit, heavy} (b = the workgroup's
-// XCD-ordered index); lane l owns cell (l&3, (l>>2)&3, l>>4) of the 4x4x4
-// tile and the CCT coil sums of coil block blockIdx.y.  kin holds
-// kw * taper * data (internal sample order).
-//
-// The samples that can reach the tile lie in the bin rows (bz, by) x
-// [bx0, bx1] of its neighbourhood, each row one contiguous run; their
-// concatenation in (bz, by) order numbers the tile's visits (the order the
-// work list was counted in).  The wavefront walks the visits in batches of
-// SG_BATCH samples:
-//  A. lane (s, q) = (l>>2, l&3) finds visit s of the batch (binary search of
-//     the row prefix sums in LDS), loads its coordinates and, for each axis,
-//     the kernel weight of tile offset q (0 outside the sample's taps).  The
-//     weights and the sample's CCT coil values go to LDS; a ballot marks the
-//     samples with a tap in the tile.
+// Grid-driven scatter on the matrix cores (8- and 16-coil blocks).  A
+// wavefront owns a 4x4x4 tile; its sums T[64 cells][2 CCT reals] are held as
+// one 16 x 16 accumulator block per (z plane, 16 reals).  The wavefront
+// walks the tile's visits in batches of SM_BATCH samples:
+//  A. lane (s, h) = (l >> 1, l & 1) takes visit s of the batch (from the
+//     tile's visit list) and writes its kernel weights of tile offsets 2h,
+//     2h + 1 on each axis (0 outside the sample's taps) to LDS; a ballot per
+//     z plane records which samples reach it.  The next batch's visits,
+//     coordinates and coil values are loaded here, so that the loads fly
+//     during step B.
+//  B. per group of 4 samples and z plane that one of them reaches,
+//     T[plane's 16 cells][16 reals] += W[16][4] D[4][16] by
+//     v_mfma_f32_16x16x4f32: lane i + 16 k holds A = W[cell i][sample k] =
+//     wz (wy wx) from LDS and B = D[sample k][real i], read straight from
+//     k-space in this layout.
+// One sample's taps cover 2.5 of a tile's 4 z planes on average (5 taps over
+// two tiles), and the samples of a group are consecutive visits, mostly of
+// one bin row, so most groups skip a plane or two.  Locating and weighting a
+// sample takes two lanes, not a whole quad per tile offset, and not 64 lanes
+// per cell.  Sums are FP32; only the order of the additions differs from a
+// per-sample loop.  Each tile is written once with plain stores (atomics
+// for a tile split over several work items).
+#define SM_BATCH 32
+template <int CCT, int KM>
+__global__ __launch_bounds__(256, 4) void k_scatter_mm(
+    float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
+    const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
+    Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum) {
+  static_assert(CCT == 8 || CCT == 16, "matrix-core scatter needs 8 or 16 coils per cell");
+  constexpr int NB = CCT / 8;            // 16-real blocks of a cell's coil values
+  constexpr int NG = SM_BATCH / 4;       // sample groups of a batch
+  __shared__ __align__(16) float s_w[4][SM_BATCH * 12];  // [sample][axis][offset] weights
+  extern __shared__ float tab_s[];         // kernel tables (g.ntab floats)
+  // tables in LDS: a lookup waits for LDS, not for the L1/L2 round trip
+  // (before any wavefront can leave: the whole workgroup reaches the barrier)
+  load_tables(tab_s, tables, g.ntab);
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
+  if (sidx >= nsub) return;
+  const int4 sp = subs[sidx];
+  const int tile = sp.x;
+  const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
+  const int t0[3] = {tix * 4, tiy * 4, tiz * 4};
+  const int ch = blockIdx.y;
+  const float *kch = reinterpret_cast<const float *>(kin + ch * chunk_k);
+  const int64_t v0 = vofs[tile];
+  const int32_t *vl = vlist + v0;
+  const int vbeg = max(sp.y, 0), vend = min(sp.z, (int)(vofs[tile + 1] - v0));
+  const int sl = lane >> 1, h = lane & 1;    // step A: sample slot, offset pair
+  const int ci = lane & 15, kq = lane >> 4;  // step B: cell / real, sample of the group
+  // internal index of visit v of this work item (the last one past the end:
+  // a valid index, so that every lane loads)
+  auto visit = [&](int v) -> int { return vl[min(v, vend - 1)]; };
+  // coil values of the batch at `base` in operand layout: group q, block nb
+  // holds real 16 nb + ci of sample 4 q + kq (whose index lane 2 (4 q + kq)
+  // holds in idx); zeros past the end
+  auto load_data = [&](int base, int idx, float (&d)[NG][NB]) {
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      const int s = 4 * q + kq;
+      const int j = __shfl(idx, 2 * s);
+      const float *src = kch + (int64_t)j * (2 * CCT) + ci;
+      const bool ok = base + s < vend;
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        const float v = src[16 * nb];  // every lane loads: j is a valid index
+        d[q][nb] = ok ? v : 0.0f;
+      }
+    }
+  };
+  mrn_v4f acc[4][NB];
+#pragma unroll
+  for (int p = 0; p < 4; p++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[p][nb] = mrn_v4f{0.f, 0.f, 0.f, 0.f};
+  float4 kraw = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto batch = [&](int base, const float (&cur)[NG][NB], float (&nxt)[NG][NB]) {
+    // ---- A: weights of the batch
+    const bool ok = base + sl < vend;
+    const float dk[3] = {kraw.x, kraw.y, kraw.z};
+    int lo[3], hi[3];  // the sample's taps as tile offsets [lo, hi] per axis
+    bool any = ok, wide = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      int s = 0, e = -1;
+      const bool in = axis_range(g, a, dk[a], s, e);
+      // more taps than the tiles' bin reach assumes: scattered by k_scatter_wide
+      wide = wide || e - s + 1 > t.NTb;
+      lo[a] = max(s - t0[a], 0);
+      hi[a] = min(e - t0[a], 3);
+      any = any && in && lo[a] <= hi[a];
+    }
+    any = any && !wide;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      float w2[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int q = 2 * h + u;
+        const float w = KM == 0 ? tap_weight_clamped(g, a, tab_s + g.toff[a], dk[a], t0[a] + q)
+                                : tap_weight<KM>(g, a, tab_s + g.toff[a], dk[a], t0[a] + q);
+        w2[u] = (any && q >= lo[a] && q <= hi[a]) ? w : 0.0f;
+      }
+      *reinterpret_cast<float2 *>(&s_w[wave][sl * 12 + 4 * a + 2 * h]) = make_float2(w2[0], w2[1]);
+    }
+    // bit 2 s of pm[p]: sample s reaches z plane p of the tile
+    unsigned long long pm[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) pm[p] = __ballot(any && h == 0 && p >= lo[2] && p <= hi[2]);
+    // ---- the next batch: locate, load coordinates and coil values
+    const int nbase = base + SM_BATCH;
+    if (nbase < vend) {
+      const int ni = visit(nbase + sl);
+      kraw = coords[ni];
+      load_data(nbase, ni, nxt);
+    }
+    wave_sync();
+    // ---- B: accumulate, plane by plane
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      const float *w = &s_w[wave][(4 * q + kq) * 12];
+      const float wyx = w[4 + (ci >> 2)] * w[ci & 3];
+      const mrn_v4f wz = *reinterpret_cast<const mrn_v4f *>(w + 8);
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        if ((pm[p] >> (8 * q)) & 0xFFull) {
+          const float a = wz[p] * wyx;
+#pragma unroll
+          for (int nb = 0; nb < NB; nb++)
+            acc[p][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, cur[q][nb], acc[p][nb], 0, 0, 0);
+        }
+      }
+    }
+    wave_sync();
+  };
+  float dA[NG][NB], dB[NG][NB];
+  if (vbeg < vend) {
+    const int i0 = visit(vbeg + sl);
+    kraw = coords[i0];
+    load_data(vbeg, i0, dA);
+  }
+  for (int base = vbeg; base < vend; base += 2 * SM_BATCH) {
+    batch(base, dA, dB);
+    if (base + SM_BATCH >= vend) break;
+    batch(base + SM_BATCH, dB, dA);
+  }
+  // accumulator block (p, nb), register r of lane l: cell 16 p + 4 (l / 16) + r,
+  // real 16 nb + l % 16 of the cell's CCT coils
+#pragma unroll
+  for (int p = 0; p < 4; p++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int cx = t0[0] + r, cy = t0[1] + kq, cz = t0[2] + p;
+      if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) continue;
+      float *dst = reinterpret_cast<float *>(grid + ch * chunk_grid +
+                                             (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT) + ci;
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        const float v = acc[p][nb][r];
+        if (sp.w) {
+          if (v != 0.f) atomicAdd(dst + 16 * nb, v);
+        } else if (accum) {  // the tile's only work item: a plain read-modify-write
+          dst[16 * nb] += v;
+        } else {
+          dst[16 * nb] = v;
+        }
+      }
+    }
+}
+
+// Grid-driven scatter on the vector ALUs (any coil block; the matrix-core
+// form above takes 8- and 16-coil blocks unless told otherwise).  Lane l
+// owns cell (l&3, (l>>2)&3, l>>4) of the wavefront's 4x4x4 tile and the CCT
+// coil sums of coil block blockIdx.y; kin holds kw * taper * data (internal
+// sample order).  The wavefront walks the visits in batches of SG_BATCH
+// samples:
+//  A. lane (s, q) = (l>>2, l&3) reads visit s of the batch from the tile's
+//     visit list, loads its coordinates and, for each axis, the kernel weight
+//     of tile offset q (0
+//     outside the sample's taps).  The weights and the sample's CCT coil
+//     values go to LDS; a ballot marks the samples with a tap in the tile.
 //  B. for each marked sample, every lane forms its weight
 //     (wz * wy) * wx from LDS and adds weight * data to its CCT sums.
 // Per sample the wavefront thus evaluates 12 weights instead of 192, and a
 // sample that misses the tile costs only its share of step A.  The sums
 // accumulate in visit order, as a lane-per-sample walk would.
-//
-// MM (CCT = 16): step B is the product T[64 cells][32] += W[64][16] D[16][32]
-// of the batch, W the cell weights of its samples and D their 16 coil values
-// as 32 reals, on the matrix cores: per pair of samples two
-// v_mfma_f32_32x32x2f32 (cells 0-31 and 32-63), A = W column pair (lane
-// i + 32k holds W[cell i][sample k]), B = D row pair (lane j + 32k holds
-// D[sample k][j]).  The VALU form spends 32 FMAs per sample on all 64 lanes
-// while only about a quarter of the cells are taps of the sample, and is
-// bound by that issue rate; the matrix cores do the same 2048 products at
-// twice the FP32 rate, and the VALU is left with the weights.  Sums are
-// FP32 throughout; only the order of the additions differs.
 #define SG_BATCH 16
-#define SG_MAXROWS 144
-template <int CCT, bool MM>
+template <int CCT>
 __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
-    const int32_t *__restrict__ bin_start, const int4 *__restrict__ subs, int nsub, Geom g,
-    Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum) {
-  __shared__ int s_rs[4][SG_MAXROWS];        // first sample of each bin row
-  __shared__ int s_rp[4][SG_MAXROWS + 1];    // visits before each row
+    const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
+    Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum) {
   __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
-  __shared__ float s_yz[4][MM ? SG_BATCH * 16 : 1];  // MM: [sample][z][y] products wz * wy
   __shared__ float2 s_d[4][SG_BATCH * CCT];  // [sample][coil] data
   extern __shared__ float tab_s[];           // kernel tables (g.ntab floats)
-  // tables in LDS: a lookup waits for LDS, not for the L1/L2 round trip
-  // (before any wavefront can leave: the whole workgroup reaches the barrier)
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -843,75 +1023,27 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
   const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
   const int ch = blockIdx.y;
   const float2 *kch = kin + ch * chunk_k;
-  int bx0, bx1, by0, by1, bz0, bz1;
-  tile_bins(t, 0, tix, bx0, bx1);
-  tile_bins(t, 1, tiy, by0, by1);
-  tile_bins(t, 2, tiz, bz0, bz1);
-  const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
-  // row starts and inclusive prefix counts (rows r = (bz - bz0) * nry + by - by0)
-  int carry = 0;
-  for (int r0 = 0; r0 < nrows; r0 += 64) {
-    const int r = r0 + lane;
-    int n = 0;
-    if (r < nrows) {
-      const int brow = ((bz0 + r / nry) * t.nb[1] + by0 + r % nry) * t.nb[0];
-      const int s0 = bin_start[brow + bx0];
-      n = bin_start[brow + bx1 + 1] - s0;
-      s_rs[wave][r] = s0;
-    }
-    int x = n;
+  const int64_t v0 = vofs[tile];
+  const int32_t *vl = vlist + v0;
+  const int vbeg = max(sp.y, 0), vend = min(sp.z, (int)(vofs[tile + 1] - v0));
+  float ar[CCT], ai[CCT];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o);
-      if (lane >= o) x += y;
-    }
-    if (r < nrows) s_rp[wave][r + 1] = carry + x;
-    carry += __shfl(x, 63);
-  }
-  if (lane == 0) s_rp[wave][0] = 0;
-  wave_sync();
-  const int vbeg = max(sp.y, 0), vend = min(sp.z, carry);
-  static_assert(!MM || CCT == 16 || CCT == 8, "matrix-core scatter needs 8 or 16 coils per cell");
-  float ar[MM ? 1 : CCT], ai[MM ? 1 : CCT];
-#pragma unroll
-  for (int c = 0; c < (MM ? 1 : CCT); c++) ar[c] = ai[c] = 0.f;
-  mrn_v16f acc0 = {}, acc1 = {};  // MM, 16 coils: cells 0-31 and 32-63
-  mrn_v4f a8[4] = {};             // MM, 8 coils: cells 16 b .. 16 b + 15
+  for (int c = 0; c < CCT; c++) ar[c] = ai[c] = 0.f;
   const int sl = lane >> 2, q = lane & 3;
   constexpr int CPL = CCT >= 4 ? CCT / 4 : 1;  // coil values a lane stages
-  // internal index of visit v of this work item (0 past the end: a valid
-  // index, so that every lane loads and a batch's loads fly together)
-  auto visit_index = [&](int v) -> int64_t {
-    if (v >= vend) return 0;
-    int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_rp[wave][mid] <= v) lo = mid;
-      else hi = mid - 1;
-    }
-    return (int64_t)s_rs[wave][lo] + (v - s_rp[wave][lo]);
-  };
-  // the batch's coordinates and coil values; MM loads the next batch's while
-  // the matrix cores work on this one
-  float4 kraw;
-  float2 dv[CPL];
-  auto load_batch = [&](int64_t i) {
-    kraw = coords[i];
-    if (q * CPL < CCT) {
-#pragma unroll
-      for (int c = 0; c < CPL; c++) dv[c] = kch[i * CCT + q * CPL + c];
-    }
-  };
-  if (MM && vbeg < vend) load_batch(visit_index(vbeg + sl));
   for (int base = vbeg; base < vend; base += SG_BATCH) {
     // ---- A: weights and data of the batch
     const int v = base + sl;
     const bool ok = v < vend;
-    if (!MM) load_batch(visit_index(v));
-    const float4 k = ok ? kraw : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t i = vl[min(v, vend - 1)];
+    float2 dv[CPL];
+    if (q * CPL < CCT) {
+#pragma unroll
+      for (int c = 0; c < CPL; c++) dv[c] = kch[i * CCT + q * CPL + c];
+    }
+    const float4 k = ok ? coords[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float dk[3] = {k.x, k.y, k.z};
     unsigned long long hit = ~0ull;
-    float wa[3];
     int sta[3], ena[3];
     bool oka[3], wide = false;
 #pragma unroll
@@ -926,71 +1058,19 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     for (int a = 0; a < 3; a++) {
       const int l = t0[a] + q;
       const bool in = oka[a] && !wide && l >= sta[a] && l <= ena[a];
-      wa[a] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
-      s_w[wave][sl * 12 + a * 4 + q] = wa[a];
+      s_w[wave][sl * 12 + a * 4 + q] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
       unsigned long long m = __ballot(in);
       m |= m >> 1;
       m |= m >> 2;
       hit &= m;  // bit 4s: sample s has a tap in the tile on every axis so far
     }
     hit &= 0x1111111111111111ull;
-    if constexpr (MM) {
-      // lane q of the sample's quad: products wz[q] * wy[0..3], wy by DPP
-      // broadcast inside the quad (quad_perm [y, y, y, y])
-#define MRN_YZ(y)                                                                                      \
-  s_yz[wave][sl * 16 + q * 4 + y] =                                                                   \
-      wa[2] * __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(wa[1]), 0x55 * y, 0xF, 0xF, false));
-      MRN_YZ(0) MRN_YZ(1) MRN_YZ(2) MRN_YZ(3)
-#undef MRN_YZ
-    }
-    if (q * CPL < CCT && (ok || MM)) {  // MM reads every sample of the batch: zeros past the end
+    if (q * CPL < CCT && ok) {
 #pragma unroll
-      for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = ok ? dv[c] : make_float2(0.f, 0.f);
+      for (int c = 0; c < CPL; c++) s_d[wave][sl * CCT + q * CPL + c] = dv[c];
     }
-    if (MM && base + SG_BATCH < vend) load_batch(visit_index(base + SG_BATCH + sl));
     wave_sync();
     // ---- B: accumulate the samples that reach the tile
-    if constexpr (MM && CCT == 8) {
-      // T[64 cells][16 reals] += W[64][16] D[16][16] as four
-      // v_mfma_f32_16x16x4f32 per 4 samples (cells 16 b .. 16 b + 15): lane
-      // i + 16 k holds A = W[cell 16 b + i][sample k] and B = D[sample k][real i]
-      const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
-      const int col = lane & 15, kq = lane >> 4;
-      if (hit)
-#pragma unroll
-        for (int kt = 0; kt < SG_BATCH / 4; kt++) {
-          const int smp = 4 * kt + kq;
-          const float b = dflat[smp * 2 * CCT + col];
-          const float wx = s_w[wave][smp * 12 + (col & 3)];
-#pragma unroll
-          for (int cb = 0; cb < 4; cb++) {
-            const float a = s_yz[wave][smp * 16 + 4 * cb + (col >> 2)] * wx;
-            a8[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, a8[cb], 0, 0, 0);
-          }
-        }
-      hit = 0;
-    } else if constexpr (MM) {
-      const float *dflat = reinterpret_cast<const float *>(&s_d[wave][0]);
-      const int c0 = lane & 31, c1 = c0 + 32;
-      // no per-pair skip: straight-line code (a sample off the tile has
-      // weight 0).  Reading every operand of the batch first, with the 16
-      // products back to back, needs 120 VGPRs (4 waves per SIMD) and was
-      // slower, 8.9 ms against 7.9.
-      if (hit)
-#pragma unroll
-      for (int kt = 0; kt < SG_BATCH / 2; kt++) {
-        const int smp = 2 * kt + (lane >> 5);
-        // cell c = 16 z + 4 y + x: (wz wy) wx, with wz wy from s_yz[16 z + 4 y]
-        // so that a step's three LDS reads are independent
-        const float wx = s_w[wave][smp * 12 + (c0 & 3)];
-        const float b = dflat[smp * 2 * CCT + c0];
-        const float a0 = s_yz[wave][smp * 16 + (c0 >> 2)] * wx;
-        const float a1 = s_yz[wave][smp * 16 + (c1 >> 2)] * wx;
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b, acc1, 0, 0, 0);
-      }
-      hit = 0;
-    }
     while (hit) {
       const int s = __builtin_ctzll(hit) >> 2;
       hit &= hit - 1;
@@ -1005,53 +1085,6 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
       }
     }
     wave_sync();
-  }
-  if constexpr (MM && CCT == 8) {
-    // a8[b] register r of lane l: cell 16 b + 4 (l / 16) + r, real number l % 16
-    // of the cell's 8 coils
-#pragma unroll
-    for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int cell = 16 * cb + 4 * (lane >> 4) + r;
-        const int cx = t0[0] + (cell & 3), cy = t0[1] + ((cell >> 2) & 3), cz = t0[2] + (cell >> 4);
-        if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) continue;
-        float *dst = reinterpret_cast<float *>(grid + ch * chunk_grid +
-                                               (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT) +
-                     (lane & 15);
-        const float v = a8[cb][r];
-        if (sp.w) {
-          if (v != 0.f) atomicAdd(dst, v);
-        } else if (accum) {
-          *dst += v;
-        } else {
-          *dst = v;
-        }
-      }
-    return;
-  } else if constexpr (MM) {
-    // accumulator register r of lane l: cell 8*(r/4) + 4*(l/32) + r%4 (+32 for
-    // acc1), real number l%32 of the cell's 16 coils
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int cell = 32 * h + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int cx = t0[0] + (cell & 3), cy = t0[1] + ((cell >> 2) & 3), cz = t0[2] + (cell >> 4);
-        if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) continue;
-        float *dst = reinterpret_cast<float *>(grid + ch * chunk_grid +
-                                               (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT) +
-                     (lane & 31);
-        const float v = h ? acc1[r] : acc0[r];
-        if (sp.w) {
-          if (v != 0.f) atomicAdd(dst, v);
-        } else if (accum) {  // the tile's only work item: a plain read-modify-write
-          *dst += v;
-        } else {
-          *dst = v;
-        }
-      }
-    return;
   }
   const int cx = t0[0] + lx, cy = t0[1] + ly, cz = t0[2] + lz;
   if (cx >= g.S[0] || cy >= g.S[1] || cz >= g.S[2]) return;
@@ -1373,23 +1406,25 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   dim3 g3((fr.nsub + 3) / 4, op->nch);
   const size_t sh = sizeof(float) * g.ntab;  // kernel tables in LDS
-  const int rows_axis = (t.T + t.NTb - 2) / t.B + 2;  // bins per axis a tile can reach
-  if (rows_axis * rows_axis > SG_MAXROWS) return fail(op, MRN_EUNSUPPORTED, "scatter neighbourhood too large");
   // no work item: a frame without valid samples whose x passes are limited to
   // its (empty) k-space support -- nothing reads the grid
   if (fr.nsub == 0) return MRN_OK;
-  if (g.CC == 16 && !op->scatter_valu) {
-    hipLaunchKernelGGL((k_scatter_gd<16, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
-                       fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);
-  } else if (g.CC == 8 && !op->scatter_valu) {
-    hipLaunchKernelGGL((k_scatter_gd<8, true>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.bin_start,
-                       fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);
+  const bool exact = g.kmode == MRN_KMODE_EXACT_KB;
+  if ((g.CC == 16 || g.CC == 8) && !op->scatter_valu) {
+#define MRN_MM(n, km)                                                                                  \
+  hipLaunchKernelGGL((k_scatter_mm<n, km>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,      \
+                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af)
+    if (g.CC == 16 && !exact) MRN_MM(16, 0);
+    else if (g.CC == 16) MRN_MM(16, 1);
+    else if (!exact) MRN_MM(8, 0);
+    else MRN_MM(8, 1);
+#undef MRN_MM
   } else {
     switch (g.CC) {
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
-    hipLaunchKernelGGL((k_scatter_gd<n, false>), g3, dim3(256), sh, st, grid, kin, fr.coords,    \
-                       fr.bin_start, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);          \
+    hipLaunchKernelGGL((k_scatter_gd<n>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,  \
+                       fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);              \
     break;
     MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)
 #undef MRN_S
@@ -1566,6 +1601,10 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipFree(fr.bin_start);
   hipFree(fr.subs);
   hipFree(fr.heavy);
+  hipFree(fr.vofs);
+  hipFree(fr.vlist);
+  fr.vofs = nullptr;
+  fr.vlist = nullptr;
   hipFree(fr.gsubs);
   hipFree(fr.sparse);
   hipFree(fr.xrng);
@@ -1683,6 +1722,29 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipLaunchKernelGGL(k_tile_nsub, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, nsub, visits,
                      fft_x_pruned(op) ? fr.xrng : nullptr, g.S[1], g.S[2]);
   MRN_BH(hipGetLastError());
+  // visit list (offsets in 64 bits: about 8 visits per sample)
+  {
+    MRN_BH(hipMalloc(&fr.vofs, sizeof(int64_t) * (nt + 1)));
+    int64_t *vw = nullptr;
+    MRN_BH(hipMalloc(&vw, sizeof(int64_t) * (nt + 1)));
+    hipLaunchKernelGGL(k_widen, dim3(nblk((int64_t)nt + 1, 256)), dim3(256), 0, st, visits, nt, vw);
+    size_t vb = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, vb, vw, fr.vofs, nt + 1, st);
+    void *vtmp = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&vtmp, vb > 0 ? vb : 16);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(vtmp, vb, vw, fr.vofs, nt + 1, st);
+    int64_t nv = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nv, fr.vofs + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(vtmp);
+    hipFree(vw);
+    MRN_BH(e);
+    MRN_BH(hipMalloc(&fr.vlist, sizeof(int32_t) * (nv > 0 ? nv : 1)));
+    if (nv > 0) {
+      hipLaunchKernelGGL(k_fill_visits, dim3(nblk(nt, 4)), dim3(256), 0, st, fr.bin_start, t, fr.vofs, fr.vlist);
+      MRN_BH(hipGetLastError());
+    }
+  }
   MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nsub, off, nt + 1, st));
   int32_t nsub_total = 0;
   MRN_BH(hipMemcpyAsync(&nsub_total, off + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));

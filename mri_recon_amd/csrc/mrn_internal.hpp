@@ -72,6 +72,8 @@ struct FrameData {
   int4 *subs = nullptr;        // [nsub] {tile, first visit, end visit, heavy}
   int32_t *heavy = nullptr;    // [nheavy] tiles split over several work items
   int nsub = 0, nheavy = 0;
+  int64_t *vofs = nullptr;     // [ntiles + 1] first entry of each tile's visits in vlist
+  int32_t *vlist = nullptr;    // [vofs[ntiles]] samples of each tile's bin rows, in visit order
   int4 *gsubs = nullptr;       // [ngsub] gather work items {tile, first, end, 0} (home samples)
   int ngsub = 0;
   int32_t *wide = nullptr;     // [nwide] samples with more than Tiles::NTb taps on some axis
@@ -261,10 +263,14 @@ __device__ __forceinline__ float dev_i0(float x) {
 // Kernel weight of tap l (with the fftshift sign), table or exact-KB mode.
 // tab points at this axis' table (in LDS).  Mirrors the tap loops of
 // gridFFT.cpp:1276-1306: the z weight is 1 on a Cartesian z axis.
+// KM: -1 = the mode in g.kmode; 0 = table mode; 1 = exact KB (a kernel
+// specialised for one mode passes it, and the other branch compiles away).
+template <int KM = -1>
 __device__ __forceinline__ float tap_weight(const Geom &g, int a, const float *tab, float dk,
                                             int l) {
   float w;
-  if (g.kmode == MRN_KMODE_EXACT_KB && g.grid_in[a]) {
+  const bool exact = KM < 0 ? g.kmode == MRN_KMODE_EXACT_KB : KM == 1;
+  if (exact && g.grid_in[a]) {
     float x = fabsf(dk - (float)l) / g.dwin[a];
     w = (x <= 1.0f) ? g.knorm[a] * dev_i0(g.beta[a] * sqrtf(fmaxf(1.0f - x * x, 0.0f))) / g.i0beta[a]
                     : 0.0f;
@@ -276,6 +282,21 @@ __device__ __forceinline__ float tap_weight(const Geom &g, int a, const float *t
     float f1 = (i0 + 1 < g.tlen[a]) ? tab[i0 + 1] : 0.0f;
     w = f0 * (1.0f - fr) + f1 * fr;
   }
+  if (a == 2 && !g.grid_in[2]) w = 1.0f;
+  if (g.fft_in[a] && !(l & 1)) w = -w;
+  return w;
+}
+
+// Table-mode weight of tap l for any l, with no guarded loads: the table
+// index is clamped to the table's last pair.  Taps inside the window never
+// reach the clamp (the table holds 600 dwin + 10 entries, zeros past
+// 600 dwin) and get tap_weight's value; past the table both entries are
+// zeros, so the weight is 0 there as in tap_weight.
+__device__ __forceinline__ float tap_weight_clamped(const Geom &g, int a, const float *tab, float dk, int l) {
+  const float del = fabsf(600.0f * (dk - (float)l));
+  const int i0 = min((int)del, g.tlen[a] - 2);
+  const float fr = del - (float)i0;
+  float w = tab[i0] * (1.0f - fr) + tab[i0 + 1] * fr;
   if (a == 2 && !g.grid_in[2]) w = 1.0f;
   if (g.fft_in[a] && !(l & 1)) w = -w;
   return w;

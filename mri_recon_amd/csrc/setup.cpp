@@ -396,6 +396,8 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.bin_start);
     hipFree(fr.subs);
     hipFree(fr.heavy);
+    hipFree(fr.vofs);
+    hipFree(fr.vlist);
     hipFree(fr.gsubs);
     hipFree(fr.sparse);
     hipFree(fr.wide);
