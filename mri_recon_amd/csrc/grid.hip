@@ -868,21 +868,20 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   // internal index of visit v of this work item (the last one past the end:
   // a valid index, so that every lane loads)
   auto visit = [&](int v) -> int { return vl[min(v, vend - 1)]; };
-  // coil values of the batch at `base` in operand layout: group q, block nb
-  // holds real 16 nb + ci of sample 4 q + kq (whose index lane 2 (4 q + kq)
-  // holds in idx); zeros past the end
-  auto load_data = [&](int base, int idx, float (&d)[NG][NB]) {
+  // coil values of a batch in operand layout: group q, block nb holds real
+  // 16 nb + ci of sample 4 q + kq (whose index lane 2 (4 q + kq) holds in
+  // idx).  Past the end idx repeats the last visit, whose weights there are
+  // 0.  Byte offsets are 32-bit (the launcher checks that a coil block's
+  // k-space stays below 4 GiB).
+  const char *kb = reinterpret_cast<const char *>(kch);
+  const char *cb = reinterpret_cast<const char *>(coords);
+  auto load_data = [&](int idx, float (&d)[NG][NB]) {
 #pragma unroll
     for (int q = 0; q < NG; q++) {
-      const int s = 4 * q + kq;
-      const int j = __shfl(idx, 2 * s);
-      const float *src = kch + (int64_t)j * (2 * CCT) + ci;
-      const bool ok = base + s < vend;
+      const uint32_t j = (uint32_t)__shfl(idx, 2 * (4 * q + kq));
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++) {
-        const float v = src[16 * nb];  // every lane loads: j is a valid index
-        d[q][nb] = ok ? v : 0.0f;
-      }
+      for (int nb = 0; nb < NB; nb++)
+        d[q][nb] = *reinterpret_cast<const float *>(kb + (j * (8u * CCT) + 4u * (16 * nb + ci)));
     }
   };
   mrn_v4f acc[4][NB];
@@ -891,6 +890,7 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[p][nb] = mrn_v4f{0.f, 0.f, 0.f, 0.f};
   float4 kraw = make_float4(0.f, 0.f, 0.f, 0.f);
+  int inext = 0;  // indices of the batch after the current one
   auto batch = [&](int base, const float (&cur)[NG][NB], float (&nxt)[NG][NB]) {
     // ---- A: weights of the batch
     const bool ok = base + sl < vend;
@@ -924,12 +924,13 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     unsigned long long pm[4];
 #pragma unroll
     for (int p = 0; p < 4; p++) pm[p] = __ballot(any && h == 0 && p >= lo[2] && p <= hi[2]);
-    // ---- the next batch: locate, load coordinates and coil values
+    // ---- the next batch: coordinates and coil values (its indices came
+    // one batch earlier), and the indices of the batch after it
     const int nbase = base + SM_BATCH;
     if (nbase < vend) {
-      const int ni = visit(nbase + sl);
-      kraw = coords[ni];
-      load_data(nbase, ni, nxt);
+      kraw = *reinterpret_cast<const float4 *>(cb + (uint32_t)inext * 16u);
+      load_data(inext, nxt);
+      inext = visit(nbase + SM_BATCH + sl);
     }
     wave_sync();
     // ---- B: accumulate, plane by plane
@@ -953,8 +954,9 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   float dA[NG][NB], dB[NG][NB];
   if (vbeg < vend) {
     const int i0 = visit(vbeg + sl);
-    kraw = coords[i0];
-    load_data(vbeg, i0, dA);
+    inext = visit(vbeg + SM_BATCH + sl);
+    kraw = *reinterpret_cast<const float4 *>(cb + (uint32_t)i0 * 16u);
+    load_data(i0, dA);
   }
   for (int base = vbeg; base < vend; base += 2 * SM_BATCH) {
     batch(base, dA, dB);
@@ -1410,7 +1412,10 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   // its (empty) k-space support -- nothing reads the grid
   if (fr.nsub == 0) return MRN_OK;
   const bool exact = g.kmode == MRN_KMODE_EXACT_KB;
-  if ((g.CC == 16 || g.CC == 8) && !op->scatter_valu) {
+  // the matrix-core form addresses a coil block's k-space and the coordinates
+  // with 32-bit byte offsets
+  const bool off32 = fr.Mv * g.CC * 8 < (1ll << 32) && fr.Mv * 16 < (1ll << 32);
+  if ((g.CC == 16 || g.CC == 8) && !op->scatter_valu && off32) {
 #define MRN_MM(n, km)                                                                                  \
   hipLaunchKernelGGL((k_scatter_mm<n, km>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,      \
                      fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af)
