@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--next-only", action="store_true",
                     help="only the DCF / LLR / gating / L1-FISTA timings (no CPU leg; for rocprofv3)")
     ap.add_argument("--profile", action="store_true", help="batched iterations only (for rocprofv3)")
+    ap.add_argument("--gather-tiles", type=int, default=0, choices=(0, 1, 2),
+                    help="gather of the dense tiles: 0 direct, 1 LDS-staged, 2 box matrix-core")
     ap.add_argument("--small", action="store_true", help="128^3 x 8 coils, 4096 x 64 radial (os 1.5, W=4)")
     ap.add_argument("--rocprof-summary", metavar="DIR", help="summarise rocprofv3 CSV output and exit")
     a = ap.parse_args()
@@ -156,7 +158,7 @@ def main():
     M = kx.numel()
     smaps = synth.coil_maps(C, N, seed=seed, device=dev)
     xtrue = synth.cn_image((N, N, N), seed=seed + 1, device=dev)
-    op = GridFFT(N, ncoils=C, overgrid=OS, dwin=DW, form="batched")
+    op = GridFFT(N, ncoils=C, overgrid=OS, dwin=DW, form="batched", gather_tiles=a.gather_tiles)
     op.set_traj(kx, ky, kz, kw)
     op.set_smaps(smaps)
     data = synth.add_noise(op.backward(xtrue), 0.01, seed=seed + 2)

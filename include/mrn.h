@@ -68,7 +68,8 @@ typedef struct mrn_params {
   int32_t bin_edge;      /* sample sort bin edge in cells (1 or 2); 0 = auto */
   int32_t xcd_run;       /* scatter work items per XCD run (1 = hardware order); 0 = auto */
   int32_t xcd_run_gather;/* gather workgroups per XCD run; 0 = auto */
-  int32_t gather_tiles;  /* 1 = stage dense tiles of the grid in LDS for the gather (default 0) */
+  int32_t gather_tiles;  /* dense tiles of the gather: 1 = stage the grid in LDS, 2 = box matrix-core
+                          * gather (16-coil blocks; default 0 = direct gather) */
   int32_t scatter_valu;  /* 1 = vector-ALU scatter accumulate instead of the matrix cores (default 0) */
   int32_t fft_stockham;  /* 1 = LDS Stockham FFT passes instead of the four-step register form (default 0) */
   int32_t xcd_run_fft;   /* y and z FFT pass workgroups per XCD run (1 = hardware order); 0 = auto (8) */

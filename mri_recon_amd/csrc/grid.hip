@@ -414,7 +414,7 @@ __device__ __forceinline__ bool in_support(const int2 *__restrict__ xrng, const 
 // wavefront serves 64 / (CCT/2) samples per pass.  A sample with more than NT
 // taps on an axis (it sits exactly on the window edge) reads its taps from the
 // grid instead.  Same arithmetic and order per coil as k_gather.
-#define GT_PER 2048  // home samples per work item
+#define GT_PER 128   // home samples per work item
 #define GT_MIN 64    // tiles with fewer home samples are gathered directly
 template <int NT, int CCT>
 __global__ __launch_bounds__(256) void k_gather_tile(const float2 *__restrict__ grid, float2 *__restrict__ out,
@@ -767,6 +767,255 @@ __global__ __launch_bounds__(256) void k_gather_mm(const float2 *__restrict__ gr
             const float2 *row = Gc + ((int64_t)lz * g.S[1] + ly) * g.S[0] * CCT;
             for (int lx = s3[0]; lx < s3[0] + n3[0]; lx++) {
               const float ww = wzy * tap_weight(g, 0, tables + g.toff[0], dk[0], lx);
+              const float2 q = row[(int64_t)lx * CCT];
+              ar += ww * q.x;
+              ai += ww * q.y;
+            }
+          }
+        }
+        const int64_t oi = ch * chunk_k + i * CCT + lane;
+        if (sub) {
+          const float2 d = sub[oi];
+          ar -= d.x;
+          ai -= d.y;
+        }
+        if (weighted) {
+          const float tp = taper[i];
+          ar = (ar * k.w) * tp;
+          ai = (ai * k.w) * tp;
+        }
+        out[oi] = make_float2(ar, ai);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// Matrix-core gather of the dense tiles without staging (16-coil blocks).
+// Work item gsubs[blockIdx.x] = {tile, first, end} over the home samples of
+// a tile (first tap inside the 4x4x4 tile); each wavefront takes batches of
+// 32 of them and forms out[32 samples][32 reals] = W[32][cells] G[cells][32]
+// with v_mfma_f32_32x32x2f32, as k_gather_mm does, but
+//  * only over the box of region cells that some sample of the batch reaches
+//    (consecutive home samples share bins: about 5 x 5 x 6 cells rather than
+//    the (NT + 3)^3 region), and
+//  * with one z plane of the box staged in LDS at a time (the 8 x cells of
+//    each row of the box: 1 KB per row, one 16-byte load per lane, 8 KB per
+//    wavefront), the next plane's loads in flight during this plane's
+//    products.
+// Each grid value read serves the 32 samples of the batch; the per-lane
+// gather reads every tap of every sample through the texture path.  Cells
+// outside the frame's k-space support (which the forward x pass did not
+// store) are staged as 0.  Step A, the samples with taps outside the region
+// and the epilogue are k_gather_mm's.
+template <int NT>
+__global__ __launch_bounds__(256) void k_gather_box(const float2 *__restrict__ grid, float2 *__restrict__ out,
+                                                    const float4 *__restrict__ coords,
+                                                    const float2 *__restrict__ sub,
+                                                    const int32_t *__restrict__ bin_start,
+                                                    const int4 *__restrict__ gsubs, int ngsub, Geom g, Tiles t,
+                                                    const float *__restrict__ tables, int64_t chunk_grid,
+                                                    int64_t chunk_k, const float *__restrict__ taper,
+                                                    int weighted, const int2 *__restrict__ xrng) {
+  constexpr int CCT = 16, R = NT + 3, RX = (R + 1) & ~1, XP = RX / 2;
+  static_assert(RX <= 8, "region offsets are kept in 8-bit masks");
+  __shared__ int s_rs[4][32], s_rp[4][33];
+  __shared__ float s_w[4][3][RX][32];  // [wave][axis][offset][sample]
+  __shared__ int64_t s_idx[4][32];     // internal sample index, -1 if none
+  __shared__ float s_kw[4][32], s_tp[4][32];
+  __shared__ float4 s_g[4][RX * 64];   // [wave][row of the plane][x cell][16 bytes]: one staged z plane
+  extern __shared__ float tab_s[];     // kernel tables (g.ntab floats)
+  // (before any wavefront can leave: the whole workgroup reaches the barrier)
+  load_tables(tab_s, tables, g.ntab);
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
+  if (sidx >= ngsub) return;
+  const int4 sp = gsubs[sidx];
+  const int tile = sp.x;
+  const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
+  const int t0[3] = {tix * 4, tiy * 4, tiz * 4};
+  const int ch = blockIdx.y;
+  // ---- home rows of the tile and their prefix counts (at most 16 rows)
+  int bx0, bx1, by0, by1, bz0, bz1;
+  home_bins(t, 0, tix, bx0, bx1);
+  home_bins(t, 1, tiy, by0, by1);
+  home_bins(t, 2, tiz, bz0, bz1);
+  const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
+  {
+    const int r = lane;
+    int n = 0;
+    if (r < nrows) {
+      const int brow = ((bz0 + r / nry) * t.nb[1] + by0 + r % nry) * t.nb[0];
+      const int s0 = bin_start[brow + bx0];
+      n = bin_start[brow + bx1 + 1] - s0;
+      s_rs[wave][r] = s0;
+    }
+    int x = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (r >= o) x += y;
+    }
+    if (r < nrows) s_rp[wave][r + 1] = x;
+    if (r == 0) s_rp[wave][0] = 0;
+  }
+  wave_sync();
+  const int col = lane & 31, kh = lane >> 5;  // MFMA row / column index, k half
+  for (int base = sp.y; base < sp.z; base += 32) {
+    // ---- A: samples of the batch, their weights per axis and region
+    // offset: lane (s, h) = (l % 32, l / 32) writes offsets 4h .. 4h + 3
+    unsigned long long slow = 0;  // samples computed tap by tap
+    unsigned box = 0;             // region offsets reached: bits 8 a + r
+    {
+      const int v = base + col;
+      int64_t i = -1;
+      float kw = 0.f, tp = 0.f;
+      bool local = false;
+      float dk[3] = {0.f, 0.f, 0.f};
+      int s3[3] = {0, 0, 0}, n3[3] = {0, 0, 0};
+      if (v < sp.z) {
+        int lo = 0, hi = nrows - 1;  // last row with rp[row] <= v
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_rp[wave][mid] <= v) lo = mid;
+          else hi = mid - 1;
+        }
+        i = (int64_t)s_rs[wave][lo] + (v - s_rp[wave][lo]);
+        const float4 k = coords[i];
+        kw = k.w;
+        tp = weighted ? taper[i] : 1.0f;
+        dk[0] = k.x;
+        dk[1] = k.y;
+        dk[2] = k.z;
+        local = true;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          int e;
+          axis_range(g, a, dk[a], s3[a], e);
+          n3[a] = e - s3[a] + 1;
+          local = local && n3[a] <= NT && s3[a] - t0[a] >= 0 && s3[a] - t0[a] + n3[a] <= R;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int u = 0; u < RX / 2; u++) {
+          const int r = (RX / 2) * kh + u;
+          const int tt = r - (s3[a] - t0[a]);
+          const bool in = local && tt >= 0 && tt < n3[a];
+          const float w = g.kmode == MRN_KMODE_EXACT_KB ? tap_weight(g, a, tab_s + g.toff[a], dk[a], t0[a] + r)
+                                                        : tap_weight_clamped(g, a, tab_s + g.toff[a], dk[a], t0[a] + r);
+          s_w[wave][a][r][col] = in ? w : 0.0f;
+          box |= in ? 1u << (8 * a + r) : 0u;
+        }
+      if (kh == 0) {
+        s_idx[wave][col] = i;
+        s_kw[wave][col] = kw;
+        s_tp[wave][col] = tp;
+      }
+      slow = __ballot(i >= 0 && !local) & 0xFFFFFFFFull;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) box |= __shfl_xor(box, o);
+    box = __builtin_amdgcn_readfirstlane(box);
+    wave_sync();
+    // ---- B: out[32][32] = W[32][box cells] G[box cells][32], one z plane
+    // of the box at a time: the wavefront stages the plane's rows (all 8 x
+    // cells of the region, 1 KB per row, one 16-byte load per lane) in LDS,
+    // issues the next plane's loads, then runs the plane's products
+    mrn_v16f acc0 = {}, acc1 = {};
+    if (box & 0xFFu) {
+      const unsigned ym = (box >> 8) & 0xFFu, zm = (box >> 16) & 0xFFu;
+      const int zlo = __builtin_ctz(zm), zhi = 31 - __builtin_clz(zm);
+      const int ylo = __builtin_ctz(ym), ny = 32 - __builtin_clz(ym) - ylo;
+      float wx[XP];
+#pragma unroll
+      for (int p = 0; p < XP; p++) wx[p] = s_w[wave][0][2 * p + kh][col];
+      // staging lane: cell x = t0 + (lane >> 3) of each row, 16 bytes q = lane & 7
+      const int sx = t0[0] + (lane >> 3);
+      const bool xin = (lane >> 3) < RX && sx < g.S[0];
+      const float4 *G4 = reinterpret_cast<const float4 *>(grid + ch * chunk_grid) + (xin ? sx * 8 + (lane & 7) : 0);
+      const int64_t rowst = (int64_t)g.S[0] * 8;  // float4 per grid row
+      float4 st[RX];
+      auto load_plane = [&](int cz) {
+        const int gz = t0[2] + cz, gy0 = t0[1] + ylo;
+        const float4 *P = G4 + ((int64_t)gz * g.S[1] + gy0) * rowst;
+#pragma unroll
+        for (int j = 0; j < RX; j++) {
+          st[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (j < ny) {
+            int2 rr = make_int2(0, g.S[0] - 1);
+            if (xrng) rr = xrng[(int64_t)gz * g.S[1] + gy0 + j];
+            if (xin && sx >= rr.x && sx <= rr.y) st[j] = P[j * rowst];
+          }
+        }
+      };
+      auto store_plane = [&]() {
+#pragma unroll
+        for (int j = 0; j < RX; j++)
+          if (j < ny) s_g[wave][j * 64 + lane] = st[j];
+      };
+      const float *pg = reinterpret_cast<const float *>(&s_g[wave][0]) + kh * 32 + col;
+      load_plane(zlo);
+      store_plane();
+      wave_sync();
+      for (int cz = zlo; cz <= zhi; cz++) {
+        if (cz < zhi) load_plane(cz + 1);
+        const float wz = s_w[wave][2][cz][col];
+        for (int j = 0; j < ny; j++) {
+          const float wzy = wz * s_w[wave][1][ylo + j][col];
+#pragma unroll
+          for (int p = 0; p < XP; p++) {
+            const float a = wzy * wx[p];
+            const float bv = pg[(j * 8 + 2 * p) * 32];
+            if (p & 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc1, 0, 0, 0);
+            else acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc0, 0, 0, 0);
+          }
+        }
+        if (cz < zhi) {
+          wave_sync();
+          store_plane();
+          wave_sync();
+        }
+      }
+    }
+    // ---- epilogue: lane holds sample 8(r/4) + 4 kh + r%4, real col
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int smp = 8 * (r >> 2) + 4 * kh + (r & 3);
+      const int64_t i = s_idx[wave][smp];
+      if (i < 0 || ((slow >> smp) & 1ull)) continue;
+      float val = acc0[r] + acc1[r];
+      float *o = reinterpret_cast<float *>(out + ch * chunk_k + i * CCT) + col;
+      if (sub) val -= reinterpret_cast<const float *>(sub + ch * chunk_k + i * CCT)[col];
+      if (weighted) val = (val * s_kw[wave][smp]) * s_tp[wave][smp];
+      *o = val;
+    }
+    // ---- samples whose taps leave the region: lane = coil, taps from the grid
+    while (slow) {
+      const int smp = __builtin_ctzll(slow);
+      slow &= slow - 1;
+      if (lane < CCT) {
+        const int64_t i = s_idx[wave][smp];
+        const float4 k = coords[i];
+        const float dk[3] = {k.x, k.y, k.z};
+        int s3[3], n3[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          int e;
+          axis_range(g, a, dk[a], s3[a], e);
+          n3[a] = e - s3[a] + 1;
+        }
+        const float2 *Gc = grid + ch * chunk_grid + lane;
+        float ar = 0.f, ai = 0.f;
+        for (int lz = s3[2]; lz < s3[2] + n3[2]; lz++) {
+          const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
+          for (int ly = s3[1]; ly < s3[1] + n3[1]; ly++) {
+            const float wzy = wz * tap_weight(g, 1, tab_s + g.toff[1], dk[1], ly);
+            const float2 *row = Gc + ((int64_t)lz * g.S[1] + ly) * g.S[0] * CCT;
+            for (int lx = s3[0]; lx < s3[0] + n3[0]; lx++) {
+              const float ww = wzy * tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
               const float2 q = row[(int64_t)lx * CCT];
               ar += ww * q.x;
               ai += ww * q.y;
@@ -1321,6 +1570,18 @@ static int launch_gather_mm(mrn_op *op, const float2 *grid, float2 *out, const m
   return MRN_OK;
 }
 
+template <int NT>
+static int launch_gather_box(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
+                             const float2 *sub, int wflag, hipStream_t st) {
+  const Geom &g = op->g;
+  int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
+  hipLaunchKernelGGL((k_gather_box<NT>), dim3((fr.ngsub + 3) / 4, op->nch), dim3(256), sizeof(float) * g.ntab, st,
+                     grid, out, fr.coords, sub, fr.bin_start, fr.gsubs, fr.ngsub, g, op->t, op->tables_d, cg, ck,
+                     fr.taper, wflag, fft_x_pruned(op) ? fr.xrng : nullptr);
+  MRN_HIP(op, hipGetLastError());
+  return MRN_OK;
+}
+
 // Direct gather over all samples, or over the samples listed in `list`.
 static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
                                 const float2 *sub, int wflag, const int32_t *list, int64_t nlist,
@@ -1363,6 +1624,16 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
   const int wf = weighted ? 1 : 0;
   const int nt = taps_main(g);
   const size_t lds = gather_tile_lds(op, nt);
+  if (op->gather_tiles == 2 && g.CC == 16 && op->t.T == 4 && fr.ngsub > 0 && nt >= 3 && nt <= 5) {
+    int rc = MRN_OK;
+    switch (nt) {
+      case 3: rc = launch_gather_box<3>(op, grid, out, fr, sub, wf, st); break;
+      case 4: rc = launch_gather_box<4>(op, grid, out, fr, sub, wf, st); break;
+      default: rc = launch_gather_box<5>(op, grid, out, fr, sub, wf, st); break;
+    }
+    if (rc != MRN_OK) return rc;
+    return fr.nsparse > 0 ? launch_gather_direct(op, grid, out, fr, sub, wf, fr.sparse, fr.nsparse, st) : MRN_OK;
+  }
   if (lds && fr.ngsub > 0 && op->gather_tiles) {
     int rc = MRN_EUNSUPPORTED;
     switch (nt * 100 + g.CC) {

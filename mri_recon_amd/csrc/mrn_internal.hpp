@@ -128,7 +128,7 @@ struct mrn_op {
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   // L1 wavelet proximal step of FISTA (fraction 0 = none)
-  bool gather_tiles = false;   // tile-staged gather over the dense tiles
+  int gather_tiles = 0;        // dense tiles: 1 = LDS-staged gather, 2 = box matrix-core gather
   bool scatter_valu = false;   // VALU scatter accumulate (else matrix cores for 16-coil blocks)
   bool fft_stockham = false;   // LDS Stockham FFT passes (else four-step where the size allows)
   int xcd_run_fft = 8;         // y and z FFT pass workgroups per XCD run

@@ -307,7 +307,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->t.maxv = p.max_visits > 0 ? p.max_visits : 4096;
   op->t.xrun = p.xcd_run > 0 ? p.xcd_run : 1;
   op->t.xrun_gather = p.xcd_run_gather > 0 ? p.xcd_run_gather : 64;
-  op->gather_tiles = p.gather_tiles != 0;
+  op->gather_tiles = p.gather_tiles == 2 ? 2 : p.gather_tiles != 0 ? 1 : 0;
   op->scatter_valu = p.scatter_valu != 0;
   op->fft_stockham = p.fft_stockham != 0;
   op->xcd_run_fft = p.xcd_run_fft > 0 ? p.xcd_run_fft : 8;

@@ -103,7 +103,7 @@ class GridFFT:
         p.bin_edge = int(bin_edge)
         p.xcd_run = int(xcd_run)
         p.xcd_run_gather = int(xcd_run_gather)
-        p.gather_tiles = int(bool(gather_tiles))
+        p.gather_tiles = 2 if gather_tiles == 2 else int(bool(gather_tiles))
         p.scatter_valu = int(bool(scatter_valu))
         p.fft_stockham = int(bool(fft_stockham))
         p.xcd_run_fft = int(xcd_run_fft)

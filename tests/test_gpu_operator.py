@@ -270,19 +270,22 @@ def test_autotune_keeps_results():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("C,dwin", [(16, 2.5), (8, 2.0), (16, 1.5), (8, 3.0), (16, 2.0), (32, 2.5), (16, 3.0)])
-def test_tile_staged_gather_equals_direct_and_oracle(C, dwin):
-    """The opt-in tile-staged gather (coil blocks of 8 or 16, LDS region per
-    dense tile, direct gather for the samples of sparse tiles; this radial
-    set has both kinds) matches the direct gather and the FP32 oracle.  With
-    8-coil blocks A sums the same products in the same order (the same
+@pytest.mark.parametrize("C,dwin,mode", [(16, 2.5, 1), (8, 2.0, 1), (16, 1.5, 1), (8, 3.0, 1), (16, 2.0, 1),
+                                         (32, 2.5, 1), (16, 3.0, 1), (16, 2.5, 2), (16, 1.5, 2), (16, 2.0, 2),
+                                         (32, 2.5, 2), (32, 2.0, 2)])
+def test_tile_staged_gather_equals_direct_and_oracle(C, dwin, mode):
+    """The opt-in gathers of the dense tiles (mode 1: coil blocks of 8 or
+    16, LDS region per dense tile; mode 2: the box matrix-core gather of
+    16-coil blocks; the direct gather for the samples of sparse tiles; this
+    radial set has both kinds) match the direct gather and the FP32 oracle.
+    With 8-coil blocks A sums the same products in the same order (the same
     bits); with 16-coil blocks the matrix cores sum them in another order
     (FP32 rounding, <= 1e-6).  dwin 2.0 puts the k = 0 samples exactly on
     the window edge (5 taps on an axis), the tap-by-tap path."""
     N = 32
     traj, X, smaps, d = radial_problem(N, C, 900, 40, seed=41)
     ops = []
-    for tiles in (True, False):
+    for tiles in (mode, False):
         op = GridFFT(N, ncoils=C, overgrid=1.25, dwin=dwin, gather_tiles=tiles)
         op.set_traj(*traj_dev(traj))
         op.set_smaps(to_dev(smaps))
@@ -292,7 +295,7 @@ def test_tile_staged_gather_equals_direct_and_oracle(C, dwin):
     a_dir = ops[1].backward(Xd).cpu().numpy()
     n_tile = ops[0].normal(Xd).cpu().numpy()
     n_dir = ops[1].normal(Xd).cpu().numpy()
-    print("tile gather C=%d dwin=%.1f: A %.2e  AHA %.2e" % (C, dwin, rel(a_tile, a_dir), rel(n_tile, n_dir)))
+    print("tile gather %d C=%d dwin=%.1f: A %.2e  AHA %.2e" % (mode, C, dwin, rel(a_tile, a_dir), rel(n_tile, n_dir)))
     if C == 8:
         assert np.array_equal(a_tile, a_dir)
     else:

@@ -32,7 +32,7 @@ def _op(N, C, os_=1.5, **kw):
 
 
 @pytest.mark.parametrize("C,kw", [(4, {}), (16, {}), (16, {"scatter_valu": True}), (8, {"gather_tiles": True}),
-                                  (1, {"form": "plain"})])
+                                  (16, {"gather_tiles": 2}), (1, {"form": "plain"})])
 def test_chop_grid_stages_match_oracle(C, kw):
     N = 24
     op, traj, X, smaps, d = _op(N, C, **kw)
@@ -57,8 +57,8 @@ def test_chop_grid_stages_match_oracle(C, kw):
     assert rel(y, yref) <= 1e-5 and rel(ys, ysref) <= 1e-5
 
 
-@pytest.mark.parametrize("kw", [{}, {"gather_tiles": True}, {"fft_stockham": True}, {"coil_chunk": 8},
-                                {"coil_chunk": 4}])
+@pytest.mark.parametrize("kw", [{}, {"gather_tiles": True}, {"gather_tiles": 2}, {"fft_stockham": True},
+                                {"coil_chunk": 8}, {"coil_chunk": 4}])
 def test_grid_contents_do_not_reach_results(kw):
     N, C = 32, 16
     op, traj, X, smaps, d = _op(N, C, **kw)
