@@ -48,7 +48,7 @@ def cpu_fft_setup(o32):
 # device kernel behind each stage of GridFFT.stage_times (as rocprofv3 names it)
 KERNEL_NAMES = {
     "pre_z": "k_fft_pre_z", "fft_y": "k_fft_lines (y)", "fft_x": "k_fft_lines (x)",
-    "gather": "k_gather", "scatter_zero": "k_zero_tiles", "scatter": "k_scatter_gd",
+    "gather": "k_gather", "scatter_zero": "k_zero_tiles", "scatter": "k_scatter_mm",
     "ifft_x": "k_fft_lines (x, inverse)", "ifft_y": "k_fft_lines (y, inverse)", "post_z": "k_fft_post_z",
 }
 
@@ -78,6 +78,18 @@ def rocprof_summary(root):
                                                   "mean/dispatch", "max/dispatch"))
         for (k, c), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:80]:
             out.append("%-64s %-11s %5d %15.1f %15.1f" % (k, c, len(v), sum(v) / len(v), max(v)))
+        # HBM counters: scale factor from the profile run's plain 1 GiB stream
+        # (torch's neg kernel, which reads and writes 1048576 KiB)
+        for ctr, verb in (("FETCH_SIZE", "reads"), ("WRITE_SIZE", "writes")):
+            best = None
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == ctr and "neg_kernel" in r["Kernel_Name"]:
+                    v = float(r["Counter_Value"])
+                    if best is None or v > best[1]:
+                        best = (r.get("Dispatch_Id", "?"), v)
+            if best and best[1] > 0:
+                out.append("# calibration %s factor %.4f (1 GiB stream, dispatch %s, %s %.6f KiB)"
+                           % (ctr, 1048576.0 / best[1], best[0], verb, best[1]))
     print("\n".join(out))
 
 
@@ -187,11 +199,12 @@ def main():
     value = a.steps / (t1 - t0)
     log("batched+graph CG: %.2f ms/iter, %.2f iter/s" % (ms_step, value))
     if a.profile:
-        # a plain 1 GiB device copy: the HBM counters of this kernel (FETCH_SIZE,
-        # WRITE_SIZE) calibrate those of the operator's kernels
+        # a plain 1 GiB stream (torch's neg kernel reads and writes 1 GiB, and
+        # nothing else here runs it): its HBM counters (FETCH_SIZE, WRITE_SIZE)
+        # calibrate those of the operator's kernels
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
         st = op.stage_times(xtrue, reps=3)
@@ -649,7 +662,7 @@ def pregate_timing(a, kx, ky, kz, M, log):
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summaries (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by the
-    same counter's reading for a 1 GiB device copy; None if the summaries do
+    same counter's reading for a plain 1 GiB stream; None if the summaries do
     not cover the kernel."""
     here = os.path.dirname(os.path.abspath(__file__))
     base = os.path.join(here, "profiles", "sense_cg_256cube_32coil_")
