@@ -1145,18 +1145,17 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     const bool ok = base + sl < vend;
     const float dk[3] = {kraw.x, kraw.y, kraw.z};
     int lo[3], hi[3];  // the sample's taps as tile offsets [lo, hi] per axis
-    bool any = ok, wide = false;
+    bool any = ok;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       int s = 0, e = -1;
       const bool in = axis_range(g, a, dk[a], s, e);
-      // more taps than the tiles' bin reach assumes: scattered by k_scatter_wide
-      wide = wide || e - s + 1 > t.NTb;
+      // the first NTb taps (the tiles' bin reach); k_scatter_wide adds the rest
+      e = min(e, s + t.NTb - 1);
       lo[a] = max(s - t0[a], 0);
       hi[a] = min(e - t0[a], 3);
       any = any && in && lo[a] <= hi[a];
     }
-    any = any && !wide;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       float w2[2];
@@ -1296,19 +1295,20 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     const float dk[3] = {k.x, k.y, k.z};
     unsigned long long hit = ~0ull;
     int sta[3], ena[3];
-    bool oka[3], wide = false;
+    bool oka[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       sta[a] = 0;
       ena[a] = -1;
       oka[a] = ok && axis_range(g, a, dk[a], sta[a], ena[a]);
-      // more taps than the tiles' bin reach assumes: scattered by k_scatter_wide
-      wide = wide || ena[a] - sta[a] + 1 > t.NTb;
+      // the first NTb taps (the tiles' bin reach); k_scatter_wide adds the rest
+      ena[a] = min(ena[a], sta[a] + t.NTb - 1);
     }
+    const bool inall = oka[0] && oka[1] && oka[2];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const int l = t0[a] + q;
-      const bool in = oka[a] && !wide && l >= sta[a] && l <= ena[a];
+      const bool in = inall && l >= sta[a] && l <= ena[a];
       s_w[wave][sl * 12 + a * 4 + q] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
       unsigned long long m = __ballot(in);
       m |= m >> 1;
@@ -1358,12 +1358,15 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
   }
 }
 
-// The samples with more than NTb taps on some axis (fr.wide), which the tile
-// scatter leaves out: every tap added with float atomics after the tiles are
-// written.  Thread = (listed sample, coil) of coil block blockIdx.y.  Such
-// samples sit exactly on a window edge, so the list is short (often empty).
+// The samples with more than NTb taps on some axis (fr.wide): the tile
+// scatter adds their first NTb taps on every axis (the box its bins reach),
+// and this kernel adds the rest -- the taps past that box, on the window's far
+// edge -- with float atomics after the tiles are written.  Thread = (listed
+// sample, coil) of coil block blockIdx.y.  Such samples sit exactly on a
+// window edge, so the list is short (often empty), and each has one plane of
+// edge taps per wide axis (25 of 150 taps for W = 5 on one axis).
 __global__ void k_scatter_wide(float2 *grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
-                               const int32_t *__restrict__ wide, int nwide, Geom g,
+                               const int32_t *__restrict__ wide, int nwide, Geom g, int NTb,
                                const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)nwide * g.CC) return;
@@ -1383,6 +1386,8 @@ __global__ void k_scatter_wide(float2 *grid, const float2 *__restrict__ kin, con
       const float wzy = wz * tap_weight(g, 1, tables + g.toff[1], dk[1], ly);
       float2 *row = G + ((int64_t)lz * g.S[1] + ly) * g.S[0] * g.CC;
       for (int lx = s[0]; lx <= e[0]; lx++) {
+        // inside the first NTb taps on every axis: the tile scatter's
+        if (lx < s[0] + NTb && ly < s[1] + NTb && lz < s[2] + NTb) continue;
         const float w = wzy * tap_weight(g, 0, tables + g.toff[0], dk[0], lx);
         atomicAdd(&row[(int64_t)lx * g.CC].x, w * v.x);
         atomicAdd(&row[(int64_t)lx * g.CC].y, w * v.y);
@@ -1711,7 +1716,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   MRN_HIP(op, hipGetLastError());
   if (fr.nwide > 0) {  // after the tiles' plain stores
     hipLaunchKernelGGL(k_scatter_wide, dim3(nblk((int64_t)fr.nwide * g.CC, 256), op->nch), dim3(256), 0, st, grid,
-                       kin, fr.coords, fr.wide, fr.nwide, g, op->tables_d, cg, ck);
+                       kin, fr.coords, fr.wide, fr.nwide, g, t.NTb, op->tables_d, cg, ck);
     MRN_HIP(op, hipGetLastError());
   }
   return MRN_OK;
