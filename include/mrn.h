@@ -74,6 +74,9 @@ typedef struct mrn_params {
   int32_t fft_stockham;  /* 1 = LDS Stockham FFT passes instead of the four-step register form (default 0) */
   int32_t xcd_run_fft;   /* y and z FFT pass workgroups per XCD run (1 = hardware order); 0 = auto (8) */
   int32_t gather_coils;  /* coils per lane in the gather: 1, 2, 4 or 8; 0 = auto */
+  int32_t fuse_frames;   /* multi-frame solver loops (batched form): 1 = one launch per stage over every
+                          * frame (F grids resident), 0 = frame by frame, -1 (default) = fuse when the
+                          * F grids take at most half the free device memory */
 } mrn_params;
 
 typedef struct mrn_op mrn_op;

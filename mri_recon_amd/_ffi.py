@@ -138,6 +138,7 @@ class MrnParams(ctypes.Structure):
         ("fft_stockham", ctypes.c_int32),
         ("xcd_run_fft", ctypes.c_int32),
         ("gather_coils", ctypes.c_int32),
+        ("fuse_frames", ctypes.c_int32),
     ]
 
 

@@ -65,6 +65,9 @@ static PrePostArgs prepost(const mrn_op *op, int ch, int overwrite, int CC) {
   a.overwrite = overwrite;
   a.scale = fft_scale(op->g.S[2]);
   a.xrun = op->xcd_run_fft;
+  a.nf = 1;
+  a.nchf = op->nch;
+  a.xfs = 0;
   return a;
 }
 
@@ -153,6 +156,127 @@ static int adj_stage(mrn_op *op, int q, float2 *X, bool overwrite, const FrameDa
 // fr: the frame whose samples read the grid (its k-space support), or null
 static int fwd_grid(mrn_op *op, const float2 *X, const FrameData *fr, hipStream_t st) {
   for (int q = 0; q < 3; q++) MRN_TRY(fwd_stage(op, q, X, fr, st));
+  return MRN_OK;
+}
+
+// ---------------------------------------------------- every frame at once
+// A multi-frame op (batched form) keeps the frames' grids side by side in
+// op->grid (op->fuse), so its solver loops run each stage once over every
+// frame: the FFT passes take frame * nch + coil block as their z index, the
+// gather and scatter read each frame's arrays from a table on the device
+// (op->fdev).  fused_prepare builds the table, the frames' k-space work
+// arrays and their supports at a fixed stride; it runs before any capture
+// and again when the trajectories, data or settings change (op->version).
+static bool frames_fused(const mrn_op *op) {
+  return op->fuse && op->fdev && op->fdev_version == op->version && op->gather_tiles == 0;
+}
+
+static int fused_prepare(mrn_op *op, hipStream_t st) {
+  if (!op->fuse || op->gather_tiles != 0) return MRN_OK;
+  if (op->fdev && op->fdev_version == op->version) return MRN_OK;
+  const Geom &g = op->g;
+  const int F = op->F;
+  for (int f = 0; f < F; f++)
+    if (!op->frames[f].has_traj) return MRN_OK;  // not ready: frame by frame
+  int64_t tot = 0;
+  for (int f = 0; f < F; f++) tot += op->frames[f].Mv * g.CC * op->nch;
+  if (tot > op->kwork_all_cap || !op->kwork_all) {
+    hipFree(op->kwork_all);
+    op->kwork_all = nullptr;
+    op->kwork_all_cap = 0;
+    MRN_HIP(op, hipMalloc(&op->kwork_all, sizeof(float2) * (tot > 0 ? tot : 1)));
+    op->kwork_all_cap = tot;
+  }
+  if (!op->fdev) MRN_HIP(op, hipMalloc(&op->fdev, sizeof(FrameDev) * F));
+  const int64_t nrow = (int64_t)g.S[1] * g.S[2];
+  if (fft_x_pruned(op)) {
+    if (!op->xrng_all) MRN_HIP(op, hipMalloc(&op->xrng_all, sizeof(int2) * nrow * F));
+    for (int f = 0; f < F; f++)
+      MRN_HIP(op, hipMemcpyAsync(op->xrng_all + f * nrow, op->frames[f].xrng, sizeof(int2) * nrow,
+                                 hipMemcpyDeviceToDevice, st));
+  }
+  if (fft_xy_pruned(op)) {
+    if (!op->yrng_all) MRN_HIP(op, hipMalloc(&op->yrng_all, sizeof(int2) * g.S[2] * F));
+    for (int f = 0; f < F; f++)
+      MRN_HIP(op, hipMemcpyAsync(op->yrng_all + (int64_t)f * g.S[2], op->frames[f].yrng, sizeof(int2) * g.S[2],
+                                 hipMemcpyDeviceToDevice, st));
+  }
+  std::vector<FrameDev> h(F);
+  int64_t off = 0;
+  op->fmax_nsub = op->fmax_nheavy = op->fmax_nwide = 0;
+  op->fmax_Mv = 0;
+  for (int f = 0; f < F; f++) {
+    const FrameData &fr = op->frames[f];
+    FrameDev &d = h[f];
+    d.coords = fr.coords;
+    d.taper = fr.taper;
+    d.subs = fr.subs;
+    d.vofs = fr.vofs;
+    d.vlist = fr.vlist;
+    d.heavy = fr.heavy;
+    d.wide = fr.wide;
+    d.kwork = op->kwork_all + off;
+    d.kdata = fr.kdata;
+    d.Mv = fr.Mv;
+    d.grid_off = (int64_t)f * op->nch * ncell(op) * g.CC;
+    d.nsub = fr.nsub;
+    d.nheavy = fr.nheavy;
+    d.nwide = fr.nwide;
+    d.pad = 0;
+    off += fr.Mv * g.CC * op->nch;
+    op->fmax_nsub = std::max(op->fmax_nsub, fr.nsub);
+    op->fmax_nheavy = std::max(op->fmax_nheavy, fr.nheavy);
+    op->fmax_nwide = std::max(op->fmax_nwide, fr.nwide);
+    op->fmax_Mv = std::max(op->fmax_Mv, fr.Mv);
+  }
+  MRN_HIP(op, hipMemcpyAsync(op->fdev, h.data(), sizeof(FrameDev) * F, hipMemcpyHostToDevice, st));
+  MRN_HIP(op, hipStreamSynchronize(st));  // h is freed on return
+  op->fdev_version = op->version;
+  return MRN_OK;
+}
+
+// image frames X (F x nvox) -> every frame's full k-space grid
+static int fwd_grid_frames(mrn_op *op, const float2 *X, hipStream_t st) {
+  const Geom &g = op->g;
+  PrePostArgs a = prepost(op, 0, 0, g.CC);
+  a.nf = op->F;
+  a.xfs = op->nvox;
+  MRN_TRY(fft_pre_z(op, op->grid, X, op->smaps, a, st));
+  LinesArgs y = ypass_args(op, false, nullptr);
+  y.rng = fft_xy_pruned(op) ? op->yrng_all : nullptr;
+  y.rng_nz = op->nch;
+  y.rng_fs = g.S[2];
+  MRN_TRY(fft_lines(op, g.S[1], -1, op->grid, y, dim3(cdiv(y.nlines, FFT_LINES), g.S[2], op->nch * op->F),
+                    op->tw_d[1], st));
+  LinesArgs x = xpass_args(op, false, nullptr);
+  x.rng = fft_x_pruned(op) ? op->xrng_all : nullptr;
+  x.rng_nz = op->nch;
+  x.rng_fs = (int64_t)g.S[1] * g.S[2];
+  return fft_lines(op, g.S[0], -1, op->grid, x, dim3(cdiv(x.nlines, FFT_LINES), g.S[2], op->nch * op->F),
+                   op->tw_d[0], st);
+}
+
+// every frame's grids -> image frames X (= or += sum over coils)
+static int adj_grid_frames(mrn_op *op, float2 *X, bool overwrite, hipStream_t st) {
+  const Geom &g = op->g;
+  LinesArgs x = xpass_args(op, true, nullptr);
+  x.rng = fft_x_pruned(op) ? op->xrng_all : nullptr;
+  x.rng_nz = op->nch;
+  x.rng_fs = (int64_t)g.S[1] * g.S[2];
+  MRN_TRY(fft_lines(op, g.S[0], 1, op->grid, x, dim3(cdiv(x.nlines, FFT_LINES), g.S[2], op->nch * op->F),
+                    op->tw_d[0], st));
+  LinesArgs y = ypass_args(op, true, nullptr);
+  y.rng = fft_xy_pruned(op) ? op->yrng_all : nullptr;
+  y.rng_nz = op->nch;
+  y.rng_fs = g.S[2];
+  MRN_TRY(fft_lines(op, g.S[1], 1, op->grid, y, dim3(cdiv(y.nlines, FFT_LINES), g.S[2], op->nch * op->F),
+                    op->tw_d[1], st));
+  for (int ch = 0; ch < op->nch; ch++) {
+    PrePostArgs a = prepost(op, ch, overwrite && ch == 0, g.CC);
+    a.nf = op->F;
+    a.xfs = op->nvox;
+    MRN_TRY(fft_post_z(op, op->grid, X, op->smaps, a, st));
+  }
   return MRN_OK;
 }
 
@@ -299,6 +423,31 @@ static int grad_frame(mrn_op *op, int f, const float2 *X, float2 *R, hipStream_t
     MRN_TRY(plain_AH_coil(op, R, sm, op->kplain, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2), traw(fr, 3), st));
   }
   return MRN_OK;
+}
+
+// out = E'E X over every frame (frames of nvox, side by side): one launch per
+// stage over all frames when they are fused, else frame by frame
+static int normal_frames(mrn_op *op, const float2 *X, float2 *out, hipStream_t st) {
+  if (!frames_fused(op)) {
+    for (int f = 0; f < op->F; f++) MRN_TRY(normal_frame(op, f, X + f * op->nvox, out + f * op->nvox, st));
+    return MRN_OK;
+  }
+  MRN_TRY(fwd_grid_frames(op, X, st));
+  MRN_TRY(launch_gather_frames(op, 0, st));
+  MRN_TRY(launch_scatter_frames(op, st));
+  return adj_grid_frames(op, out, true, st);
+}
+
+// R = E^H W (E X - d) over every frame, as normal_frames
+static int grad_frames(mrn_op *op, const float2 *X, float2 *R, hipStream_t st) {
+  if (!frames_fused(op)) {
+    for (int f = 0; f < op->F; f++) MRN_TRY(grad_frame(op, f, X + f * op->nvox, R + f * op->nvox, st));
+    return MRN_OK;
+  }
+  MRN_TRY(fwd_grid_frames(op, X, st));
+  MRN_TRY(launch_gather_frames(op, 1, st));
+  MRN_TRY(launch_scatter_frames(op, st));
+  return adj_grid_frames(op, R, true, st);
 }
 
 // R = E^H W d of frame f (R overwritten)
@@ -549,8 +698,7 @@ int mrn_coil_forward_sos(mrn_op *op, void *X, const void *data, int64_t M, const
 static int cg_step(mrn_op *op, float2 *X, bool with_trace, hipStream_t s) {
   const int64_t n = (int64_t)op->F * op->nvox;
   double *tr = with_trace ? op->dtrace : nullptr;
-  for (int f = 0; f < op->F; f++)
-    MRN_TRY(normal_frame(op, f, op->vP + f * op->nvox, op->vL + f * op->nvox, s));
+  MRN_TRY(normal_frames(op, op->vP, op->vL, s));
   MRN_TRY(op_cg_alpha(op, op->vR, op->vP, op->vL, n, tr, s));
   if (!batched(op)) {  // plain form: the reference's host-side scalar round trip
     double host_scal[16];
@@ -571,6 +719,7 @@ int mrn_cg_init(mrn_op *op, void *Xv, void *stream) {
   for (int f = 0; f < op->F; f++) MRN_TRY(check_frame(op, f, true));
   MRN_TRY(ensure_solver(op));
   hipStream_t st = pick(op, stream);
+  MRN_TRY(fused_prepare(op, st));
   const int64_t n = (int64_t)op->F * op->nvox;
   MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
   for (int f = 0; f < op->F; f++) MRN_TRY(adjoint_data_frame(op, f, op->vR + f * op->nvox, st));
@@ -585,6 +734,7 @@ int mrn_cg_iterate(mrn_op *op, void *Xv, int iters, int use_graph, double *trace
   if (!op || !Xv || iters < 0) return MRN_EINVAL;
   if (!op->cg_ready) return fail(op, MRN_ESTATE, "mrn_cg_init not called");
   hipStream_t st = pick(op, stream);
+  MRN_TRY(fused_prepare(op, st));  // before any capture
   float2 *X = (float2 *)Xv;
   const int64_t first = op->cg_count;
   if (use_graph && batched(op) && iters > 0) {
@@ -632,10 +782,10 @@ int mrn_power_method(mrn_op *op, int iters, double *step, void *stream) {
   hipStream_t st = pick(op, stream);
   const int64_t n = (int64_t)op->F * op->nvox;
   MRN_HIP(op, hipMemsetAsync(op->dcounter, 0, sizeof(int), st));
+  MRN_TRY(fused_prepare(op, st));
   MRN_TRY(op_fill(op, op->vX0, n, make_float2(-0.5f, -0.5f), st));
   for (int it = 0; it < iters; it++) {
-    for (int f = 0; f < op->F; f++)
-      MRN_TRY(normal_frame(op, f, op->vX0 + f * op->nvox, op->vR + f * op->nvox, st));
+    MRN_TRY(normal_frames(op, op->vX0, op->vR, st));
     MRN_TRY(op_power_step(op, op->vX0, op->vR, n, op->dtrace, st));
   }
   double sc[16];
@@ -660,14 +810,13 @@ int mrn_fista(mrn_op *op, void *Xv, int iters, double step, int use_graph, doubl
   const bool cauchy = op->step_type == MRN_STEP_CAUCHY;
   if (l1) MRN_TRY(ensure_l1_state(op));  // before any capture
   MRN_TRY(llr_fista_prepare(op));
+  MRN_TRY(fused_prepare(op, st));
   if (cauchy) MRN_TRY(op_set_step(op, stf, 0.0f, st));  // the step until the first Cauchy update
   auto body = [&](hipStream_t s) -> int {
     MRN_TRY(op_fista_momentum(op, X, op->vP, n, s));
-    for (int f = 0; f < op->F; f++)
-      MRN_TRY(grad_frame(op, f, X + f * op->nvox, op->vR + f * op->nvox, s));
+    MRN_TRY(grad_frames(op, X, op->vR, s));
     if (cauchy) {  // P = E'E R per frame, step = sum|R|^2 / sum conj(R) P (recon_lib.cpp:2013-2041)
-      for (int f = 0; f < op->F; f++)
-        MRN_TRY(normal_frame(op, f, op->vR + f * op->nvox, op->vL + f * op->nvox, s));
+      MRN_TRY(normal_frames(op, op->vR, op->vL, s));
       MRN_TRY(op_cauchy_step(op, op->vR, op->vL, n, s));
     }
     MRN_TRY(op_fista_finish(op, X, op->vR, n, stf, op->dtrace, s, cauchy));

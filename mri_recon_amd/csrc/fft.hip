@@ -324,7 +324,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z(
   __shared__ float2 buf[FFT_LINES * (S + FFT_PAD)];
   __shared__ float2 tws[S];
   load_twiddles<S>(tws, tw);
-  const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
+  // frame and coil block of this workgroup
+  const int f = blockIdx.z / a.nchf, CC = a.CC, y = blockIdx.y, ch = blockIdx.z - f * a.nchf;
+  X += f * a.xfs;
+  grid += (int64_t)f * a.nchf * a.chunk_grid;
   // runs of consecutive workgroups (x columns) per XCD: neighbouring 128-byte
   // rows of the grid and of X share an L2
   const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
@@ -394,6 +397,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
   __shared__ float2 tws[S];
   load_twiddles<S>(tws, tw);
   const int CC = a.CC, y = blockIdx.y, ch = a.ch;
+  X += blockIdx.z * a.xfs;  // frame blockIdx.z
+  grid += (int64_t)blockIdx.z * a.nchf * a.chunk_grid;
   const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);  // as in k_fft_pre_z
   const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   constexpr int NJ = (S + 15) / 16;
@@ -572,7 +577,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
   float2 *ln = grid + base + (int64_t)r * a.rstride + cc;
   int ld0 = a.ld0, ld1 = a.ld1, st0 = a.st0, st1 = a.st1;
   if (a.rng && ok) {  // k-space support of this line's row
-    const int2 rr = a.rng[(int64_t)blockIdx.y * a.rng_zs + (int64_t)r * a.rng_rs];
+    const int64_t rf = a.rng_fs ? (int64_t)(blockIdx.z / a.rng_nz) * a.rng_fs : 0;  // frame's support
+    const int2 rr = a.rng[rf + (int64_t)blockIdx.y * a.rng_zs + (int64_t)r * a.rng_rs];
     if (a.rng_load) {
       ld0 = max(ld0, rr.x);
       ld1 = min(ld1, rr.y + 1);
@@ -619,7 +625,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
   __shared__ float2 tws[S];
   __shared__ float wzs[S];
   float2 *buf = reinterpret_cast<float2 *>(buf4);
-  const int CC = a.CC, y = blockIdx.y, ch = blockIdx.z;
+  // frame and coil block of this workgroup
+  const int f = blockIdx.z / a.nchf, CC = a.CC, y = blockIdx.y, ch = blockIdx.z - f * a.nchf;
+  X += f * a.xfs;
+  grid += (int64_t)f * a.nchf * a.chunk_grid;
   const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
   const int nl = a.N[0] * CC;
   const int ogz = a.og[2];
@@ -714,6 +723,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
   __shared__ float wzs[S];
   float2 *buf = reinterpret_cast<float2 *>(buf4);
   const int CC = a.CC, y = blockIdx.y, ch = a.ch;
+  X += blockIdx.z * a.xfs;  // frame blockIdx.z
+  grid += (int64_t)blockIdx.z * a.nchf * a.chunk_grid;
   const int bx = xcd_block(blockIdx.x, gridDim.x, a.xrun);
   const int nl = a.N[0] * CC, g0 = bx * FFT_LINES;
   const int nxl = FFT_LINES / CC;
@@ -868,7 +879,7 @@ int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3
 
 int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
               const PrePostArgs &a, hipStream_t st) {
-  dim3 g3((a.N[0] * a.CC + FFT_LINES - 1) / FFT_LINES, a.N[1], op->nch);
+  dim3 g3((a.N[0] * a.CC + FFT_LINES - 1) / FFT_LINES, a.N[1], op->nch * a.nf);
   switch (a.S[2]) {
 #define MRN_CASE_PRE(n)                                                                    \
   case n:                                                                                  \
@@ -893,7 +904,7 @@ int fft_pre_z(mrn_op *op, float2 *grid, const float2 *X, const float2 *smaps,
 
 int fft_post_z(mrn_op *op, const float2 *grid, float2 *X, const float2 *smaps,
                const PrePostArgs &a, hipStream_t st) {
-  dim3 g3((a.N[0] * a.CC + FFT_LINES - 1) / FFT_LINES, a.N[1], 1);
+  dim3 g3((a.N[0] * a.CC + FFT_LINES - 1) / FFT_LINES, a.N[1], a.nf);
   switch (a.S[2]) {
 #define MRN_CASE_POST(n)                                                                   \
   case n:                                                                                  \

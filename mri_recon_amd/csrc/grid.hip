@@ -293,10 +293,20 @@ __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
                                                 int64_t chunk_grid, int64_t chunk_k,
                                                 const float *__restrict__ taper, int weighted,
                                                 int xrun, const int32_t *__restrict__ list,
-                                                int64_t nlist) {
+                                                int64_t nlist, const FrameDev *__restrict__ fdt, int fsub) {
   extern __shared__ float tab_s[];
   load_tables(tab_s, tables, g.ntab);
   __syncthreads();
+  if (fdt) {  // every frame at once: frame blockIdx.z (sub = its data when fsub)
+    const FrameDev &fd = fdt[blockIdx.z];
+    grid += fd.grid_off;
+    out = fd.kwork;
+    coords = fd.coords;
+    sub = fsub ? fd.kdata : nullptr;
+    taper = fd.taper;
+    Mv = fd.Mv;
+    chunk_k = fd.Mv * g.CC;
+  }
   const int64_t gid = (int64_t)xcd_block(blockIdx.x, gridDim.x, xrun) * blockDim.x + threadIdx.x;
   constexpr int LGNC = NC == 8 ? 3 : NC == 4 ? 2 : NC == 2 ? 1 : 0;
   const int lgl = g.lgCC - LGNC;  // log2 of the lanes per sample
@@ -1048,7 +1058,13 @@ static size_t gather_mm_lds(int nt) {
 
 // ------------------------------------------------------------ scatter
 __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Geom g, Tiles t,
-                             int64_t chunk_grid) {
+                             int64_t chunk_grid, const FrameDev *__restrict__ fdt) {
+  if (fdt) {  // every frame at once: frame blockIdx.z
+    const FrameDev &fd = fdt[blockIdx.z];
+    if ((int)blockIdx.x >= fd.nheavy) return;
+    grid += fd.grid_off;
+    tiles = fd.heavy;
+  }
   const int tile = tiles[blockIdx.x];
   const int ch = blockIdx.y;
   const int tx0 = (tile % t.nt[0]) * t.T, ty0 = ((tile / t.nt[0]) % t.nt[1]) * t.T,
@@ -1089,7 +1105,8 @@ template <int CCT, int KM>
 __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
-    Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum) {
+    Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum,
+    const FrameDev *__restrict__ fdt) {
   static_assert(CCT == 8 || CCT == 16, "matrix-core scatter needs 8 or 16 coils per cell");
   constexpr int NB = CCT / 8;            // 16-real blocks of a cell's coil values
   constexpr int NG = SM_BATCH / 4;       // sample groups of a batch
@@ -1101,6 +1118,17 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  if (fdt) {  // every frame at once: frame blockIdx.z
+    const FrameDev &fd = fdt[blockIdx.z];
+    grid += fd.grid_off;
+    kin = fd.kwork;
+    coords = fd.coords;
+    vofs = fd.vofs;
+    vlist = fd.vlist;
+    subs = fd.subs;
+    nsub = fd.nsub;
+    chunk_k = fd.Mv * g.CC;
+  }
   const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
   if (sidx >= nsub) return;
   const int4 sp = subs[sidx];
@@ -1256,7 +1284,8 @@ template <int CCT>
 __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
-    Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum) {
+    Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum,
+    const FrameDev *__restrict__ fdt) {
   __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
   __shared__ float2 s_d[4][SG_BATCH * CCT];  // [sample][coil] data
   extern __shared__ float tab_s[];           // kernel tables (g.ntab floats)
@@ -1264,6 +1293,17 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  if (fdt) {  // every frame at once: frame blockIdx.z
+    const FrameDev &fd = fdt[blockIdx.z];
+    grid += fd.grid_off;
+    kin = fd.kwork;
+    coords = fd.coords;
+    vofs = fd.vofs;
+    vlist = fd.vlist;
+    subs = fd.subs;
+    nsub = fd.nsub;
+    chunk_k = fd.Mv * g.CC;
+  }
   const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
   if (sidx >= nsub) return;
   const int4 sp = subs[sidx];
@@ -1367,7 +1407,17 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
 // edge taps per wide axis (25 of 150 taps for W = 5 on one axis).
 __global__ void k_scatter_wide(float2 *grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
                                const int32_t *__restrict__ wide, int nwide, Geom g, int NTb,
-                               const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k) {
+                               const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k,
+                               const FrameDev *__restrict__ fdt) {
+  if (fdt) {  // every frame at once: frame blockIdx.z
+    const FrameDev &fd = fdt[blockIdx.z];
+    grid += fd.grid_off;
+    kin = fd.kwork;
+    coords = fd.coords;
+    wide = fd.wide;
+    nwide = fd.nwide;
+    chunk_k = fd.Mv * g.CC;
+  }
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)nwide * g.CC) return;
   const int c = (int)(idx & (g.CC - 1));
@@ -1588,9 +1638,11 @@ static int launch_gather_box(mrn_op *op, const float2 *grid, float2 *out, const 
 }
 
 // Direct gather over all samples, or over the samples listed in `list`.
+// fdt: every frame of the op at once (op->fdev; out, sub from the table, sub
+// when fsub), grid dimension z = frame.
 static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameData &fr,
                                 const float2 *sub, int wflag, const int32_t *list, int64_t nlist,
-                                hipStream_t st) {
+                                hipStream_t st, const FrameDev *fdt = nullptr, int fsub = 0) {
   const Geom &g = op->g;
   // coils per lane: 4 by default.  A wave then serves 16 samples (4 lanes
   // each) instead of 8, whose taps overlap more, so fewer of its loads miss
@@ -1598,16 +1650,16 @@ static int launch_gather_direct(mrn_op *op, const float2 *grid, float2 *out, con
   // lane, 5.7 with 2, 7.5 with 8)
   int nc = op->gather_coils > 0 ? op->gather_coils : 4;
   if (nc > g.CC) nc = g.CC;
-  const int64_t n = list ? nlist : fr.Mv;
+  const int64_t n = fdt ? op->fmax_Mv : list ? nlist : fr.Mv;
   if (n == 0) return MRN_OK;
-  dim3 g3(nblk(n * (g.CC / nc), 256), op->nch);
+  dim3 g3(nblk(n * (g.CC / nc), 256), op->nch, fdt ? op->F : 1);
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
   switch (taps_main(g) * 16 + nc) {
 #define MRN_G1(n, c)                                                                                \
   case n * 16 + c:                                                                                  \
     hipLaunchKernelGGL((k_gather<n, c>), g3, dim3(256), sizeof(float) * g.ntab, st, grid, out,     \
                        fr.coords, sub, fr.Mv, g, op->tables_d, cg, ck, fr.taper, wflag,            \
-                       op->t.xrun_gather, list, nlist);                                             \
+                       op->t.xrun_gather, list, nlist, fdt, fsub);                                  \
     break;
 #define MRN_G(n) MRN_G1(n, 1) MRN_G1(n, 2) MRN_G1(n, 4) MRN_G1(n, 8)
     MRN_G(1) MRN_G(2) MRN_G(3) MRN_G(4) MRN_G(5) MRN_G(6) MRN_G(7) MRN_G(8)
@@ -1661,13 +1713,21 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
   return launch_gather_direct(op, grid, out, fr, sub, wf, nullptr, 0, st);
 }
 
+// Gather of every frame at once (the frames' grids side by side in op->grid,
+// the results in their kwork arrays), the data subtracted when fsub.
+int launch_gather_frames(mrn_op *op, int fsub, hipStream_t st) {
+  return launch_gather_direct(op, op->grid, nullptr, op->frames[0], nullptr, 1, nullptr, 0, st, op->fdev, fsub);
+}
+
 // Zero the tiles that several work items flush with atomics (part of the scatter).
-int launch_scatter_zero(mrn_op *op, float2 *grid, const mrn::FrameData &fr, hipStream_t st) {
+int launch_scatter_zero(mrn_op *op, float2 *grid, const mrn::FrameData &fr, hipStream_t st,
+                        const FrameDev *fdt) {
   const Geom &g = op->g;
-  if (fr.nheavy > 0) {
+  const int nh = fdt ? op->fmax_nheavy : fr.nheavy;
+  if (nh > 0) {
     const int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC;
-    hipLaunchKernelGGL(k_zero_tiles, dim3(fr.nheavy, op->nch), dim3(256), 0, st, grid, fr.heavy, g,
-                       op->t, cg);
+    hipLaunchKernelGGL(k_zero_tiles, dim3(nh, op->nch, fdt ? op->F : 1), dim3(256), 0, st, grid, fr.heavy, g,
+                       op->t, cg, fdt);
     MRN_HIP(op, hipGetLastError());
   }
   return MRN_OK;
@@ -1677,24 +1737,28 @@ int launch_scatter_zero(mrn_op *op, float2 *grid, const mrn::FrameData &fr, hipS
 // accum (add into the grid's contents: tiles split over several work items
 // add with atomics, the others read, add and store).
 int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn::FrameData &fr,
-                         hipStream_t st, bool accum) {
+                         hipStream_t st, bool accum, const FrameDev *fdt) {
   const int af = accum ? 1 : 0;
   const Geom &g = op->g;
   const Tiles &t = op->t;
+  // fdt: every frame at once (grid dimension z = frame; the frames' largest
+  // work lists size the grid, each frame's kernel reads its own)
+  const int nsub = fdt ? op->fmax_nsub : fr.nsub, nwide = fdt ? op->fmax_nwide : fr.nwide;
+  const int64_t Mv = fdt ? op->fmax_Mv : fr.Mv;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
-  dim3 g3((fr.nsub + 3) / 4, op->nch);
+  dim3 g3((nsub + 3) / 4, op->nch, fdt ? op->F : 1);
   const size_t sh = sizeof(float) * g.ntab;  // kernel tables in LDS
   // no work item: a frame without valid samples whose x passes are limited to
   // its (empty) k-space support -- nothing reads the grid
-  if (fr.nsub == 0) return MRN_OK;
+  if (nsub == 0) return MRN_OK;
   const bool exact = g.kmode == MRN_KMODE_EXACT_KB;
   // the matrix-core form addresses a coil block's k-space and the coordinates
   // with 32-bit byte offsets
-  const bool off32 = fr.Mv * g.CC * 8 < (1ll << 32) && fr.Mv * 16 < (1ll << 32);
+  const bool off32 = Mv * g.CC * 8 < (1ll << 32) && Mv * 16 < (1ll << 32);
   if ((g.CC == 16 || g.CC == 8) && !op->scatter_valu && off32) {
 #define MRN_MM(n, km)                                                                                  \
   hipLaunchKernelGGL((k_scatter_mm<n, km>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,      \
-                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af)
+                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt)
     if (g.CC == 16 && !exact) MRN_MM(16, 0);
     else if (g.CC == 16) MRN_MM(16, 1);
     else if (!exact) MRN_MM(8, 0);
@@ -1705,7 +1769,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
     hipLaunchKernelGGL((k_scatter_gd<n>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,  \
-                       fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af);              \
+                       fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt);              \
     break;
     MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)
 #undef MRN_S
@@ -1714,9 +1778,9 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
     }
   }
   MRN_HIP(op, hipGetLastError());
-  if (fr.nwide > 0) {  // after the tiles' plain stores
-    hipLaunchKernelGGL(k_scatter_wide, dim3(nblk((int64_t)fr.nwide * g.CC, 256), op->nch), dim3(256), 0, st, grid,
-                       kin, fr.coords, fr.wide, fr.nwide, g, t.NTb, op->tables_d, cg, ck);
+  if (nwide > 0) {  // after the tiles' plain stores
+    hipLaunchKernelGGL(k_scatter_wide, dim3(nblk((int64_t)nwide * g.CC, 256), op->nch, fdt ? op->F : 1), dim3(256), 0,
+                       st, grid, kin, fr.coords, fr.wide, fr.nwide, g, t.NTb, op->tables_d, cg, ck, fdt);
     MRN_HIP(op, hipGetLastError());
   }
   return MRN_OK;
@@ -1726,6 +1790,12 @@ int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const mrn::Frame
                    hipStream_t st, bool accum) {
   if (!accum) MRN_TRY(launch_scatter_zero(op, grid, fr, st));
   return launch_scatter_tiles(op, grid, kin, fr, st, accum);
+}
+
+// Scatter of every frame at once, from their kwork arrays into their grids.
+int launch_scatter_frames(mrn_op *op, hipStream_t st) {
+  MRN_TRY(launch_scatter_zero(op, op->grid, op->frames[0], st, op->fdev));
+  return launch_scatter_tiles(op, op->grid, nullptr, op->frames[0], st, false, op->fdev);
 }
 
 int launch_weight(mrn_op *op, float2 *kout, const float2 *kin, const mrn::FrameData &fr,

@@ -61,6 +61,24 @@ struct Tiles {
   int xrun_gather;  // gather workgroups per XCD run
 };
 
+// One frame's device arrays for a launch over every frame of the op at once
+// (blockIdx.z = frame): the solver loops of a multi-frame op run one launch
+// per stage, with the frames' grids side by side in op->grid.
+struct FrameDev {
+  const float4 *coords;
+  const float *taper;
+  const int4 *subs;
+  const int64_t *vofs;
+  const int32_t *vlist;
+  const int32_t *heavy;
+  const int32_t *wide;
+  float2 *kwork;           // this frame's [nch][Mv][CC] k-space work array
+  const float2 *kdata;     // measured k-space, same layout
+  int64_t Mv;
+  int64_t grid_off;        // first element of this frame's coil blocks in the grid
+  int nsub, nheavy, nwide, pad;
+};
+
 struct FrameData {
   int64_t M = 0;               // samples (caller order)
   int64_t Mv = 0;              // valid samples
@@ -105,6 +123,18 @@ struct mrn_op {
   float2 *cgrid = nullptr;     // plain form: one coil grid [Sz][Sy][Sx]
   float2 *kwork = nullptr;     // [nch][Mv_max][CC]
   int64_t kwork_cap = 0;       // samples
+  // frames fused into one launch per stage (batched form, F > 1): op->grid
+  // holds F frames of nch coil blocks; fdev / kwork_all / supports are set up
+  // by fused_prepare for op->version fdev_version
+  bool fuse = false;
+  mrn::FrameDev *fdev = nullptr;  // [F] on the device
+  int fdev_version = -1;
+  float2 *kwork_all = nullptr; // frames' k-space work arrays, frame after frame
+  int64_t kwork_all_cap = 0;   // elements
+  int2 *xrng_all = nullptr;    // [F][Sz][Sy] x support rows (k_row_xrange) of every frame
+  int2 *yrng_all = nullptr;    // [F][Sz] y support of every frame, when the y passes are pruned
+  int fmax_nsub = 0, fmax_nheavy = 0, fmax_nwide = 0;
+  int64_t fmax_Mv = 0;
   float2 *kplain = nullptr;    // plain form: one coil of k-space [M_max]
   int64_t kplain_cap = 0;
   std::vector<mrn::FrameData> frames;
@@ -341,6 +371,10 @@ struct LinesArgs {
   // inverse line's stores (no later pass reads that row).
   const int2 *rng;
   int rng_zs, rng_rs, rng_store, rng_load;
+  // launches over several frames (blockIdx.z = frame * rng_nz + coil block):
+  // frame f's support at rng + f * rng_fs (one frame: rng_fs = 0)
+  int rng_nz;
+  int64_t rng_fs;
 };
 
 struct PrePostArgs {
@@ -352,6 +386,11 @@ struct PrePostArgs {
   int overwrite;        // post: X = sum instead of X += sum
   float scale;
   int xrun;             // x-column workgroups per XCD run (xcd_block)
+  // launches over several frames: pre blockIdx.z = frame * nchf + coil
+  // block, post blockIdx.z = frame; the frame's image at X + frame * xfs and
+  // its grids at coil blocks frame * nchf ... (one frame: nf = 1, xfs = 0)
+  int nf, nchf;
+  int64_t xfs;
 };
 
 // taps per axis of the unrolled gather and of the tiles' bin reach: 2 dwin
@@ -367,9 +406,14 @@ int launch_weight(mrn_op *op, float2 *kout, const float2 *kin, const FrameData &
 // or with accum add into the grid's contents
 int launch_scatter(mrn_op *op, float2 *grid, const float2 *kin, const FrameData &fr, hipStream_t st,
                    bool accum = false);
-int launch_scatter_zero(mrn_op *op, float2 *grid, const FrameData &fr, hipStream_t st);
+int launch_scatter_zero(mrn_op *op, float2 *grid, const FrameData &fr, hipStream_t st,
+                        const FrameDev *fdt = nullptr);
 int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const FrameData &fr, hipStream_t st,
-                         bool accum = false);
+                         bool accum = false, const FrameDev *fdt = nullptr);
+// every frame at once (op->fdev set up): gather from / scatter into the
+// frames' grids, k-space in their kwork arrays
+int launch_gather_frames(mrn_op *op, int fsub, hipStream_t st);
+int launch_scatter_frames(mrn_op *op, hipStream_t st);
 int fft_lines(mrn_op *op, int S, int dir, float2 *grid, const LinesArgs &a, dim3 grid3,
               const float2 *tw, hipStream_t st);
 bool fft_x_pruned(const mrn_op *op);

@@ -166,6 +166,7 @@ void mrn_params_default(mrn_params *p) {
   p->nframes = 1;
   p->kernel_mode = MRN_KMODE_TABLE;
   p->form = MRN_FORM_BATCHED;
+  p->fuse_frames = -1;
 }
 
 int mrn_parse_args(mrn_params *p, int argc, const char **argv) {
@@ -361,13 +362,21 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   }
   const int64_t ncell = (int64_t)g.S[0] * g.S[1] * g.S[2];
   if (batched) {
-    if (amalloc((void **)&op->grid, sizeof(float2) * ncell * g.CC * op->nch) != hipSuccess)
-      return cleanup_fail(MRN_ENOMEM);
+    // frames fused into one launch per stage keep F grids side by side:
+    // forced (fuse_frames > 0), off (0), or when they take at most half the
+    // free device memory (-1)
+    op->fuse = false;
+    if (op->F > 1 && p.fuse_frames != 0) {
+      const double bytes = (double)sizeof(float2) * ncell * g.CC * op->nch * op->F;
+      size_t fm = 0, tm = 0;
+      op->fuse = p.fuse_frames > 0 || (hipMemGetInfo(&fm, &tm) == hipSuccess && bytes <= 0.5 * (double)fm);
+    }
+    const int64_t ngrid = ncell * g.CC * op->nch * (op->fuse ? op->F : 1);
+    if (amalloc((void **)&op->grid, sizeof(float2) * ngrid) != hipSuccess) return cleanup_fail(MRN_ENOMEM);
     // zeroed once: the forward x pass leaves the cells outside a frame's
     // k-space support as they are, and readers of whole tile regions weight
     // them by zero, so they must hold finite values
-    if (hipMemset(op->grid, 0, sizeof(float2) * ncell * g.CC * op->nch) != hipSuccess)
-      return cleanup_fail(MRN_EHIP);
+    if (hipMemset(op->grid, 0, sizeof(float2) * ngrid) != hipSuccess) return cleanup_fail(MRN_EHIP);
     if (amalloc((void **)&op->smaps, sizeof(float2) * op->nvox * g.CC * op->nch) != hipSuccess)
       return cleanup_fail(MRN_ENOMEM);
   } else {
@@ -407,6 +416,10 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.kraw);
     hipFree(fr.traw);
   }
+  hipFree(op->fdev);
+  hipFree(op->kwork_all);
+  hipFree(op->xrng_all);
+  hipFree(op->yrng_all);
   hipFree(op->tables_d);
   for (int a = 0; a < 3; a++) {
     hipFree(op->win_d[a]);

@@ -78,7 +78,7 @@ class GridFFT:
                  grid_in=(1, 1, 1), fft_in=(1, 1, 1), grid=-1.0, grid_scale=1.0, k_rad=9999.0,
                  kernel_mode="table", form="batched", coil_chunk=0, tile=0, max_visits=0,
                  bin_edge=0, xcd_run=0, xcd_run_gather=0, gather_tiles=False, scatter_valu=False, fft_stockham=False, xcd_run_fft=0,
-                 gather_coils=0, args=None):
+                 gather_coils=0, fuse_frames=-1, args=None):
         L = lib()
         p = MrnParams()
         L.mrn_params_default(ctypes.byref(p))
@@ -108,6 +108,7 @@ class GridFFT:
         p.fft_stockham = int(bool(fft_stockham))
         p.xcd_run_fft = int(xcd_run_fft)
         p.gather_coils = int(gather_coils)
+        p.fuse_frames = int(fuse_frames)
         if args:  # gridFFT::read_commandline flags
             arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
             check(L.mrn_parse_args(ctypes.byref(p), len(args), arr))
