@@ -1681,15 +1681,25 @@ int launch_gather(mrn_op *op, const float2 *grid, float2 *out, const mrn::FrameD
   const int wf = weighted ? 1 : 0;
   const int nt = taps_main(g);
   const size_t lds = gather_tile_lds(op, nt);
-  if (op->gather_tiles == 2 && g.CC == 16 && op->t.T == 4 && fr.ngsub > 0 && nt >= 3 && nt <= 5) {
+  if (op->gather_tiles == 2 && g.CC == 16 && op->t.T == 4 && fr.ngsub > 0 && nt >= 3 && nt <= 5 && op->side) {
+    // the box gather (matrix cores, LDS) of the dense tiles on the side
+    // stream, beside the direct gather (texture path) of the sparse ones:
+    // the two kernels are bound by different units.  Fork and join by events
+    // (captured into a graph as two branches).
+    MRN_HIP(op, hipEventRecord(op->ev_fork, st));
+    MRN_HIP(op, hipStreamWaitEvent(op->side, op->ev_fork, 0));
     int rc = MRN_OK;
     switch (nt) {
-      case 3: rc = launch_gather_box<3>(op, grid, out, fr, sub, wf, st); break;
-      case 4: rc = launch_gather_box<4>(op, grid, out, fr, sub, wf, st); break;
-      default: rc = launch_gather_box<5>(op, grid, out, fr, sub, wf, st); break;
+      case 3: rc = launch_gather_box<3>(op, grid, out, fr, sub, wf, op->side); break;
+      case 4: rc = launch_gather_box<4>(op, grid, out, fr, sub, wf, op->side); break;
+      default: rc = launch_gather_box<5>(op, grid, out, fr, sub, wf, op->side); break;
     }
-    if (rc != MRN_OK) return rc;
-    return fr.nsparse > 0 ? launch_gather_direct(op, grid, out, fr, sub, wf, fr.sparse, fr.nsparse, st) : MRN_OK;
+    if (rc == MRN_OK && fr.nsparse > 0)
+      rc = launch_gather_direct(op, grid, out, fr, sub, wf, fr.sparse, fr.nsparse, st);
+    // join even after an error, so that a capture in progress stays well formed
+    MRN_HIP(op, hipEventRecord(op->ev_join, op->side));
+    MRN_HIP(op, hipStreamWaitEvent(st, op->ev_join, 0));
+    return rc;
   }
   if (lds && fr.ngsub > 0 && op->gather_tiles) {
     int rc = MRN_EUNSUPPORTED;

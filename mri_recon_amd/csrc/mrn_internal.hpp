@@ -139,6 +139,10 @@ struct mrn_op {
   int64_t kplain_cap = 0;
   std::vector<mrn::FrameData> frames;
   hipStream_t stream = nullptr;
+  // box matrix-core gather (gather_tiles = 2): the dense tiles run on `side`
+  // beside the direct gather of the sparse ones (fork / join events)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // solver workspace
   float2 *vR = nullptr, *vP = nullptr, *vL = nullptr, *vX0 = nullptr;
   double *dpart = nullptr;     // reduction partials

@@ -342,6 +342,14 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
     delete op;
     return MRN_EHIP;
   }
+  if (op->gather_tiles == 2) {  // default priority, like op->stream
+    if (hipStreamCreateWithFlags(&op->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&op->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&op->ev_join, hipEventDisableTiming) != hipSuccess) {
+      mrn_op_destroy(op);
+      return MRN_EHIP;
+    }
+  }
   auto cleanup_fail = [&](int code) {
     mrn_op_destroy(op);
     return code;
@@ -448,6 +456,9 @@ int mrn_op_destroy(mrn_op *op) {
   hipFree(op->llr_sel);
   hipFree(op->llr_scal);
   if (op->stream) hipStreamDestroy(op->stream);
+  if (op->side) hipStreamDestroy(op->side);
+  if (op->ev_fork) hipEventDestroy(op->ev_fork);
+  if (op->ev_join) hipEventDestroy(op->ev_join);
   delete op;
   return MRN_OK;
 }
