@@ -131,7 +131,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the DCF / LLR / gating / L1-FISTA timings")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the 128^3 x 8 CG and 160^3 x 24 x 10-frame FISTA+LLR lines")
+                    help="skip the 128^3 x 8 CG, 160^3 x 24 x 10-frame FISTA+LLR, 192^3 x 16 cones gradient "
+                         "and 320^3 x 48 L1-FISTA lines")
     ap.add_argument("--next-only", action="store_true",
                     help="only the DCF / LLR / gating / L1-FISTA timings (no CPU leg; for rocprofv3)")
     ap.add_argument("--profile", action="store_true", help="batched iterations only (for rocprofv3)")
@@ -346,7 +347,9 @@ def main():
         del data, smaps
         torch.cuda.empty_cache()
         res["configs"] = {"sense_cg_128cube_8coil": small_cg(a, log),
-                          "gated_llr_fista_160cube_24coil_10frame": gated_fista(a, log)}
+                          "gated_llr_fista_160cube_24coil_10frame": gated_fista(a, log),
+                          "cones_gradients_192cube_16coil": cones_gradients(a, log),
+                          "l1_fista_320cube_48coil": large_l1_fista(a, log)}
 
     dom_b = b[dom]
     achieved = dom_b / (st[dom] * 1e-3) / 1e9
@@ -505,6 +508,133 @@ def gated_fista(a, log):
                                          "scaled by voxels (%.1f s)" % (tg, C * F, n, F, tl)}
     log("160^3 x 24 x 10 FISTA+LLR: %.1f ms/iter, dominant %s %.2f ms (%.1f%% of HBM)"
         % (per * 1e3, out["roofline"]["stage"], out["roofline"]["avg_ms"], 100 * out["roofline"]["frac"]))
+    op.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def cones_gradients(a, log):
+    """Differentiation workload 192^3 x 16 coils, 8192 x 512 stack-of-cones
+    (grid 288^3, exact Kaiser-Bessel): ms per evaluation of the loss
+    0.5 ||W^1/2 (A(x; k) - y)||^2 with its gradients w.r.t. x and k through
+    SenseEncoding (the trajectory bound anew each evaluation, as in a
+    trajectory optimisation), the operator's dominant stage and the CPU
+    oracle's A and A^H."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import SenseEncoding, synth
+    N, C, seed = 192, 16, 1004
+    kx, ky, kz, kw = synth.cones_traj(8192, 512, N, device="cuda")
+    k = torch.stack([kx, ky, kz], dim=1).contiguous()
+    M = k.shape[0]
+    smaps = synth.coil_maps(C, N, seed=seed, device="cuda")
+    x = synth.cn_image((N, N, N), seed=seed + 1, device="cuda")
+    y = synth.cn_image((C, M), seed=seed + 2, device="cuda")
+    w = torch.clamp(kw / kw.max(), min=0.05)
+    enc = SenseEncoding(N, smaps, overgrid=1.5, kernel_mode="exact_kb")
+
+    def evaluate():
+        xt = x.clone().requires_grad_(True)
+        kt = k.clone().requires_grad_(True)
+        r = enc(xt, kt) - y
+        (0.5 * torch.sum(w * (r.real ** 2 + r.imag ** 2))).backward()
+        return xt.grad, kt.grad
+
+    evaluate()
+    torch.cuda.synchronize()
+    n = 5
+    t0 = time.perf_counter()
+    for _ in range(n):
+        evaluate()
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / n
+    t0 = time.perf_counter()
+    for _ in range(n):
+        enc.set_traj(k)
+    torch.cuda.synchronize()
+    bind = (time.perf_counter() - t0) / n
+    st = enc.op.stage_times(x, reps=3)
+    out = {"workload": "loss 0.5||W^1/2 (A(x;k) - y)||^2 and its gradients in x and k, 192^3 x 16 coils, "
+                       "stack-of-cones 8192 x 512, os 1.5, exact KB W=5 (grid 288^3)",
+           "value": 1.0 / per, "unit": "evaluations/s", "ms_per_step": per * 1e3,
+           "trajectory_bind_ms": bind * 1e3,
+           "roofline": roofline_of(st, stage_bytes(N, C, M, enc.op.S[0])), "stage_ms": st}
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        o32 = orc.OracleOp(N, overgrid=1.5, precision="f32")
+        cpu_fft_setup(o32)
+        th = [t.cpu().numpy() for t in (kx, ky, kz, kw)]
+        sm = smaps[:1].cpu().numpy()
+        xh = x.cpu().numpy()
+        t0 = time.perf_counter()
+        r = o32.image_to_kspace(xh, sm, th)
+        o32.kspace_to_image(r, sm, th)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / (tc * C), "unit": "evaluations/s", "cores": orc.threads(),
+                               "kind": "port", "fft_backend": CPU_FFT_BACKEND,
+                               "sample": "FP32 oracle A then A^H of 1 of 16 coils, %.2f s, x16, with table "
+                                         "taps (the oracle has no exact-KB taps and no trajectory gradient), "
+                                         "so this bounds the CPU evaluation from below" % tc}
+    log("192^3 x 16 cones: %.1f ms per loss + gradients evaluation (trajectory bind %.1f ms)"
+        % (per * 1e3, bind * 1e3))
+    enc.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def large_l1_fista(a, log):
+    """Large L1-wavelet FISTA: 320^3 x 48 coils, 60000 x 320 radial, os 1.5
+    (grid 480^3, 42 GB of coil grids), 50% fraction threshold on 3-level db4
+    with 4 cycle spins: ms per graph-captured iteration, the dominant stage
+    and the CPU oracle on a bounded sample."""
+    import numpy as np
+    import torch
+
+    from mri_recon_amd import GridFFT, synth
+    N, C, seed = 320, 48, 1005
+    traj = synth.radial_traj(60000, 320, N, device="cuda")
+    M = traj[0].numel()
+    smaps = synth.coil_maps(C, N, seed=seed, device="cuda")
+    ph = synth.ellipsoid_phantom(N, seed=seed + 1, device="cuda")
+    op = GridFFT(N, ncoils=C, overgrid=1.5, dwin=2.5)
+    op.set_traj(*traj)
+    op.set_smaps(smaps)
+    op.set_kdata(synth.add_noise(op.backward(ph.contiguous()), 0.01, seed=seed + 2))
+    step = 0.5 * op.power_method(6)
+    op.set_fista_l1(0.5, levels=(3, 3, 3), spins=4, seed=seed)
+    op.fista(1, step, graph=True)
+    torch.cuda.synchronize()
+    tt = []
+    for kk in (1, 3):
+        t0 = time.perf_counter()
+        op.fista(kk, step, graph=True)
+        torch.cuda.synchronize()
+        tt.append(time.perf_counter() - t0)
+    per = (tt[1] - tt[0]) / 2
+    st = op.stage_times(ph.contiguous(), reps=2)
+    out = {"workload": "L1-wavelet FISTA (fraction 0.5, 3-level db4, 4 cycle spins), 320^3 x 48 coils, "
+                       "3-D radial 60000 x 320, os 1.5, KB W=5 (grid 480^3)",
+           "value": 1.0 / per, "unit": "iter/s", "ms_per_step": per * 1e3, "samples_per_coil": M,
+           "roofline": roofline_of(st, stage_bytes(N, C, M, op.S[0])), "stage_ms": st}
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        o32 = orc.OracleOp(N, overgrid=1.5, dwin=2.5, precision="f32")
+        cpu_fft_setup(o32)
+        th = [t.cpu().numpy() for t in traj]
+        sm = smaps[:1].cpu().numpy()
+        P = ph.cpu().numpy()
+        d0 = np.zeros((1, M), np.complex64)
+        t0 = time.perf_counter()
+        r = o32.image_to_kspace(P, sm, th, sub=d0)
+        o32.kspace_to_image(r, sm, th)
+        tg = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / (tg * C), "unit": "iter/s", "cores": orc.threads(), "kind": "port",
+                               "fft_backend": CPU_FFT_BACKEND,
+                               "sample": "FP32 oracle gradient (A - d, then A^H) of 1 of 48 coils, %.1f s, x48; "
+                                         "the L1 step and the vector updates are left out" % tg}
+    log("320^3 x 48 L1 FISTA: %.1f ms/iter, dominant %s %.2f ms" % (per * 1e3, out["roofline"]["stage"],
+                                                                  out["roofline"]["avg_ms"]))
     op.close()
     torch.cuda.empty_cache()
     return out
