@@ -67,7 +67,13 @@ __device__ __forceinline__ float tap_dweight(const Geom &g, int a, const float *
   return d;
 }
 
-// grad_k[m][a] += Re sum_c conj(g_cm) dY_cm/dk_a over the CC coils of a block
+// grad_k[m][a] += Re sum_c conj(g_cm) dY_cm/dk_a over the CC coils of a block.
+// Thread = (sample, coil).  The kernel weights and their derivatives are
+// evaluated once per axis and tap (3 NT of each, in registers), not once per
+// tap of the 3-D stencil: in exact-KB mode each is an I0 or I1 evaluation
+// with an exponential.  A sample with more than NT taps on an axis (exactly on
+// the window edge) takes the loop path.
+template <int NT>
 __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ grid,
                                                    const float2 *__restrict__ gin,
                                                    const float4 *__restrict__ coords,
@@ -86,27 +92,64 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
   const float4 k = coords[i];
   const float dk[3] = {k.x, k.y, k.z};
   int s[3], e[3];
-  for (int a = 0; a < 3; a++) axis_range(g, a, dk[a], s[a], e[a]);
+  bool wide = false;
+  for (int a = 0; a < 3; a++) {
+    axis_range(g, a, dk[a], s[a], e[a]);
+    wide = wide || e[a] - s[a] + 1 > NT;
+  }
   const float2 *G = grid + ch * chunk_grid + c;
   float dr[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
-  for (int lz = s[2]; lz <= e[2]; lz++) {
-    const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
-    const float dz = tap_dweight(g, 2, tab_s + g.toff[2], dk[2], lz);
-    for (int ly = s[1]; ly <= e[1]; ly++) {
-      const float wy = tap_weight(g, 1, tab_s + g.toff[1], dk[1], ly);
-      const float dy = tap_dweight(g, 1, tab_s + g.toff[1], dk[1], ly);
-      const float2 *row = G + ((int64_t)lz * g.S[1] + ly) * g.S[0] * g.CC;
-      for (int lx = s[0]; lx <= e[0]; lx++) {
-        const float wx = tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
-        const float dx = tap_dweight(g, 0, tab_s + g.toff[0], dk[0], lx);
-        const float2 v = row[(int64_t)lx * g.CC];
-        const float q0 = dx * wy * wz, q1 = wx * dy * wz, q2 = wx * wy * dz;
-        dr[0] += q0 * v.x;
-        di[0] += q0 * v.y;
-        dr[1] += q1 * v.x;
-        di[1] += q1 * v.y;
-        dr[2] += q2 * v.x;
-        di[2] += q2 * v.y;
+  if (!wide) {
+    float w[3][NT], d[3][NT];
+    int o[3][NT];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const bool in = s[a] + t <= e[a];
+        const int l = in ? s[a] + t : s[a];
+        w[a][t] = in ? tap_weight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
+        d[a][t] = in ? tap_dweight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
+        o[a][t] = l;
+      }
+#pragma unroll
+    for (int tz = 0; tz < NT; tz++)
+#pragma unroll
+      for (int ty = 0; ty < NT; ty++) {
+        const float2 *row = G + ((int64_t)o[2][tz] * g.S[1] + o[1][ty]) * g.S[0] * g.CC;
+        const float wzy = w[1][ty] * w[2][tz], dzy = w[1][ty] * d[2][tz], wzd = d[1][ty] * w[2][tz];
+#pragma unroll
+        for (int tx = 0; tx < NT; tx++) {
+          const float2 v = row[(int64_t)o[0][tx] * g.CC];
+          const float q0 = d[0][tx] * wzy, q1 = w[0][tx] * wzd, q2 = w[0][tx] * dzy;
+          dr[0] += q0 * v.x;
+          di[0] += q0 * v.y;
+          dr[1] += q1 * v.x;
+          di[1] += q1 * v.y;
+          dr[2] += q2 * v.x;
+          di[2] += q2 * v.y;
+        }
+      }
+  } else {
+    for (int lz = s[2]; lz <= e[2]; lz++) {
+      const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
+      const float dz = tap_dweight(g, 2, tab_s + g.toff[2], dk[2], lz);
+      for (int ly = s[1]; ly <= e[1]; ly++) {
+        const float wy = tap_weight(g, 1, tab_s + g.toff[1], dk[1], ly);
+        const float dy = tap_dweight(g, 1, tab_s + g.toff[1], dk[1], ly);
+        const float2 *row = G + ((int64_t)lz * g.S[1] + ly) * g.S[0] * g.CC;
+        for (int lx = s[0]; lx <= e[0]; lx++) {
+          const float wx = tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
+          const float dx = tap_dweight(g, 0, tab_s + g.toff[0], dk[0], lx);
+          const float2 v = row[(int64_t)lx * g.CC];
+          const float q0 = dx * wy * wz, q1 = wx * dy * wz, q2 = wx * wy * dz;
+          dr[0] += q0 * v.x;
+          di[0] += q0 * v.y;
+          dr[1] += q1 * v.x;
+          di[1] += q1 * v.y;
+          dr[2] += q2 * v.x;
+          di[2] += q2 * v.y;
+        }
       }
     }
   }
@@ -130,9 +173,18 @@ int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float 
   MRN_TRY(op_permute_in(op, op->kwork, gup, fr, st));
   const Geom &g = op->g;
   dim3 g3((unsigned)((fr.Mv * g.CC + 255) / 256), op->nch);
-  hipLaunchKernelGGL(k_traj_grad, g3, dim3(256), sizeof(float) * g.ntab, st, op->grid, op->kwork,
-                     fr.coords, fr.perm, fr.Mv, g, op->tables_d,
-                     (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, fr.Mv * g.CC, grad_k);
+  const int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
+  switch (taps_main(g)) {
+#define MRN_TG(n)                                                                                           \
+  case n:                                                                                                   \
+    hipLaunchKernelGGL((k_traj_grad<n>), g3, dim3(256), sizeof(float) * g.ntab, st, op->grid, op->kwork,     \
+                       fr.coords, fr.perm, fr.Mv, g, op->tables_d, cg, ck, grad_k);                          \
+    break;
+    MRN_TG(1) MRN_TG(2) MRN_TG(3) MRN_TG(4) MRN_TG(5) MRN_TG(6) MRN_TG(7) MRN_TG(8)
+#undef MRN_TG
+    default:
+      return fail(op, MRN_EUNSUPPORTED, "kernel width above 8 taps");
+  }
   MRN_HIP(op, hipGetLastError());
   return MRN_OK;
 }
