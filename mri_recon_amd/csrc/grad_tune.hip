@@ -68,12 +68,13 @@ __device__ __forceinline__ float tap_dweight(const Geom &g, int a, const float *
 }
 
 // grad_k[m][a] += Re sum_c conj(g_cm) dY_cm/dk_a over the CC coils of a block.
-// Thread = (sample, coil).  The kernel weights and their derivatives are
-// evaluated once per axis and tap (3 NT of each, in registers), not once per
-// tap of the 3-D stencil: in exact-KB mode each is an I0 or I1 evaluation
-// with an exponential.  A sample with more than NT taps on an axis (exactly on
-// the window edge) takes the loop path.
-template <int NT>
+// Thread = (sample, NC consecutive coils): one 8 NC-byte load per tap, as in
+// the gather.  The kernel weights and their derivatives are evaluated once
+// per axis and tap (3 NT of each, in registers), not once per tap of the 3-D
+// stencil: in exact-KB mode each is an I0 or I1 evaluation with an
+// exponential.  A sample with more than NT taps on an axis (exactly on the
+// window edge) takes the loop path.
+template <int NT, int NC>
 __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ grid,
                                                    const float2 *__restrict__ gin,
                                                    const float4 *__restrict__ coords,
@@ -84,9 +85,11 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
   extern __shared__ float tab_s[];
   for (int q = threadIdx.x; q < g.ntab; q += blockDim.x) tab_s[q] = tables[q];
   __syncthreads();
+  constexpr int LGNC = NC == 8 ? 3 : NC == 4 ? 2 : NC == 2 ? 1 : 0;
+  const int lgl = g.lgCC - LGNC;  // log2 of the lanes per sample
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = (int)(gid & (g.CC - 1));
-  const int64_t i = gid >> g.lgCC;
+  const int c = (int)(gid & ((1 << lgl) - 1)) * NC;
+  const int64_t i = gid >> lgl;
   if (i >= Mv) return;
   const int ch = blockIdx.y;
   const float4 k = coords[i];
@@ -98,7 +101,23 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
     wide = wide || e[a] - s[a] + 1 > NT;
   }
   const float2 *G = grid + ch * chunk_grid + c;
-  float dr[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
+  float dr[3][NC], di[3][NC];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int q = 0; q < NC; q++) dr[a][q] = di[a][q] = 0.f;
+  auto tap = [&](const float2 *cell, float q0, float q1, float q2) {
+    const CoilVec<NC> v = CoilVec<NC>::load(cell);
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      dr[0][q] += q0 * v.v[q].x;
+      di[0][q] += q0 * v.v[q].y;
+      dr[1][q] += q1 * v.v[q].x;
+      di[1][q] += q1 * v.v[q].y;
+      dr[2][q] += q2 * v.v[q].x;
+      di[2][q] += q2 * v.v[q].y;
+    }
+  };
   if (!wide) {
     float w[3][NT], d[3][NT];
     int o[3][NT];
@@ -119,16 +138,8 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
         const float2 *row = G + ((int64_t)o[2][tz] * g.S[1] + o[1][ty]) * g.S[0] * g.CC;
         const float wzy = w[1][ty] * w[2][tz], dzy = w[1][ty] * d[2][tz], wzd = d[1][ty] * w[2][tz];
 #pragma unroll
-        for (int tx = 0; tx < NT; tx++) {
-          const float2 v = row[(int64_t)o[0][tx] * g.CC];
-          const float q0 = d[0][tx] * wzy, q1 = w[0][tx] * wzd, q2 = w[0][tx] * dzy;
-          dr[0] += q0 * v.x;
-          di[0] += q0 * v.y;
-          dr[1] += q1 * v.x;
-          di[1] += q1 * v.y;
-          dr[2] += q2 * v.x;
-          di[2] += q2 * v.y;
-        }
+        for (int tx = 0; tx < NT; tx++)
+          tap(row + (int64_t)o[0][tx] * g.CC, d[0][tx] * wzy, w[0][tx] * wzd, w[0][tx] * dzy);
       }
   } else {
     for (int lz = s[2]; lz <= e[2]; lz++) {
@@ -141,25 +152,23 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
         for (int lx = s[0]; lx <= e[0]; lx++) {
           const float wx = tap_weight(g, 0, tab_s + g.toff[0], dk[0], lx);
           const float dx = tap_dweight(g, 0, tab_s + g.toff[0], dk[0], lx);
-          const float2 v = row[(int64_t)lx * g.CC];
-          const float q0 = dx * wy * wz, q1 = wx * dy * wz, q2 = wx * wy * dz;
-          dr[0] += q0 * v.x;
-          di[0] += q0 * v.y;
-          dr[1] += q1 * v.x;
-          di[1] += q1 * v.y;
-          dr[2] += q2 * v.x;
-          di[2] += q2 * v.y;
+          tap(row + (int64_t)lx * g.CC, dx * wy * wz, wx * dy * wz, wx * wy * dz);
         }
       }
     }
   }
-  const float2 gv = gin[ch * chunk_k + i * g.CC + c];
   const int m = perm[i];
   for (int a = 0; a < 3; a++) {
-    // Re(conj(g) * dY) * d(dk)/dk
-    float val = (gv.x * dr[a] + gv.y * di[a]) * g.grid[a];
-    // sum the CC lanes of this sample before one atomic
-    for (int off = 1; off < g.CC; off <<= 1) val += __shfl_xor(val, off);
+    // Re(conj(g) * dY) * d(dk)/dk over this lane's coils
+    float val = 0.f;
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      const float2 gv = gin[ch * chunk_k + i * g.CC + c + q];
+      val += gv.x * dr[a][q] + gv.y * di[a][q];
+    }
+    val *= g.grid[a];
+    // sum the lanes of this sample before one atomic
+    for (int off = 1; off < (1 << lgl); off <<= 1) val += __shfl_xor(val, off);
     if (c == 0) atomicAdd(&grad_k[3 * m + a], val);
   }
 }
@@ -174,14 +183,19 @@ int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float 
   const Geom &g = op->g;
   dim3 g3((unsigned)((fr.Mv * g.CC + 255) / 256), op->nch);
   const int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
-  switch (taps_main(g)) {
-#define MRN_TG(n)                                                                                           \
-  case n:                                                                                                   \
-    hipLaunchKernelGGL((k_traj_grad<n>), g3, dim3(256), sizeof(float) * g.ntab, st, op->grid, op->kwork,     \
+  // coils per lane: 4 (2 for 2-coil blocks, 1 for 1)
+  const int nc = g.CC >= 4 ? 4 : g.CC;
+  g3.x = (unsigned)((fr.Mv * (g.CC / nc) + 255) / 256);
+  switch (taps_main(g) * 16 + nc) {
+#define MRN_TG1(n, c)                                                                                        \
+  case n * 16 + c:                                                                                          \
+    hipLaunchKernelGGL((k_traj_grad<n, c>), g3, dim3(256), sizeof(float) * g.ntab, st, op->grid, op->kwork,  \
                        fr.coords, fr.perm, fr.Mv, g, op->tables_d, cg, ck, grad_k);                          \
     break;
+#define MRN_TG(n) MRN_TG1(n, 1) MRN_TG1(n, 2) MRN_TG1(n, 4)
     MRN_TG(1) MRN_TG(2) MRN_TG(3) MRN_TG(4) MRN_TG(5) MRN_TG(6) MRN_TG(7) MRN_TG(8)
 #undef MRN_TG
+#undef MRN_TG1
     default:
       return fail(op, MRN_EUNSUPPORTED, "kernel width above 8 taps");
   }

@@ -245,45 +245,6 @@ __global__ void k_compact(const int32_t *__restrict__ flag, const int32_t *__res
 // Taps per axis handled by the unrolled path; samples with more taps on an
 // axis (a coordinate exactly an integer away from the window edge) take the
 // loop path.
-// NC coils per lane (NC = 2: one 16-byte load per tap covers two coils, which
-// halves the vector-memory instructions the texture path has to address).
-template <int NC>
-struct CoilVec;
-template <>
-struct CoilVec<1> {
-  float2 v[1];
-  __device__ __forceinline__ static CoilVec load(const float2 *p) {
-    CoilVec r;
-    r.v[0] = *p;
-    return r;
-  }
-};
-template <>
-struct CoilVec<2> {
-  float2 v[2];
-  __device__ __forceinline__ static CoilVec load(const float2 *p) {
-    const float4 q = *reinterpret_cast<const float4 *>(p);
-    CoilVec r;
-    r.v[0] = make_float2(q.x, q.y);
-    r.v[1] = make_float2(q.z, q.w);
-    return r;
-  }
-};
-template <int NC>  // NC = 4, 8: NC / 2 16-byte loads of consecutive coils
-struct CoilVec {
-  float2 v[NC];
-  __device__ __forceinline__ static CoilVec load(const float2 *p) {
-    CoilVec r;
-#pragma unroll
-    for (int h = 0; h < NC / 2; h++) {
-      const float4 q = reinterpret_cast<const float4 *>(p)[h];
-      r.v[2 * h] = make_float2(q.x, q.y);
-      r.v[2 * h + 1] = make_float2(q.z, q.w);
-    }
-    return r;
-  }
-};
-
 template <int NT, int NC>
 __global__ __launch_bounds__(256) void k_gather(const float2 *__restrict__ grid,
                                                 float2 *__restrict__ out,
