@@ -105,7 +105,10 @@ int mrn_get_window(const mrn_op *op, int axis, float *host_out, int cap, int *le
 
 /* Trajectory of one frame (the kx, ky, kz, kw arguments of gridFFT::forward /
  * backward, gridFFT.h:118-141).  Sorts samples into tiles once; kept until
- * replaced.  Device pointers, M samples. */
+ * replaced.  Device pointers, M samples.  Waits for the device first; a
+ * frame's arrays are reused when the new trajectory fits in them.  Stored
+ * k-space data (mrn_set_kdata) is re-sorted for the new trajectory when M is
+ * unchanged and dropped otherwise. */
 int mrn_set_traj(mrn_op *op, int frame, int64_t M, const float *kx, const float *ky,
                  const float *kz, const float *kw, void *stream);
 /* Coil sensitivities smaps[C][Nz][Ny][Nx] (the smap argument, gridFFT.h:119). */

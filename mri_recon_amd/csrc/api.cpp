@@ -546,13 +546,17 @@ int mrn_set_traj(mrn_op *op, int f, int64_t M, const float *kx, const float *ky,
   if (M > 0 && (!kx || !ky || !kz || !kw)) return fail(op, MRN_EINVAL, "null trajectory pointer");
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
-  hipFree(fr.traw);
-  fr.traw = nullptr;
-  MRN_HIP(op, hipMalloc(&fr.traw, sizeof(float) * 4 * (M > 0 ? M : 1)));
+  // the frame's arrays are reused when large enough: first let every launch
+  // that may still read the old trajectory finish
+  MRN_HIP(op, hipDeviceSynchronize());
+  MRN_HIP(op, grow(fr.traw, fr.cap.traw, 4 * M));
   const float *src[4] = {kx, ky, kz, kw};
   for (int q = 0; q < 4; q++)
     if (M > 0) MRN_HIP(op, hipMemcpyAsync(fr.traw + q * M, src[q], sizeof(float) * M, hipMemcpyDeviceToDevice, st));
   fr.M = M;
+  // stored data of another sample count no longer matches the trajectory
+  // (kraw is [C][kraw_M]): the caller sets it again
+  if (fr.has_kdata && fr.kraw_M != M) fr.has_kdata = false;
   if (batched(op)) {
     MRN_TRY(build_bins(op, fr, M, fr.traw, fr.traw + M, fr.traw + 2 * M, fr.traw + 3 * M, st));
     const int64_t need = fr.Mv * op->g.CC * op->nch;
@@ -563,9 +567,7 @@ int mrn_set_traj(mrn_op *op, int f, int64_t M, const float *kx, const float *ky,
       op->kwork_cap = need;
     }
     if (fr.has_kdata) {  // re-sort stored data for the new trajectory
-      hipFree(fr.kdata);
-      fr.kdata = nullptr;
-      MRN_HIP(op, hipMalloc(&fr.kdata, sizeof(float2) * (need > 0 ? need : 1)));
+      MRN_HIP(op, grow(fr.kdata, fr.cap.kdata, need));
       MRN_TRY(op_permute_in(op, fr.kdata, fr.kraw, fr, st));
     }
   }
@@ -598,17 +600,15 @@ int mrn_set_kdata(mrn_op *op, int f, const void *data, void *stream) {
   MRN_TRY(check_frame(op, f, false));
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
-  hipFree(fr.kraw);
-  fr.kraw = nullptr;
-  MRN_HIP(op, hipMalloc(&fr.kraw, sizeof(float2) * op->C * (fr.M > 0 ? fr.M : 1)));
+  MRN_HIP(op, hipDeviceSynchronize());  // arrays reused in place: no launch may still read them
+  MRN_HIP(op, grow(fr.kraw, fr.cap.kraw, (int64_t)op->C * fr.M));
   MRN_HIP(op, hipMemcpyAsync(fr.kraw, data, sizeof(float2) * op->C * fr.M, hipMemcpyDeviceToDevice, st));
   if (batched(op)) {
     const int64_t need = fr.Mv * op->g.CC * op->nch;
-    hipFree(fr.kdata);
-    fr.kdata = nullptr;
-    MRN_HIP(op, hipMalloc(&fr.kdata, sizeof(float2) * (need > 0 ? need : 1)));
+    MRN_HIP(op, grow(fr.kdata, fr.cap.kdata, need));
     MRN_TRY(op_permute_in(op, fr.kdata, fr.kraw, fr, st));
   }
+  fr.kraw_M = fr.M;
   fr.has_kdata = true;
   op->version++;
   op->cg_ready = false;

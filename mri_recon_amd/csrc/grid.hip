@@ -157,7 +157,8 @@ __global__ void k_widen(const int32_t *__restrict__ v, int n, int64_t *out) {
 // Visit list: tile t's visits (the samples of its bin rows, rows in (bz, by)
 // order, each row one contiguous run of internal indices) at
 // vlist[vofs[t] ...], so that the scatter reads visit v of a tile with one
-// load.  One wavefront per tile.
+// load.  One wavefront per tile, one lane per bin row: the lanes' run
+// lengths are loaded together and scanned, then each lane writes its run.
 __global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, const int64_t *__restrict__ vofs,
                               int32_t *vlist) {
   const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -168,14 +169,37 @@ __global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, co
   tile_bins(t, 0, tx, bx0, bx1);
   tile_bins(t, 1, ty, by0, by1);
   tile_bins(t, 2, tz, bz0, bz1);
+  const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
   int64_t o = vofs[tile];
-  for (int bz = bz0; bz <= bz1; bz++)
-    for (int by = by0; by <= by1; by++) {
-      const int b = (bz * t.nb[1] + by) * t.nb[0];
-      const int s0 = bin_start[b + bx0], n = bin_start[b + bx1 + 1] - s0;
-      for (int q = lane; q < n; q += 64) vlist[o + q] = s0 + q;
-      o += n;
+  for (int r0 = 0; r0 < nrows; r0 += 64) {
+    const int r = r0 + lane;
+    int s0 = 0, n = 0;
+    if (r < nrows) {
+      const int b = ((bz0 + r / nry) * t.nb[1] + by0 + r % nry) * t.nb[0];
+      s0 = bin_start[b + bx0];
+      n = bin_start[b + bx1 + 1] - s0;
     }
+    int x = n;  // inclusive prefix over the lanes
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    // Visit p of this group of rows belongs to the row r whose inclusive
+    // prefix is the first above p (binary search over the lanes' prefixes),
+    // so consecutive lanes write consecutive entries.
+    const int tot = __shfl(x, 63);
+    for (int p0 = 0; p0 < tot; p0 += 64) {
+      const int p = p0 + lane;
+      int r = 0;
+#pragma unroll
+      for (int step = 32; step; step >>= 1)
+        if (__shfl(x, r + step - 1) <= p) r += step;
+      const int ex = __shfl(x, r) - __shfl(n, r), base = __shfl(s0, r);
+      if (p < tot) vlist[o + p] = base + (p - ex);
+    }
+    o += tot;
+  }
 }
 
 // Gather work list: the samples whose first tap lies in tile t (its "home"
@@ -1877,118 +1901,72 @@ __global__ void k_plane_yrange(int2 *xrng, int S1, int S2, int2 *yrng) {
     if (y < y0 || y > y1) row[y] = make_int2(INT_MAX, INT_MIN);
 }
 
-// Sort one frame's samples into tiles and build the scatter work list.
+// Sort one frame's samples into tiles and build the scatter work list.  The
+// frame's arrays and the bind's scratch only grow (FrameCaps, BindScratch), so
+// binding a trajectory no larger than the last one allocates nothing; the
+// device is synchronised first, as the hipFree of the old arrays did, so that
+// no launch still reading the old trajectory sees the new one.
 int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const float *ky,
                const float *kz, const float *kw, hipStream_t st) {
   const Geom &g = op->g;
   const Tiles &t = op->t;
   const int nt = t.ntiles, nbin = t.nbins;
-  uint32_t *keys = nullptr, *keys2 = nullptr;
-  int32_t *vals = nullptr, *counts = nullptr, *nsub = nullptr, *visits = nullptr, *off = nullptr,
-          *hflag = nullptr, *hoff = nullptr;
-  int2 *brow = nullptr;
-  int32_t *wflag = nullptr, *woff = nullptr;
-  void *tmp = nullptr, *wtmp = nullptr;
+  BindScratch &b = op->bind;
+  FrameCaps &cp = fr.cap;
   size_t tmp_bytes = 0, b2 = 0;
-  int rc = MRN_OK;
-  auto cleanup = [&]() {
-    hipFree(wflag);
-    hipFree(woff);
-    hipFree(wtmp);
-    hipFree(keys);
-    hipFree(keys2);
-    hipFree(vals);
-    hipFree(counts);
-    hipFree(nsub);
-    hipFree(visits);
-    hipFree(off);
-    hipFree(hflag);
-    hipFree(hoff);
-    hipFree(brow);
-    hipFree(tmp);
-  };
 #define MRN_BH(call)                                                                         \
   do {                                                                                       \
     hipError_t e_ = (call);                                                                  \
     if (e_ != hipSuccess) {                                                                  \
-      cleanup();                                                                             \
+      fr.has_traj = false;                                                                   \
       return fail(op, e_ == hipErrorOutOfMemory ? MRN_ENOMEM : MRN_EHIP, hipGetErrorString(e_)); \
     }                                                                                        \
   } while (0)
-  // release the previous trajectory of this frame
-  hipFree(fr.coords);
-  hipFree(fr.taper);
-  hipFree(fr.perm);
-  hipFree(fr.inv);
-  hipFree(fr.bin_start);
-  hipFree(fr.subs);
-  hipFree(fr.heavy);
-  hipFree(fr.vofs);
-  hipFree(fr.vlist);
-  fr.vofs = nullptr;
-  fr.vlist = nullptr;
-  hipFree(fr.gsubs);
-  hipFree(fr.sparse);
-  hipFree(fr.xrng);
-  hipFree(fr.yrng);
-  hipFree(fr.wide);
-  fr.wide = nullptr;
+  MRN_BH(hipDeviceSynchronize());
+  fr.has_traj = false;
   fr.nwide = 0;
-  fr.gsubs = nullptr;
-  fr.sparse = nullptr;
-  fr.xrng = nullptr;
-  fr.yrng = nullptr;
   fr.ngsub = 0;
   fr.nsparse = 0;
-  fr.coords = nullptr;
-  fr.taper = nullptr;
-  fr.perm = nullptr;
-  fr.inv = nullptr;
-  fr.bin_start = nullptr;
-  fr.subs = nullptr;
-  fr.heavy = nullptr;
-  fr.has_traj = false;
+  fr.nsub = 0;
+  fr.nheavy = 0;
   fr.M = M;
-  const int64_t Mm = M > 0 ? M : 1;
-  MRN_BH(hipMalloc(&keys, sizeof(uint32_t) * Mm));
-  MRN_BH(hipMalloc(&keys2, sizeof(uint32_t) * Mm));
-  MRN_BH(hipMalloc(&vals, sizeof(int32_t) * Mm));
-  MRN_BH(hipMalloc(&fr.perm, sizeof(int32_t) * Mm));
-  MRN_BH(hipMalloc(&fr.inv, sizeof(int32_t) * Mm));
-  MRN_BH(hipMalloc(&counts, sizeof(int32_t) * (nbin + 1)));
-  MRN_BH(hipMalloc(&fr.bin_start, sizeof(int32_t) * (nbin + 1)));
-  MRN_BH(hipMemsetAsync(counts, 0, sizeof(int32_t) * (nbin + 1), st));
+  MRN_BH(grow(b.keys, b.c_keys, M));
+  MRN_BH(grow(b.keys2, b.c_keys2, M));
+  MRN_BH(grow(b.vals, b.c_vals, M));
+  MRN_BH(grow(fr.perm, cp.perm, M));
+  MRN_BH(grow(fr.inv, cp.inv, M));
+  MRN_BH(grow(b.counts, b.c_counts, (int64_t)nbin + 1));
+  MRN_BH(grow(fr.bin_start, cp.bin_start, (int64_t)nbin + 1));
+  MRN_BH(hipMemsetAsync(b.counts, 0, sizeof(int32_t) * (nbin + 1), st));
+  int bits = 1;
+  while ((1u << bits) <= (uint32_t)nbin) bits++;
+  {
+    size_t b3 = 0;
+    if (M > 0)
+      MRN_BH(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, b.keys, b.keys2, b.vals, fr.perm, (int)M, 0,
+                                                bits, st));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, b.counts, fr.bin_start, nbin + 1, st));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, b.counts, fr.bin_start, nt + 1, st));
+    b2 = b2 > b3 ? b2 : b3;
+    tmp_bytes = tmp_bytes > b2 ? tmp_bytes : b2;
+    MRN_BH(grow(b.tmp, b.c_tmp, (int64_t)tmp_bytes));
+  }
   if (M > 0) {
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(M, 256)), dim3(256), 0, st, fr.inv, -1, M);
     hipLaunchKernelGGL(k_traj_keys, dim3(nblk(M, 256)), dim3(256), 0, st, kx, ky, kz, kw, M, g, t,
-                       keys, vals, counts);
+                       b.keys, b.vals, b.counts);
     MRN_BH(hipGetLastError());
-    int bits = 1;
-    while ((1u << bits) <= (uint32_t)nbin) bits++;
-    MRN_BH(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, vals, fr.perm, (int)M, 0, bits, st));
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, counts, fr.bin_start, nbin + 1, st));
-    size_t b3 = 0;
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, counts, fr.bin_start, nt + 1, st));
-    b2 = b2 > b3 ? b2 : b3;
-    tmp_bytes = tmp_bytes > b2 ? tmp_bytes : b2;
-    MRN_BH(hipMalloc(&tmp, tmp_bytes));
-    MRN_BH(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, vals, fr.perm, (int)M, 0, bits, st));
+    MRN_BH(hipcub::DeviceRadixSort::SortPairs(b.tmp, tmp_bytes, b.keys, b.keys2, b.vals, fr.perm, (int)M, 0, bits,
+                                              st));
   }
   size_t tb = tmp_bytes;
-  if (!tmp) {
-    size_t b3 = 0;
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts, fr.bin_start, nbin + 1, st));
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, counts, fr.bin_start, nt + 1, st));
-    tb = tb > b3 ? tb : b3;
-    MRN_BH(hipMalloc(&tmp, tb));
-  }
-  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, fr.bin_start, nbin + 1, st));
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, b.counts, fr.bin_start, nbin + 1, st));
   int32_t Mv32 = 0;
   MRN_BH(hipMemcpyAsync(&Mv32, fr.bin_start + nbin, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MRN_BH(hipStreamSynchronize(st));
   fr.Mv = Mv32;
-  MRN_BH(hipMalloc(&fr.coords, sizeof(float4) * (fr.Mv > 0 ? fr.Mv : 1)));
-  MRN_BH(hipMalloc(&fr.taper, sizeof(float) * (fr.Mv > 0 ? fr.Mv : 1)));
+  MRN_BH(grow(fr.coords, cp.coords, fr.Mv));
+  MRN_BH(grow(fr.taper, cp.taper, fr.Mv));
   if (fr.Mv > 0) {
     hipLaunchKernelGGL(k_traj_finish, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, kx, ky, kz, kw,
                        fr.perm, fr.Mv, g, fr.coords, fr.taper, fr.inv);
@@ -1997,24 +1975,24 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   // when 2 dwin is an integer on some gridded axis)
   if (t.NTb < t.NT && fr.Mv > 0) {
     const int64_t n = fr.Mv + 1;
-    MRN_BH(hipMalloc(&wflag, sizeof(int32_t) * n));
-    MRN_BH(hipMalloc(&woff, sizeof(int32_t) * n));
-    MRN_BH(hipMemsetAsync(wflag, 0, sizeof(int32_t) * n, st));
-    hipLaunchKernelGGL(k_wide_flag, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, fr.coords, fr.Mv, g, t.NTb, wflag);
+    MRN_BH(grow(b.wflag, b.c_wflag, n));
+    MRN_BH(grow(b.woff, b.c_woff, n));
+    MRN_BH(hipMemsetAsync(b.wflag, 0, sizeof(int32_t) * n, st));
+    hipLaunchKernelGGL(k_wide_flag, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, fr.coords, fr.Mv, g, t.NTb, b.wflag);
     MRN_BH(hipGetLastError());
     size_t wb = 0;
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, wb, wflag, woff, (int)n, st));
-    MRN_BH(hipMalloc(&wtmp, wb > 0 ? wb : 16));
-    MRN_BH(hipcub::DeviceScan::ExclusiveSum(wtmp, wb, wflag, woff, (int)n, st));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, wb, b.wflag, b.woff, (int)n, st));
+    MRN_BH(grow(b.wtmp, b.c_wtmp, (int64_t)wb));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.wtmp, wb, b.wflag, b.woff, (int)n, st));
     int32_t nw = 0;
-    MRN_BH(hipMemcpyAsync(&nw, woff + fr.Mv, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    MRN_BH(hipMemcpyAsync(&nw, b.woff + fr.Mv, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     MRN_BH(hipStreamSynchronize(st));
-    fr.nwide = nw;
     if (nw > 0) {
-      MRN_BH(hipMalloc(&fr.wide, sizeof(int32_t) * nw));
-      hipLaunchKernelGGL(k_compact64, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, wflag, woff, fr.Mv, fr.wide);
+      MRN_BH(grow(fr.wide, cp.wide, nw));
+      hipLaunchKernelGGL(k_compact64, dim3(nblk(fr.Mv, 256)), dim3(256), 0, st, b.wflag, b.woff, fr.Mv, fr.wide);
       MRN_BH(hipGetLastError());
     }
+    fr.nwide = nw;
   }
   // k-space support per grid row, for the x passes
   {
@@ -2022,23 +2000,33 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
     const int64_t nrow = (int64_t)g.S[1] * g.S[2];
     int W[3];
     for (int a = 0; a < 3; a++) W[a] = g.grid_in[a] ? (int)floorf(2.0f * g.dwin[a]) : 0;
-    MRN_BH(hipMalloc(&brow, sizeof(int2) * nbr));
-    MRN_BH(hipMalloc(&fr.xrng, sizeof(int2) * nrow));
-    hipLaunchKernelGGL(k_binrow_x, dim3(nblk((int64_t)nbr * 64, 256)), dim3(256), 0, st, fr.bin_start, t, brow);
-    hipLaunchKernelGGL(k_row_xrange, dim3(nblk(nrow, 256)), dim3(256), 0, st, brow, t, g.S[0], g.S[1], g.S[2],
+    MRN_BH(grow(b.brow, b.c_brow, nbr));
+    MRN_BH(grow(fr.xrng, cp.xrng, nrow));
+    hipLaunchKernelGGL(k_binrow_x, dim3(nblk((int64_t)nbr * 64, 256)), dim3(256), 0, st, fr.bin_start, t, b.brow);
+    hipLaunchKernelGGL(k_row_xrange, dim3(nblk(nrow, 256)), dim3(256), 0, st, b.brow, t, g.S[0], g.S[1], g.S[2],
                        W[0], W[1], W[2], fr.xrng);
     MRN_BH(hipGetLastError());
     if (fft_xy_pruned(op)) {
-      MRN_BH(hipMalloc(&fr.yrng, sizeof(int2) * g.S[2]));
+      MRN_BH(grow(fr.yrng, cp.yrng, g.S[2]));
       hipLaunchKernelGGL(k_plane_yrange, dim3(nblk(g.S[2], 64)), dim3(64), 0, st, fr.xrng, g.S[1], g.S[2], fr.yrng);
       MRN_BH(hipGetLastError());
+    } else {  // a null yrng means "every row" to the y passes
+      hipFree(fr.yrng);
+      fr.yrng = nullptr;
+      cp.yrng = 0;
     }
   }
-  MRN_BH(hipMalloc(&nsub, sizeof(int32_t) * (nt + 1)));
-  MRN_BH(hipMalloc(&visits, sizeof(int32_t) * (nt + 1)));
-  MRN_BH(hipMalloc(&off, sizeof(int32_t) * (nt + 1)));
-  MRN_BH(hipMalloc(&hflag, sizeof(int32_t) * (nt + 1)));
-  MRN_BH(hipMalloc(&hoff, sizeof(int32_t) * (nt + 1)));
+  int32_t *nsub = nullptr, *visits = nullptr, *off = nullptr, *hflag = nullptr, *hoff = nullptr;
+  MRN_BH(grow(b.nsub, b.c_nsub, (int64_t)nt + 1));
+  MRN_BH(grow(b.visits, b.c_visits, (int64_t)nt + 1));
+  MRN_BH(grow(b.off, b.c_off, (int64_t)nt + 1));
+  MRN_BH(grow(b.hflag, b.c_hflag, (int64_t)nt + 1));
+  MRN_BH(grow(b.hoff, b.c_hoff, (int64_t)nt + 1));
+  nsub = b.nsub;
+  visits = b.visits;
+  off = b.off;
+  hflag = b.hflag;
+  hoff = b.hoff;
   MRN_BH(hipMemsetAsync(nsub, 0, sizeof(int32_t) * (nt + 1), st));
   MRN_BH(hipMemsetAsync(hflag, 0, sizeof(int32_t) * (nt + 1), st));
   hipLaunchKernelGGL(k_tile_nsub, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, nsub, visits,
@@ -2046,42 +2034,37 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   MRN_BH(hipGetLastError());
   // visit list (offsets in 64 bits: about 8 visits per sample)
   {
-    MRN_BH(hipMalloc(&fr.vofs, sizeof(int64_t) * (nt + 1)));
-    int64_t *vw = nullptr;
-    MRN_BH(hipMalloc(&vw, sizeof(int64_t) * (nt + 1)));
-    hipLaunchKernelGGL(k_widen, dim3(nblk((int64_t)nt + 1, 256)), dim3(256), 0, st, visits, nt, vw);
+    MRN_BH(grow(fr.vofs, cp.vofs, (int64_t)nt + 1));
+    MRN_BH(grow(b.vw, b.c_vw, (int64_t)nt + 1));
+    hipLaunchKernelGGL(k_widen, dim3(nblk((int64_t)nt + 1, 256)), dim3(256), 0, st, visits, nt, b.vw);
     size_t vb = 0;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, vb, vw, fr.vofs, nt + 1, st);
-    void *vtmp = nullptr;
-    if (e == hipSuccess) e = hipMalloc(&vtmp, vb > 0 ? vb : 16);
-    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(vtmp, vb, vw, fr.vofs, nt + 1, st);
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, vb, b.vw, fr.vofs, nt + 1, st));
+    MRN_BH(grow(b.vtmp, b.c_vtmp, (int64_t)vb));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.vtmp, vb, b.vw, fr.vofs, nt + 1, st));
     int64_t nv = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&nv, fr.vofs + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(vtmp);
-    hipFree(vw);
-    MRN_BH(e);
-    MRN_BH(hipMalloc(&fr.vlist, sizeof(int32_t) * (nv > 0 ? nv : 1)));
+    MRN_BH(hipMemcpyAsync(&nv, fr.vofs + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MRN_BH(hipStreamSynchronize(st));
+    MRN_BH(grow(fr.vlist, cp.vlist, nv));
     if (nv > 0) {
       hipLaunchKernelGGL(k_fill_visits, dim3(nblk(nt, 4)), dim3(256), 0, st, fr.bin_start, t, fr.vofs, fr.vlist);
       MRN_BH(hipGetLastError());
     }
   }
-  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nsub, off, nt + 1, st));
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, nsub, off, nt + 1, st));
   int32_t nsub_total = 0;
   MRN_BH(hipMemcpyAsync(&nsub_total, off + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MRN_BH(hipStreamSynchronize(st));
+  MRN_BH(grow(fr.subs, cp.subs, nsub_total));
   fr.nsub = nsub_total;
-  MRN_BH(hipMalloc(&fr.subs, sizeof(int4) * (nsub_total > 0 ? nsub_total : 1)));
   hipLaunchKernelGGL(k_fill_subs, dim3(nblk(nt, 256)), dim3(256), 0, st, nsub, off, visits, t,
                      fr.subs, hflag);
   MRN_BH(hipGetLastError());
-  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, hflag, hoff, nt + 1, st));
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, hflag, hoff, nt + 1, st));
   int32_t nh = 0;
   MRN_BH(hipMemcpyAsync(&nh, hoff + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MRN_BH(hipStreamSynchronize(st));
+  MRN_BH(grow(fr.heavy, cp.heavy, nh));
   fr.nheavy = nh;
-  MRN_BH(hipMalloc(&fr.heavy, sizeof(int32_t) * (nh > 0 ? nh : 1)));
   if (nh > 0) {
     hipLaunchKernelGGL(k_compact, dim3(nblk(nt, 256)), dim3(256), 0, st, hflag, hoff, nt, fr.heavy);
     MRN_BH(hipGetLastError());
@@ -2091,23 +2074,23 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipLaunchKernelGGL(k_gtile_count, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, GT_PER, GT_MIN,
                      nsub, visits, hflag);
   MRN_BH(hipGetLastError());
-  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nsub, off, nt + 1, st));
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, nsub, off, nt + 1, st));
   int32_t ng = 0;
   MRN_BH(hipMemcpyAsync(&ng, off + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MRN_BH(hipStreamSynchronize(st));
+  MRN_BH(grow(fr.gsubs, cp.gsubs, ng));
   fr.ngsub = ng;
-  MRN_BH(hipMalloc(&fr.gsubs, sizeof(int4) * (ng > 0 ? ng : 1)));
   if (ng > 0) {
     hipLaunchKernelGGL(k_gtile_fill, dim3(nblk(nt, 256)), dim3(256), 0, st, nsub, off, visits, t, GT_PER, fr.gsubs);
     MRN_BH(hipGetLastError());
   }
   // samples of the sparse tiles, for the direct gather
-  MRN_BH(hipcub::DeviceScan::ExclusiveSum(tmp, tb, hflag, hoff, nt + 1, st));
+  MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, hflag, hoff, nt + 1, st));
   int32_t nsp = 0;
   MRN_BH(hipMemcpyAsync(&nsp, hoff + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MRN_BH(hipStreamSynchronize(st));
+  MRN_BH(grow(fr.sparse, cp.sparse, nsp));
   fr.nsparse = nsp;
-  MRN_BH(hipMalloc(&fr.sparse, sizeof(int32_t) * (nsp > 0 ? nsp : 1)));
   if (nsp > 0) {
     hipLaunchKernelGGL(k_gtile_sparse, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, hflag, hoff,
                        fr.sparse);
@@ -2115,9 +2098,8 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   }
   MRN_BH(hipStreamSynchronize(st));
 #undef MRN_BH
-  cleanup();
   fr.has_traj = true;
-  return rc;
+  return MRN_OK;
 }
 
 }  // namespace mrn

@@ -79,7 +79,47 @@ struct FrameDev {
   int nsub, nheavy, nwide, pad;
 };
 
+// Device arrays that only grow: binding a trajectory no larger than an
+// earlier one of the same op reuses their memory, with no hipFree/hipMalloc
+// (each of which waits for the whole device).  grow() keeps the contents only
+// when the capacity suffices.
+template <class T>
+inline hipError_t grow(T *&p, size_t &cap, int64_t n) {
+  const size_t want = n > 0 ? (size_t)n : 1;
+  if (p && want <= cap) return hipSuccess;
+  hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const hipError_t e = hipMalloc((void **)&p, sizeof(T) * want);
+  if (e == hipSuccess) cap = want;
+  return e;
+}
+
+// Capacities (elements) of a frame's grow-only arrays.
+struct FrameCaps {
+  size_t coords = 0, taper = 0, perm = 0, inv = 0, bin_start = 0, subs = 0, heavy = 0, vofs = 0, vlist = 0,
+         gsubs = 0, wide = 0, sparse = 0, xrng = 0, yrng = 0, kdata = 0, kraw = 0, traw = 0;
+};
+
+// Scratch of a trajectory bind, kept by the op between binds (grow-only).
+struct BindScratch {
+  uint32_t *keys = nullptr, *keys2 = nullptr;
+  int32_t *vals = nullptr, *counts = nullptr, *nsub = nullptr, *visits = nullptr, *off = nullptr,
+          *hflag = nullptr, *hoff = nullptr, *wflag = nullptr, *woff = nullptr;
+  int2 *brow = nullptr;
+  int64_t *vw = nullptr;
+  uint8_t *tmp = nullptr, *wtmp = nullptr, *vtmp = nullptr;
+  size_t c_keys = 0, c_keys2 = 0, c_vals = 0, c_counts = 0, c_nsub = 0, c_visits = 0, c_off = 0, c_hflag = 0,
+         c_hoff = 0, c_wflag = 0, c_woff = 0, c_brow = 0, c_vw = 0, c_tmp = 0, c_wtmp = 0, c_vtmp = 0;
+  void release() {
+    void *ps[] = {keys, keys2, vals, counts, nsub, visits, off, hflag, hoff, wflag, woff, brow, vw, tmp, wtmp, vtmp};
+    for (void *q : ps) hipFree(q);
+    *this = BindScratch{};
+  }
+};
+
 struct FrameData {
+  FrameCaps cap;
   int64_t M = 0;               // samples (caller order)
   int64_t Mv = 0;              // valid samples
   float4 *coords = nullptr;    // [Mv] dkx, dky, dkz, kw in tile order
@@ -102,7 +142,8 @@ struct FrameData {
                                // ({INT_MAX, INT_MIN}: row outside yrng[z], when yrng is set)
   int2 *yrng = nullptr;        // [Sz] rows y of plane z with support (y passes pruned), else null
   float2 *kdata = nullptr;     // measured k-space, internal layout [nch][Mv][CC]
-  float2 *kraw = nullptr;      // measured k-space, caller layout [C][M] (plain form)
+  float2 *kraw = nullptr;      // measured k-space, caller layout [C][kraw_M] (plain form)
+  int64_t kraw_M = -1;         // sample count kraw was set for
   float *traw = nullptr;       // trajectory kx|ky|kz|kw, caller order [4][M]
   bool has_traj = false, has_kdata = false;
 };
@@ -137,6 +178,7 @@ struct mrn_op {
   int64_t fmax_Mv = 0;
   float2 *kplain = nullptr;    // plain form: one coil of k-space [M_max]
   int64_t kplain_cap = 0;
+  mrn::BindScratch bind;       // trajectory-bind scratch (grow-only)
   std::vector<mrn::FrameData> frames;
   hipStream_t stream = nullptr;
   // box matrix-core gather (gather_tiles = 2): the dense tiles run on `side`

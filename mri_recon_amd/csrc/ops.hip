@@ -92,6 +92,75 @@ __global__ void k_permute_out(float2 *data, const float2 *kint, const int32_t *i
   }
 }
 
+// The same two permutations with one lane per (caller sample m, coil block
+// ch): the block's CC coils are read from (or written to) C rows at
+// consecutive samples, so the caller side is coalesced, and the internal side
+// moves one whole 8*CC-byte record per lane.  Every internal record belongs
+// to exactly one valid caller sample (inv is a bijection onto [0, Mv)).
+template <int CC>
+__global__ void k_permute_in_rec(float2 *__restrict__ kint, const float2 *__restrict__ data,
+                                 const int32_t *__restrict__ inv, int64_t M, int64_t Mv, int C) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (m >= M) return;
+  const int i = inv[m];
+  if (i < 0) return;
+  float2 v[CC];
+#pragma unroll
+  for (int cc = 0; cc < CC; cc++) {
+    const int c = ch * CC + cc;
+    v[cc] = c < C ? data[(int64_t)c * M + m] : make_float2(0.f, 0.f);
+  }
+  float2 *dst = kint + ((int64_t)ch * Mv + i) * CC;
+  if constexpr (CC == 1) {
+    dst[0] = v[0];
+  } else {
+#pragma unroll
+    for (int cc = 0; cc < CC; cc += 2)
+      *reinterpret_cast<float4 *>(dst + cc) = make_float4(v[cc].x, v[cc].y, v[cc + 1].x, v[cc + 1].y);
+  }
+}
+
+template <int CC>
+__global__ void k_permute_out_rec(float2 *__restrict__ data, const float2 *__restrict__ kint,
+                                  const int32_t *__restrict__ inv, const float2 *__restrict__ sub, int64_t M,
+                                  int64_t Mv, int C) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (m >= M) return;
+  const int i = inv[m];
+  float2 v[CC];
+  if (i >= 0) {
+    const float2 *src = kint + ((int64_t)ch * Mv + i) * CC;
+    if constexpr (CC == 1) {
+      v[0] = src[0];
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < CC; cc += 2) {
+        const float4 q = *reinterpret_cast<const float4 *>(src + cc);
+        v[cc] = make_float2(q.x, q.y);
+        v[cc + 1] = make_float2(q.z, q.w);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int cc = 0; cc < CC; cc++) v[cc] = make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int cc = 0; cc < CC; cc++) {
+    const int c = ch * CC + cc;
+    if (c < C) {
+      float2 w = v[cc];
+      if (sub && i >= 0) {
+        const float2 d = sub[(int64_t)c * M + m];
+        w.x -= d.x;
+        w.y -= d.y;
+      }
+      data[(int64_t)c * M + m] = w;
+    }
+  }
+}
+
 // plain form: grid[og + r] = (w(r) * X(r)) * smap(r)  (grid pre-zeroed)
 __global__ void k_set_image_plain(float2 *grid, const float2 *X, const float2 *smap,
                                   const float *wx, const float *wy, const float *wz,
@@ -410,9 +479,29 @@ int op_smaps_in(mrn_op *op, float2 *dst, const float2 *src, hipStream_t st) {
   LAUNCH_CHECK(op);
   return MRN_OK;
 }
+// The record kernels need 16-byte aligned records (CC >= 2) and one grid row
+// per coil block; anything else takes the element-wise kernels.
+static bool permute_by_record(const mrn_op *op, const void *kint, const FrameData &fr) {
+  return fr.M > 0 && op->nch <= 65535 && (fr.M + 255) / 256 <= INT32_MAX &&
+         (op->g.CC == 1 || ((uintptr_t)kint & 15) == 0);
+}
 int op_permute_in(mrn_op *op, float2 *kint, const float2 *data, const FrameData &fr, hipStream_t st) {
   const int64_t total = (int64_t)op->nch * fr.Mv * op->g.CC;
   if (total == 0) return MRN_OK;
+  if (permute_by_record(op, kint, fr)) {
+    const dim3 grid((unsigned)((fr.M + 255) / 256), (unsigned)op->nch);
+    switch (op->g.CC) {
+#define MRN_PIN(n)                                                                                      \
+  case n:                                                                                               \
+    hipLaunchKernelGGL(k_permute_in_rec<n>, grid, dim3(256), 0, st, kint, data, fr.inv, fr.M, fr.Mv, op->C); \
+    LAUNCH_CHECK(op);                                                                                   \
+    return MRN_OK;
+      MRN_PIN(1) MRN_PIN(2) MRN_PIN(4) MRN_PIN(8) MRN_PIN(16)
+#undef MRN_PIN
+      default:
+        break;
+    }
+  }
   hipLaunchKernelGGL(k_permute_in, dim3(gs(total)), dim3(256), 0, st, kint, data, fr.perm, fr.M,
                      fr.Mv, op->C, op->g.CC, op->g.lgCC, total);
   LAUNCH_CHECK(op);
@@ -421,6 +510,21 @@ int op_permute_in(mrn_op *op, float2 *kint, const float2 *data, const FrameData 
 int op_permute_out(mrn_op *op, float2 *data, const float2 *kint, const FrameData &fr,
                    const float2 *sub, hipStream_t st) {
   if (fr.M == 0) return MRN_OK;
+  if (permute_by_record(op, kint, fr)) {
+    const dim3 grid((unsigned)((fr.M + 255) / 256), (unsigned)op->nch);
+    switch (op->g.CC) {
+#define MRN_POUT(n)                                                                                    \
+  case n:                                                                                              \
+    hipLaunchKernelGGL(k_permute_out_rec<n>, grid, dim3(256), 0, st, data, kint, fr.inv, sub, fr.M, fr.Mv, \
+                       op->C);                                                                         \
+    LAUNCH_CHECK(op);                                                                                  \
+    return MRN_OK;
+      MRN_POUT(1) MRN_POUT(2) MRN_POUT(4) MRN_POUT(8) MRN_POUT(16)
+#undef MRN_POUT
+      default:
+        break;
+    }
+  }
   hipLaunchKernelGGL(k_permute_out, dim3(gs((int64_t)op->C * fr.M)), dim3(256), 0, st, data, kint,
                      fr.inv, sub, fr.M, fr.Mv, op->C, op->g.CC, op->g.lgCC);
   LAUNCH_CHECK(op);

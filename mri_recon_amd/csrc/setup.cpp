@@ -424,6 +424,7 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.kraw);
     hipFree(fr.traw);
   }
+  op->bind.release();
   hipFree(op->fdev);
   hipFree(op->kwork_all);
   hipFree(op->xrng_all);
