@@ -220,9 +220,12 @@ int mrn_get_fista_thresholds(mrn_op *op, double *out, int n, void *stream);
 /* Trajectory gradient: given X and the upstream gradient g[C][M] of the
  * k-space output y = A(X; k) of one frame (g = dL/dRe y + i dL/dIm y for a
  * real loss L), writes grad_k[M][3] (float32, reference k units)
- * = Re sum_c conj(g_cm) dy_cm/dk = dL/dk.  The oversampled grid is
- * recomputed from X.  No counterpart in the reference (its trajectory is
- * fixed). */
+ * = Re sum_c conj(g_cm) dy_cm/dk = dL/dk.  The oversampled grid of X is
+ * recomputed, except straight after mrn_image_to_kspace of the same frame,
+ * image pointer and stream with no other call on the op in between (and
+ * mrn_get_grid never called): then the grid that call left is used, so X's
+ * contents must not change in between.  No counterpart in the reference (its
+ * trajectory is fixed). */
 int mrn_traj_grad(mrn_op *op, int frame, const void *X, const void *g, float *grad_k,
                   void *stream);
 

@@ -9,8 +9,9 @@ ABI:
   compensation enters only as a 0/1 sample mask), mrn_kspace_to_image;
 * w.r.t. the trajectory: mrn_traj_grad, the derivative of the gather through
   the kernel weights (table slope, or the analytic Kaiser-Bessel derivative in
-  exact_kb mode), which recomputes the oversampled grid from x instead of
-  keeping it from the forward pass.
+  exact_kb mode).  It needs the oversampled grid of x: when nothing has run on
+  the operator since the forward call (same image, same stream) the grid that
+  call left is used, otherwise it is recomputed from x.
 
 Gradients follow torch's convention for complex tensors (the returned image
 gradient is dL/dRe x + i dL/dIm x), so a real loss built from y gives
@@ -55,10 +56,14 @@ class _EncodeFn(torch.autograd.Function):
             ctx.token = owner._token
         gy = _plain(gy)
         gx = gk = None
-        if ctx.needs_input_grad[0]:
-            gx = owner.op.forward(gy, accumulate=False)
+        # the trajectory gradient first: straight after the forward call, on
+        # the same image and stream, the operator's grid still holds that
+        # image's oversampled grid and mrn_traj_grad reuses it (the adjoint
+        # below overwrites the grid)
         if ctx.needs_input_grad[1]:
             gk = owner.op.traj_grad(_plain(x), gy)
+        if ctx.needs_input_grad[0]:
+            gx = owner.op.forward(gy, accumulate=False)
         return gx, gk, None, None
 
 

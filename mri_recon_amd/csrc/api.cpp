@@ -40,7 +40,7 @@ int op_fista_momentum(mrn_op *, float2 *, float2 *, int64_t, hipStream_t);
 int op_fista_finish(mrn_op *, float2 *, float2 *, int64_t, float, double *, hipStream_t, bool);
 int op_cauchy_step(mrn_op *, const float2 *, const float2 *, int64_t, hipStream_t);
 int op_set_step(mrn_op *, float, float, hipStream_t);
-int traj_grad_impl(mrn_op *, int, const float2 *, const float2 *, float *, hipStream_t);
+int traj_grad_impl(mrn_op *, int, const float2 *, const float2 *, float *, hipStream_t, bool);
 int autotune_impl(mrn_op *, int, const char *, hipStream_t);
 
 static inline bool batched(const mrn_op *op) { return op->p.form == MRN_FORM_BATCHED; }
@@ -203,7 +203,7 @@ static int fused_prepare(mrn_op *op, hipStream_t st) {
   }
   std::vector<FrameDev> h(F);
   int64_t off = 0;
-  op->fmax_nsub = op->fmax_nheavy = op->fmax_nwide = 0;
+  op->fmax_nsub = op->fmax_nheavy = op->fmax_nwide = op->fmax_xsplit = 0;
   op->fmax_Mv = 0;
   for (int f = 0; f < F; f++) {
     const FrameData &fr = op->frames[f];
@@ -215,6 +215,7 @@ static int fused_prepare(mrn_op *op, hipStream_t st) {
     d.vlist = fr.vlist;
     d.heavy = fr.heavy;
     d.wide = fr.wide;
+    d.xsplit = fr.xsplit;
     d.kwork = op->kwork_all + off;
     d.kdata = fr.kdata;
     d.Mv = fr.Mv;
@@ -227,6 +228,7 @@ static int fused_prepare(mrn_op *op, hipStream_t st) {
     op->fmax_nsub = std::max(op->fmax_nsub, fr.nsub);
     op->fmax_nheavy = std::max(op->fmax_nheavy, fr.nheavy);
     op->fmax_nwide = std::max(op->fmax_nwide, fr.nwide);
+    op->fmax_xsplit = std::max(op->fmax_xsplit, fr.xsplit_max);
     op->fmax_Mv = std::max(op->fmax_Mv, fr.Mv);
   }
   MRN_HIP(op, hipMemcpyAsync(op->fdev, h.data(), sizeof(FrameDev) * F, hipMemcpyHostToDevice, st));
@@ -623,9 +625,16 @@ int mrn_image_to_kspace(mrn_op *op, int f, const void *X, void *data, const void
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
   if (batched(op)) {
+    op->fwd_f = -1;
     MRN_TRY(fwd_grid(op, (const float2 *)X, &fr, st));
     MRN_TRY(launch_gather(op, op->grid, op->kwork, fr, nullptr, false, st));
-    return op_permute_out(op, (float2 *)data, op->kwork, fr, (const float2 *)sub, st);
+    MRN_TRY(op_permute_out(op, (float2 *)data, op->kwork, fr, (const float2 *)sub, st));
+    // the grid now holds X's forward grid (mrn_traj_grad may reuse it)
+    op->fwd_f = f;
+    op->fwd_X = X;
+    op->fwd_stream = stream;
+    op->fwd_calls = op->calls;
+    return MRN_OK;
   }
   for (int c = 0; c < op->C; c++)
     MRN_TRY(plain_A_coil(op, (const float2 *)X, op->smaps + (int64_t)c * op->nvox,
@@ -901,7 +910,11 @@ int mrn_traj_grad(mrn_op *op, int f, const void *X, const void *g, float *grad_k
   if (!op || !X || !g || !grad_k) return MRN_EINVAL;
   MRN_TRY(check_frame(op, f, false));
   if (!batched(op)) return fail(op, MRN_EUNSUPPORTED, "trajectory gradient needs the batched form");
-  return traj_grad_impl(op, f, (const float2 *)X, (const float2 *)g, grad_k, pick(op, stream));
+  // straight after mrn_image_to_kspace of the same frame, image and stream the
+  // grid already holds X's forward grid
+  const bool grid_ready = !op->grid_exposed && op->fwd_f == f && op->fwd_X == X && op->fwd_stream == stream &&
+                          op->fwd_calls == op->calls;
+  return traj_grad_impl(op, f, (const float2 *)X, (const float2 *)g, grad_k, pick(op, stream), grid_ready);
 }
 
 int mrn_autotune(mrn_op *op, int f, const char *cache_path, void *stream) {
@@ -1033,6 +1046,7 @@ int mrn_get_grid(mrn_op *op, void **grid, int64_t dims[5]) {
   if (!op || !grid || !dims) return MRN_EINVAL;
   const bool b = batched(op);
   *grid = b ? (void *)op->grid : (void *)op->cgrid;
+  op->grid_exposed = true;  // the caller may write it: no grid is reused from now on
   dims[0] = b ? op->nch : 1;
   dims[1] = b ? op->g.CC : 1;
   for (int a = 0; a < 3; a++) dims[2 + a] = op->g.S[a];

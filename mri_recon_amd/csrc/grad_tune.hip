@@ -174,11 +174,11 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
 }
 
 int traj_grad_impl(mrn_op *op, int f, const float2 *X, const float2 *gup, float *grad_k,
-                   hipStream_t st) {
+                   hipStream_t st, bool grid_ready) {
   FrameData &fr = op->frames[f];
   MRN_HIP(op, hipMemsetAsync(grad_k, 0, sizeof(float) * 3 * fr.M, st));
   if (fr.Mv == 0) return MRN_OK;
-  MRN_TRY(fwd_grid_pub(op, X, &fr, st));
+  if (!grid_ready) MRN_TRY(fwd_grid_pub(op, X, &fr, st));
   MRN_TRY(op_permute_in(op, op->kwork, gup, fr, st));
   const Geom &g = op->g;
   dim3 g3((unsigned)((fr.Mv * g.CC + 255) / 256), op->nch);

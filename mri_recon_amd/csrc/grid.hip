@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
     Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum,
-    const FrameDev *__restrict__ fdt) {
+    const FrameDev *__restrict__ fdt, const int32_t *__restrict__ xs) {
   static_assert(CCT == 8 || CCT == 16, "matrix-core scatter needs 8 or 16 coils per cell");
   constexpr int NB = CCT / 8;            // 16-real blocks of a cell's coil values
   constexpr int NG = SM_BATCH / 4;       // sample groups of a batch
@@ -1113,9 +1113,10 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     subs = fd.subs;
     nsub = fd.nsub;
     chunk_k = fd.Mv * g.CC;
+    xs = fd.xsplit;
   }
-  const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
-  if (sidx >= nsub) return;
+  const int sidx = scatter_item(blockIdx.x, gridDim.x, wave, t.xrun, xs);
+  if (sidx < 0 || sidx >= nsub) return;
   const int4 sp = subs[sidx];
   const int tile = sp.x;
   const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
@@ -1270,7 +1271,7 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
     Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum,
-    const FrameDev *__restrict__ fdt) {
+    const FrameDev *__restrict__ fdt, const int32_t *__restrict__ xs) {
   __shared__ float s_w[4][SG_BATCH * 12];    // [sample][axis][offset] weights
   __shared__ float2 s_d[4][SG_BATCH * CCT];  // [sample][coil] data
   extern __shared__ float tab_s[];           // kernel tables (g.ntab floats)
@@ -1288,9 +1289,10 @@ __global__ __launch_bounds__(256, 6) void k_scatter_gd(
     subs = fd.subs;
     nsub = fd.nsub;
     chunk_k = fd.Mv * g.CC;
+    xs = fd.xsplit;
   }
-  const int sidx = xcd_block(blockIdx.x, gridDim.x, t.xrun) * 4 + wave;
-  if (sidx >= nsub) return;
+  const int sidx = scatter_item(blockIdx.x, gridDim.x, wave, t.xrun, xs);
+  if (sidx < 0 || sidx >= nsub) return;
   const int4 sp = subs[sidx];
   const int tile = sp.x;
   const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
@@ -1741,7 +1743,11 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   const int nsub = fdt ? op->fmax_nsub : fr.nsub, nwide = fdt ? op->fmax_nwide : fr.nwide;
   const int64_t Mv = fdt ? op->fmax_Mv : fr.Mv;
   int64_t cg = (int64_t)g.S[0] * g.S[1] * g.S[2] * g.CC, ck = fr.Mv * g.CC;
-  dim3 g3((nsub + 3) / 4, op->nch, fdt ? op->F : 1);
+  // per-XCD ranges (xcd_run < 0): MRN_XCDS workgroup columns, each as long
+  // as the longest range needs
+  const int xmax = fdt ? op->fmax_xsplit : fr.xsplit_max;
+  const bool slabs = t.xrun < 0 && (fdt || fr.xsplit);
+  dim3 g3(slabs ? MRN_XCDS * ((xmax + 3) / 4) : (nsub + 3) / 4, op->nch, fdt ? op->F : 1);
   const size_t sh = sizeof(float) * g.ntab;  // kernel tables in LDS
   // no work item: a frame without valid samples whose x passes are limited to
   // its (empty) k-space support -- nothing reads the grid
@@ -1753,7 +1759,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   if ((g.CC == 16 || g.CC == 8) && !op->scatter_valu && off32) {
 #define MRN_MM(n, km)                                                                                  \
   hipLaunchKernelGGL((k_scatter_mm<n, km>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,      \
-                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt)
+                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt, fr.xsplit)
     if (g.CC == 16 && !exact) MRN_MM(16, 0);
     else if (g.CC == 16) MRN_MM(16, 1);
     else if (!exact) MRN_MM(8, 0);
@@ -1764,7 +1770,7 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
 #define MRN_S(n)                                                                                 \
   case n:                                                                                        \
     hipLaunchKernelGGL((k_scatter_gd<n>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,  \
-                       fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt);              \
+                       fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt, fr.xsplit);   \
     break;
     MRN_S(1) MRN_S(2) MRN_S(4) MRN_S(8) MRN_S(16)
 #undef MRN_S
@@ -1899,6 +1905,34 @@ __global__ void k_plane_yrange(int2 *xrng, int S1, int S2, int2 *yrng) {
   yrng[z] = y1 < 0 ? make_int2(1, 0) : make_int2(y0, y1);
   for (int y = 0; y < S1; y++)
     if (y < y0 || y > y1) row[y] = make_int2(INT_MAX, INT_MIN);
+}
+
+// Cost of scatter work item i (entry n: 0, for the exclusive scan): its
+// visits plus a fixed part for the tile's zeroing and epilogue stores.
+__global__ void k_item_cost(const int4 *__restrict__ subs, int n, int64_t *cost) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cost[i] = max(subs[i].z - subs[i].y, 0) + 32;
+  else if (i == n) cost[i] = 0;
+}
+
+// Boundaries of MRN_XCDS contiguous ranges of the work list of equal cost
+// (pre: exclusive prefix of the costs, n + 1 entries): xs[x] is the first
+// item whose prefix reaches x/MRN_XCDS of the total.  One thread per entry.
+__global__ void k_xcd_split(const int64_t *__restrict__ pre, int n, int32_t *xs) {
+  const int x = threadIdx.x;
+  if (x > MRN_XCDS) return;
+  if (x == 0 || x == MRN_XCDS) {
+    xs[x] = x == 0 ? 0 : n;
+    return;
+  }
+  const int64_t target = pre[n] * x / MRN_XCDS;
+  int lo = 0, hi = n;  // first i in [0, n] with pre[i] >= target
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] >= target) hi = mid;
+    else lo = mid + 1;
+  }
+  xs[x] = lo;
 }
 
 // Sort one frame's samples into tiles and build the scatter work list.  The
@@ -2059,6 +2093,26 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipLaunchKernelGGL(k_fill_subs, dim3(nblk(nt, 256)), dim3(256), 0, st, nsub, off, visits, t,
                      fr.subs, hflag);
   MRN_BH(hipGetLastError());
+  // the work list cut into MRN_XCDS ranges of equal cost (xcd_run < 0)
+  {
+    const int64_t n = nsub_total;
+    MRN_BH(grow(b.icost, b.c_icost, n + 1));
+    MRN_BH(grow(b.ipre, b.c_ipre, n + 1));
+    MRN_BH(grow(fr.xsplit, cp.xsplit, MRN_XCDS + 1));
+    hipLaunchKernelGGL(k_item_cost, dim3(nblk(n + 1, 256)), dim3(256), 0, st, fr.subs, (int)n, b.icost);
+    MRN_BH(hipGetLastError());
+    size_t cb = 0;
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(nullptr, cb, b.icost, b.ipre, (int)n + 1, st));
+    MRN_BH(grow(b.vtmp, b.c_vtmp, (int64_t)cb));
+    MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.vtmp, cb, b.icost, b.ipre, (int)n + 1, st));
+    hipLaunchKernelGGL(k_xcd_split, dim3(1), dim3(64), 0, st, b.ipre, (int)n, fr.xsplit);
+    MRN_BH(hipGetLastError());
+    int32_t xs[MRN_XCDS + 1];
+    MRN_BH(hipMemcpyAsync(xs, fr.xsplit, sizeof(xs), hipMemcpyDeviceToHost, st));
+    MRN_BH(hipStreamSynchronize(st));
+    fr.xsplit_max = 0;
+    for (int x = 0; x < MRN_XCDS; x++) fr.xsplit_max = std::max(fr.xsplit_max, xs[x + 1] - xs[x]);
+  }
   MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, hflag, hoff, nt + 1, st));
   int32_t nh = 0;
   MRN_BH(hipMemcpyAsync(&nh, hoff + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));

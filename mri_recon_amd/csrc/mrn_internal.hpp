@@ -72,6 +72,7 @@ struct FrameDev {
   const int32_t *vlist;
   const int32_t *heavy;
   const int32_t *wide;
+  const int32_t *xsplit;   // [MRN_XCDS + 1] per-XCD ranges of the scatter work list
   float2 *kwork;           // this frame's [nch][Mv][CC] k-space work array
   const float2 *kdata;     // measured k-space, same layout
   int64_t Mv;
@@ -98,7 +99,7 @@ inline hipError_t grow(T *&p, size_t &cap, int64_t n) {
 // Capacities (elements) of a frame's grow-only arrays.
 struct FrameCaps {
   size_t coords = 0, taper = 0, perm = 0, inv = 0, bin_start = 0, subs = 0, heavy = 0, vofs = 0, vlist = 0,
-         gsubs = 0, wide = 0, sparse = 0, xrng = 0, yrng = 0, kdata = 0, kraw = 0, traw = 0;
+         gsubs = 0, wide = 0, sparse = 0, xrng = 0, yrng = 0, kdata = 0, kraw = 0, traw = 0, xsplit = 0;
 };
 
 // Scratch of a trajectory bind, kept by the op between binds (grow-only).
@@ -107,12 +108,14 @@ struct BindScratch {
   int32_t *vals = nullptr, *counts = nullptr, *nsub = nullptr, *visits = nullptr, *off = nullptr,
           *hflag = nullptr, *hoff = nullptr, *wflag = nullptr, *woff = nullptr;
   int2 *brow = nullptr;
-  int64_t *vw = nullptr;
+  int64_t *vw = nullptr, *icost = nullptr, *ipre = nullptr;
   uint8_t *tmp = nullptr, *wtmp = nullptr, *vtmp = nullptr;
   size_t c_keys = 0, c_keys2 = 0, c_vals = 0, c_counts = 0, c_nsub = 0, c_visits = 0, c_off = 0, c_hflag = 0,
-         c_hoff = 0, c_wflag = 0, c_woff = 0, c_brow = 0, c_vw = 0, c_tmp = 0, c_wtmp = 0, c_vtmp = 0;
+         c_hoff = 0, c_wflag = 0, c_woff = 0, c_brow = 0, c_vw = 0, c_icost = 0, c_ipre = 0, c_tmp = 0, c_wtmp = 0,
+         c_vtmp = 0;
   void release() {
-    void *ps[] = {keys, keys2, vals, counts, nsub, visits, off, hflag, hoff, wflag, woff, brow, vw, tmp, wtmp, vtmp};
+    void *ps[] = {keys, keys2, vals, counts, nsub, visits, off, hflag, hoff, wflag, woff, brow, vw, icost, ipre,
+                  tmp, wtmp, vtmp};
     for (void *q : ps) hipFree(q);
     *this = BindScratch{};
   }
@@ -128,6 +131,8 @@ struct FrameData {
   int32_t *inv = nullptr;      // [M] caller index -> tile order, -1 if skipped
   int32_t *bin_start = nullptr;// [nbins + 1] first sample of each bin
   int4 *subs = nullptr;        // [nsub] {tile, first visit, end visit, heavy}
+  int32_t *xsplit = nullptr;   // [MRN_XCDS + 1] work-list ranges of equal cost, one per XCD
+  int xsplit_max = 0;          // longest of those ranges (work items)
   int32_t *heavy = nullptr;    // [nheavy] tiles split over several work items
   int nsub = 0, nheavy = 0;
   int64_t *vofs = nullptr;     // [ntiles + 1] first entry of each tile's visits in vlist
@@ -174,11 +179,20 @@ struct mrn_op {
   int64_t kwork_all_cap = 0;   // elements
   int2 *xrng_all = nullptr;    // [F][Sz][Sy] x support rows (k_row_xrange) of every frame
   int2 *yrng_all = nullptr;    // [F][Sz] y support of every frame, when the y passes are pruned
-  int fmax_nsub = 0, fmax_nheavy = 0, fmax_nwide = 0;
+  int fmax_nsub = 0, fmax_nheavy = 0, fmax_nwide = 0, fmax_xsplit = 0;
   int64_t fmax_Mv = 0;
   float2 *kplain = nullptr;    // plain form: one coil of k-space [M_max]
   int64_t kplain_cap = 0;
   mrn::BindScratch bind;       // trajectory-bind scratch (grow-only)
+  // The grid holds the forward grid of image fwd_X for frame fwd_f (from
+  // mrn_image_to_kspace on fwd_stream) while op->calls == fwd_calls, i.e. no
+  // other call has run on the op since; never once the grid's address was
+  // handed out (mrn_get_grid), since the caller may write it.
+  uint64_t calls = 0, fwd_calls = 0;
+  int fwd_f = -1;
+  const void *fwd_X = nullptr;
+  void *fwd_stream = nullptr;
+  bool grid_exposed = false;
   std::vector<mrn::FrameData> frames;
   hipStream_t stream = nullptr;
   // box matrix-core gather (gather_tiles = 2): the dense tiles run on `side`
@@ -246,7 +260,12 @@ int fail(mrn_op *op, int code, const char *msg);
     if (s_ != MRN_OK) return s_; \
   } while (0)
 
-inline hipStream_t pick(mrn_op *, void *s) { return (hipStream_t)s; }  // NULL = null stream
+// The stream of a call (NULL = null stream).  Every entry point that launches
+// work picks its stream here, so op->calls counts those calls.
+inline hipStream_t pick(mrn_op *op, void *s) {
+  op->calls++;
+  return (hipStream_t)s;
+}
 
 // Owning wrappers, released on every path out of a scope (errors included).
 struct Events {
@@ -438,6 +457,21 @@ __device__ __forceinline__ int xcd_block(int b, int nb, int run) {
   if (run <= 1 || b >= (nb / G) * G) return b;
   const int w = b % G;
   return (b - w) + (w % MRN_XCDS) * run + w / MRN_XCDS;
+}
+
+// Scatter work item of wavefront `wave` (4 per workgroup) of workgroup b.
+// run < 0 with per-XCD ranges xs: the dispatcher deals workgroups to the XCDs
+// in turn, so workgroup b runs on XCD b % MRN_XCDS and takes the next 4 items
+// of that XCD's contiguous range of the work list (tiles in row order): a
+// sample's neighbouring tiles then share one L2.  -1: no item.  Otherwise
+// xcd_block's runs (run <= 1: hardware order).
+__device__ __forceinline__ int scatter_item(int b, int nb, int wave, int run, const int32_t *xs) {
+  if (run < 0 && xs) {
+    const int x = b % MRN_XCDS;
+    const int i = xs[x] + (b / MRN_XCDS) * 4 + wave;
+    return i < xs[x + 1] ? i : -1;
+  }
+  return xcd_block(b, nb, run) * 4 + wave;
 }
 
 // ------------------------------------------------------ FFT pass args

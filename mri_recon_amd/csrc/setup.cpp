@@ -306,7 +306,7 @@ int mrn_op_create(const mrn_params *pin, mrn_op **out) {
   op->t.nbins = op->t.nb[0] * op->t.nb[1] * op->t.nb[2];
   op->t.ntiles = op->t.nt[0] * op->t.nt[1] * op->t.nt[2];
   op->t.maxv = p.max_visits > 0 ? p.max_visits : 4096;
-  op->t.xrun = p.xcd_run > 0 ? p.xcd_run : 1;
+  op->t.xrun = p.xcd_run > 0 ? p.xcd_run : (p.xcd_run < 0 ? -1 : 1);
   op->t.xrun_gather = p.xcd_run_gather > 0 ? p.xcd_run_gather : 64;
   op->gather_tiles = p.gather_tiles == 2 ? 2 : p.gather_tiles != 0 ? 1 : 0;
   op->scatter_valu = p.scatter_valu != 0;
@@ -412,6 +412,7 @@ int mrn_op_destroy(mrn_op *op) {
     hipFree(fr.inv);
     hipFree(fr.bin_start);
     hipFree(fr.subs);
+    hipFree(fr.xsplit);
     hipFree(fr.heavy);
     hipFree(fr.vofs);
     hipFree(fr.vlist);

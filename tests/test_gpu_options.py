@@ -159,6 +159,50 @@ def test_xcd_block_order_same_bits():
     assert rel(outs[0][1], o32.kspace_to_image(d, smaps, traj)) <= 1e-5
 
 
+@pytest.mark.parametrize("C,kw", [(16, {}), (8, {}), (4, {}), (16, {"max_visits": 64}),
+                                  (16, {"scatter_valu": True})])
+def test_xcd_slab_scatter(C, kw):
+    """xcd_run = -1: the scatter's work list is cut into 8 contiguous ranges
+    of equal cost and workgroup b takes its items from range b % 8.  Every
+    item must run exactly once: A^H as with the hardware order (split tiles'
+    atomics aside, the same bits) and as the oracle, for the matrix-core and
+    vector-ALU forms, split tiles, accumulation and three fused frames of
+    different sizes."""
+    N = 32
+    traj, X, smaps, d = problem(N, C, 91)
+    o32 = orc.OracleOp(N, overgrid=1.5, dwin=2.5, precision="f32")
+    outs = []
+    for run in (1, -1):
+        op = GridFFT(N, ncoils=C, overgrid=1.5, dwin=2.5, xcd_run=run, **kw)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        h = op.forward(to_dev(d)).cpu().numpy()
+        h2 = op.forward(to_dev(d), X=op.forward(to_dev(d))).cpu().numpy()  # accumulating form
+        outs.append((h, h2))
+    print("C=%d %s: slabs vs hardware order %.1e / %.1e, vs oracle %.1e"
+          % (C, kw, rel(outs[1][0], outs[0][0]), rel(outs[1][1], outs[0][1]),
+             rel(outs[1][0], o32.kspace_to_image(d, smaps, traj))))
+    for i in range(2):
+        if "max_visits" in kw:
+            assert rel(outs[1][i], outs[0][i]) <= 1e-6
+        else:
+            assert np.array_equal(outs[1][i], outs[0][i])
+    assert rel(outs[1][0], o32.kspace_to_image(d, smaps, traj)) <= 1e-5
+    # fused frames of uneven length: each frame's own ranges
+    frames = [[t.numpy() for t in tr] for tr in synth.gated_radial_frames(3, 90, N, N, seed=5)]
+    frames[2] = [t[: t.size // 4].copy() for t in frames[2]]
+    res = []
+    for run in (1, -1):
+        op = GridFFT(N, ncoils=C, nframes=3, overgrid=1.5, dwin=2.5, fuse_frames=1, xcd_run=run, **kw)
+        op.set_smaps(to_dev(smaps))
+        for f, tr in enumerate(frames):
+            op.set_traj(*traj_dev(tr), frame=f)
+            op.set_kdata(to_dev(o32.image_to_kspace(X, smaps, tr)), frame=f)
+        res.append(op.cg(3)[0].cpu().numpy())
+    print("fused frames CG: slabs vs hardware order %.1e" % rel(res[1], res[0]))
+    assert rel(res[1], res[0]) <= 1e-6
+
+
 def test_kspace_support_pruning_across_frames():
     """With 16-coil blocks and a four-step grid the forward x pass stores, and
     the inverse x pass loads, only the x cells that some sample of the frame
