@@ -41,6 +41,21 @@ __device__ __forceinline__ float2 add(float2 a, float2 b) { return make_float2(a
 __device__ __forceinline__ float2 sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 scl(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
+// Grid and coil-map traffic of the four-step passes: each element is read
+// once and written once per pass, far more than the caches hold, so these
+// loads and stores are nontemporal (streamed past the caches' keep policy).
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_nt(const float2 *p) {
+  const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2 *>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(float2 *p, float2 v) {
+  nt_f2 q;
+  q.x = v.x;
+  q.y = v.y;
+  __builtin_nontemporal_store(q, reinterpret_cast<nt_f2 *>(p));
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2 *x);
 
@@ -595,7 +610,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
   for (int j = 0; j < P; j++) {
     const int p = t + 16 * j;
     v[j] = make_float2(0.f, 0.f);
-    if (ok && p >= ld0 && p < ld1) v[j] = ln[(int64_t)p * a.pstride];
+    if (ok && p >= ld0 && p < ld1) v[j] = ld_nt(ln + (int64_t)p * a.pstride);
   }
   load_twiddles<S>(tws, tw);
   __syncthreads();
@@ -608,7 +623,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
 #pragma unroll
       for (int k2 = 0; k2 < 16; k2++) {
         const int p = k1 + P * k2;
-        if (p >= st0 && p < st1) ln[(int64_t)p * a.pstride] = scl(w[k2], a.scale);
+        if (p >= st0 && p < st1) st_nt(ln + (int64_t)p * a.pstride, scl(w[k2], a.scale));
       }
     }
   }
@@ -646,7 +661,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
     if (lok && zi >= 0 && zi < a.N[2]) {
       const int64_t r = ((int64_t)zi * a.N[1] + y) * a.N[0];
       xv[j] = X[r + xi];
-      sv[j] = sm[(((int64_t)y * a.N[0] + xi) * a.N[2] + zi) * CC + (gl & (CC - 1))];
+      sv[j] = ld_nt(sm + (((int64_t)y * a.N[0] + xi) * a.N[2] + zi) * CC + (gl & (CC - 1)));
     }
   }
   // the z window in LDS with the twiddles, while the loads are in flight
@@ -674,7 +689,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pre_z4(
     fft4_cols<S, -1>(w, buf + l * LD + k1 * 16, tws);
     if (lok) {
 #pragma unroll
-      for (int k2 = 0; k2 < 16; k2++) gch[(int64_t)(k1 + P * k2) * pstride] = scl(w[k2], a.scale);
+      for (int k2 = 0; k2 < 16; k2++) st_nt(gch + (int64_t)(k1 + P * k2) * pstride, scl(w[k2], a.scale));
     }
   }
 }
@@ -743,7 +758,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
     const int zi = t + P * k2 - ogz;
     smv[k2] = xo[k2] = make_float2(0.f, 0.f);
     if (xok && t < P && zi >= 0 && zi < a.N[2]) {
-      smv[k2] = smc[(int64_t)zi * CC];
+      smv[k2] = ld_nt(smc + (int64_t)zi * CC);
       if (!a.overwrite && c == 0) xo[k2] = X[xr + (int64_t)zi * zst];
     }
   }
@@ -753,7 +768,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
 #pragma unroll
   for (int j = 0; j < P; j++) {
     v[j] = make_float2(0.f, 0.f);
-    if (gl < nl) v[j] = src[(int64_t)(t + 16 * j) * pstride];
+    if (gl < nl) v[j] = ld_nt(src + (int64_t)(t + 16 * j) * pstride);
   }
   for (int q = threadIdx.x; q < S; q += FFT_THREADS) {
     tws[q] = tw[q];
@@ -771,7 +786,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z4(
       const int zi = k1 + P * k2 - ogz;
       smv[k2] = xo[k2] = make_float2(0.f, 0.f);
       if (xok && zi >= 0 && zi < a.N[2]) {
-        smv[k2] = smc[(int64_t)zi * CC];
+        smv[k2] = ld_nt(smc + (int64_t)zi * CC);
         if (!a.overwrite && c == 0) xo[k2] = X[xr + (int64_t)zi * zst];
       }
     }
