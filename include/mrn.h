@@ -242,13 +242,15 @@ typedef struct mrn_tuning {
   int32_t tile[3];        /* scatter tile edge (4, 4, 4) */
   int32_t from_cache;     /* 1: the last mrn_autotune applied a cached choice without timing */
   float normal_ms;        /* E'E time of the tuned frame with the choice (timed or cached) */
-  int32_t reserved[3];
+  int32_t gather_tiles;   /* 0: direct gather; 2: box matrix-core gather of dense tiles beside it */
+  int32_t reserved[2];
 } mrn_tuning;
 
 /* Time the batched form's launch shapes on one frame's trajectory and keep
  * the fastest: scatter bin edge x samples per work item x accumulate form,
  * gather coils per lane x XCD run length, the FFT passes' XCD run length,
- * and the FFT pass form (four-step with pruning or Stockham).  The choice
+ * and the FFT pass form (four-step with pruning or Stockham).  The gather
+ * form (gather_tiles) stays the op's own.  The choice
  * is looked up in, or else added to, the JSON file cache_path (NULL: no
  * cache), keyed by sizes, kernel, sample and coil counts and device; a hit
  * applies it without timing.  On failure the previous shapes are restored.

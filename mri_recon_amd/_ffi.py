@@ -46,7 +46,8 @@ class MrnTuning(ctypes.Structure):
         ("tile", ctypes.c_int32 * 3),
         ("from_cache", ctypes.c_int32),
         ("normal_ms", ctypes.c_float),
-        ("reserved", ctypes.c_int32 * 3),
+        ("gather_tiles", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

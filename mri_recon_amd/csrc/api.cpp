@@ -939,6 +939,7 @@ int mrn_get_tuning(const mrn_op *op, mrn_tuning *out) {
   out->scatter_valu = (op->scatter_valu || (op->g.CC != 16 && op->g.CC != 8)) ? 1 : 0;
   for (int a = 0; a < 3; a++) out->tile[a] = op->t.T;
   out->from_cache = op->tune_from_cache;
+  out->gather_tiles = op->gather_tiles;
   out->normal_ms = op->tune_ms;
   return MRN_OK;
 }

@@ -6,12 +6,14 @@ Candidates per knob (one knob at a time from the defaults, each an op built
 with that setting): scatter bin edge {1, 2}, samples per work item {1024,
 4096, 16384} and a small value that splits tiles over work items, matrix-core
 or vector-ALU tile accumulate, gather coils per lane {1, 2, 4, 8}, gather XCD
-run {16, 64, 256}, FFT XCD run {1, 4, 8, 16}, four-step or Stockham FFT.
+run {16, 64, 256}, FFT XCD run {1, 4, 8, 16}, four-step or Stockham FFT;
+and the op's own gather form, direct or box matrix-core for the dense tiles.
 """
 import json
 import os
 
 import numpy as np
+import torch
 import pytest
 
 from oracle import oracle as orc
@@ -24,7 +26,7 @@ N, C = 32, 16
 CANDIDATES = ([dict(bin_edge=b) for b in (1, 2)] + [dict(max_visits=v) for v in (64, 1024, 4096, 16384)] +
               [dict(scatter_valu=v) for v in (False, True)] + [dict(gather_coils=c) for c in (1, 2, 4, 8)] +
               [dict(xcd_run_gather=r) for r in (16, 64, 256)] + [dict(xcd_run_fft=r) for r in (1, 4, 8, 16)] +
-              [dict(fft_stockham=s) for s in (False, True)])
+              [dict(fft_stockham=s) for s in (False, True)] + [dict(gather_tiles=g) for g in (0, 2)])
 
 
 @pytest.fixture(scope="module")
@@ -104,3 +106,26 @@ def test_autotune_failure_restores_previous_shapes(problem):
     h = op.forward(to_dev(d), accumulate=False).cpu().numpy()
     assert rel(h, H32) <= 1e-5
     op.close()
+
+
+def test_box_gather_in_captured_cg(problem):
+    """The box gather runs on a side stream (fork and join by events, two
+    branches of the captured CG graph).  Three graph replays, with eager
+    calls and synchronisations between them, each equal the eager CG with the
+    direct gather (FP32 summation order)."""
+    traj, X, smaps, d, A32, H32 = problem
+    box, direct = _op(problem, gather_tiles=2), _op(problem)
+    for op in (box, direct):
+        op.set_kdata(to_dev(d))
+    assert box.tuning()["gather_tiles"] == 2 and direct.tuning()["gather_tiles"] == 0
+    ref, tref = direct.cg(4, graph=False)
+    ref = ref.cpu().numpy()
+    for rep in range(3):
+        Xg, tg = box.cg(4, graph=True)
+        e = rel(Xg.cpu().numpy(), ref)
+        print("box gather, graph replay %d: CG vs eager direct %.1e" % (rep, e))
+        assert e <= 1e-6 and np.allclose(tg.numpy(), tref.numpy(), rtol=1e-5, atol=1e-6 * np.abs(tref.numpy()).max())
+        box.backward(to_dev(X))  # eager work between replays
+        torch.cuda.synchronize()
+    box.close()
+    direct.close()
