@@ -131,16 +131,49 @@ __global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ gr
         d[a][t] = in ? tap_dweight(g, a, tab_s + g.toff[a], dk[a], l) : 0.0f;
         o[a][t] = l;
       }
+    // rows of NT taps, software-pipelined as in the gather: row r + 1's loads
+    // are issued before row r's products (two register buffers, held in that
+    // order by the scheduling barriers).  Per row the x sums with the weights
+    // and with their derivatives, then one update per axis:
+    //   d/dkx += wy wz sum_x dwx G,  d/dky += dwy wz sum_x wx G,
+    //   d/dkz += wy dwz sum_x wx G.
+    CoilVec<NC> vb[2][NT];
+    auto load_row = [&](int r, CoilVec<NC> *v) {
+      const float2 *row = G + ((int64_t)o[2][r / NT] * g.S[1] + o[1][r % NT]) * g.S[0] * g.CC;
 #pragma unroll
-    for (int tz = 0; tz < NT; tz++)
+      for (int tx = 0; tx < NT; tx++) v[tx] = CoilVec<NC>::load(row + (int64_t)o[0][tx] * g.CC);
+    };
+    load_row(0, vb[0]);
 #pragma unroll
-      for (int ty = 0; ty < NT; ty++) {
-        const float2 *row = G + ((int64_t)o[2][tz] * g.S[1] + o[1][ty]) * g.S[0] * g.CC;
-        const float wzy = w[1][ty] * w[2][tz], dzy = w[1][ty] * d[2][tz], wzd = d[1][ty] * w[2][tz];
+    for (int r = 0; r < NT * NT; r++) {
+      if (r + 1 < NT * NT) load_row(r + 1, vb[(r + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      const int tz = r / NT, ty = r % NT;
+      float sdr[NC], sdi[NC], swr[NC], swi[NC];
 #pragma unroll
-        for (int tx = 0; tx < NT; tx++)
-          tap(row + (int64_t)o[0][tx] * g.CC, d[0][tx] * wzy, w[0][tx] * wzd, w[0][tx] * dzy);
+      for (int q = 0; q < NC; q++) sdr[q] = sdi[q] = swr[q] = swi[q] = 0.f;
+#pragma unroll
+      for (int tx = 0; tx < NT; tx++)
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+          const float2 v = vb[r & 1][tx].v[q];
+          sdr[q] += d[0][tx] * v.x;
+          sdi[q] += d[0][tx] * v.y;
+          swr[q] += w[0][tx] * v.x;
+          swi[q] += w[0][tx] * v.y;
+        }
+      const float wzy = w[1][ty] * w[2][tz], wzd = d[1][ty] * w[2][tz], dzy = w[1][ty] * d[2][tz];
+#pragma unroll
+      for (int q = 0; q < NC; q++) {
+        dr[0][q] += wzy * sdr[q];
+        di[0][q] += wzy * sdi[q];
+        dr[1][q] += wzd * swr[q];
+        di[1][q] += wzd * swi[q];
+        dr[2][q] += dzy * swr[q];
+        di[2][q] += dzy * swi[q];
       }
+      __builtin_amdgcn_sched_barrier(0);
+    }
   } else {
     for (int lz = s[2]; lz <= e[2]; lz++) {
       const float wz = tap_weight(g, 2, tab_s + g.toff[2], dk[2], lz);
