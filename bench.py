@@ -725,8 +725,9 @@ def llr_timing(a, N, log):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / reps
     res = {"ms": ms, "frames": F, "image": [N, N, N], "blocks": (N // 4) ** 3, "passes": 5,
-           "GBps": 5 * 2 * 8 * F * N ** 3 / (ms * 1e-3) / 1e9,
-           "bytes_model": "update_threshold reads the image once, each of the 4 shifted passes reads and writes it"}
+           "GBps": 13 * 8 * F * N ** 3 / (ms * 1e-3) / 1e9,
+           "bytes_model": "update_threshold reads the image once; each of the 4 shifted passes reads it twice "
+                          "(Gram matrix, then the product) and writes it once"}
     del X
     log("LLR (%d^3 x %d frames, 4 passes): %.1f ms" % (N, F, ms))
     if not a.no_cpu:

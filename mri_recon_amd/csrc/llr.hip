@@ -3,14 +3,16 @@
 // recon_lib calls them after each gradient step (recon_lib.cpp:2117-2126).
 //
 // Block b of a pass is the Np x N matrix A (N = frames along `dim`, Np = the
-// other frames x block voxels).  One wavefront per block:
-//   1. gather A into LDS (periodic wrap, the pass's shift);
-//   2. G = A^H A in FP64;
+// other frames x block voxels).  One wavefront (or half of one) per block:
+//   1-2. G = A^H A in FP64, A's rows streamed through LDS one chunk of LPB
+//      rows at a time (periodic wrap, the pass's shift), so a block holds
+//      N x (LPB + 1) values of A in LDS instead of all Np rows;
 //   3. G = V diag(lambda) V^H by cyclic complex Jacobi with the round-robin
 //      parallel order (N/2 disjoint rotations at a time, FP64, in LDS);
 //   4. update_threshold: the block's largest singular value sqrt(max lambda);
 //      thresh: A <- A V diag(f) V^H, f = max(s - tau, 0) / s, s = sqrt(lambda),
-//      which is U diag(max(s - tau, 0)) V^H without forming U.
+//      which is U diag(max(s - tau, 0)) V^H without forming U (A's chunks
+//      read again from the image and written back in place).
 // The threshold is the median of the nonzero block maxima (exact radix select
 // on their bit patterns, all on the device) times 1 + amp*exp(-iter/t2), times
 // alpha, over block_iter.  Everything after the launch parameters lives in
@@ -61,8 +63,8 @@ __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x *
 __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
 // row-major N x N complex matrices in LDS, rows padded to N + 1 entries and
-// A's columns to Np + 1, so that lanes walking a column (or reading several
-// columns at one row) fall on different LDS banks
+// the chunk's columns to LPB + 1, so that lanes walking a column (or reading
+// several columns at one row) fall on different LDS banks
 #define GM(r, c) G[(r) * LDG + (c)]
 #define VM(r, c) V[(r) * LDG + (c)]
 
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
   float2 *A = reinterpret_cast<float2 *>(lds + g.lds_A);
   double *R = reinterpret_cast<double *>(lds + g.lds_R);  // rotations (c, s, e.x, e.y), then f
   const int lane = threadIdx.x % LPB;
-  const int N = g.N, Np = g.Np, LDG = N + 1, LDA = Np + 1;
+  const int N = g.N, Np = g.Np, LDG = N + 1;
   const long long bq = (long long)blockIdx.x * BPW + half_id;
   const bool live = bq < g.nblk;  // a padding half repeats the last block and writes nothing
   const long long b = live ? bq : g.nblk - 1;
@@ -127,15 +129,48 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
   auto frame_of = [&](int col, int o) -> long long {
     return g.dim == 0 ? ((long long)col * g.Ne + o) * g.nvox : ((long long)o * g.Ne + col) * g.nvox;
   };
-  // 1. gather
-  for (int row = lane; row < Np; row += LPB) {
-    int o;
-    const long long vo = row_of(row, o);
-    for (int col = 0; col < N; col++) A[col * LDA + row] = X[frame_of(col, o) + vo];
-  }
-  __syncthreads();
-  // 2. Gram matrix, upper triangle by lanes, mirrored
+  // chunk of A: rows r0 .. r0 + LPB - 1, C[col][lane] (zero past Np)
+  const int LDC = LPB + 1;
+  auto load_chunk = [&](int r0) {
+    const int row = r0 + lane;
+    if (row < Np) {
+      int o;
+      const long long vo = row_of(row, o);
+      for (int col = 0; col < N; col++) A[col * LDC + lane] = X[frame_of(col, o) + vo];
+    } else {
+      for (int col = 0; col < N; col++) A[col * LDC + lane] = make_float2(0.f, 0.f);
+    }
+  };
+  // 1-2. Gram matrix: lanes own the pairs a <= c of the upper triangle and add
+  // each chunk's sum to them, then mirror
   const int npair = N * (N + 1) / 2;
+  for (int u = lane; u < N * N; u += LPB) {
+    GM(u / N, u % N) = make_double2(0.0, 0.0);
+    VM(u / N, u % N) = make_double2(u / N == u % N ? 1.0 : 0.0, 0.0);
+  }
+  for (int r0 = 0; r0 < Np; r0 += LPB) {
+    load_chunk(r0);
+    __syncthreads();
+    const int nr = Np - r0 < LPB ? Np - r0 : LPB;
+    for (int u = lane; u < npair; u += LPB) {
+      int a = 0, rem = u;
+      while (rem >= N - a) {
+        rem -= N - a;
+        a++;
+      }
+      const int c = a + rem;
+      const float2 *Aa = A + a * LDC, *Ac = A + c * LDC;
+      double sr = 0.0, si = 0.0;
+      for (int r = 0; r < nr; r++) {
+        const double xr = Aa[r].x, xi = Aa[r].y, yr = Ac[r].x, yi = Ac[r].y;
+        sr += xr * yr + xi * yi;  // conj(x) * y
+        si += xr * yi - xi * yr;
+      }
+      const double2 gv = GM(a, c);
+      GM(a, c) = make_double2(gv.x + sr, gv.y + si);
+    }
+    __syncthreads();  // the chunk is read before the next one replaces it
+  }
   for (int u = lane; u < npair; u += LPB) {
     int a = 0, rem = u;
     while (rem >= N - a) {
@@ -143,17 +178,11 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
       a++;
     }
     const int c = a + rem;
-    const float2 *Aa = A + (long long)a * LDA, *Ac = A + (long long)c * LDA;
-    double sr = 0.0, si = 0.0;
-    for (int r = 0; r < Np; r++) {
-      const double xr = Aa[r].x, xi = Aa[r].y, yr = Ac[r].x, yi = Ac[r].y;
-      sr += xr * yr + xi * yi;  // conj(x) * y
-      si += xr * yi - xi * yr;
+    if (c != a) {
+      const double2 gv = GM(a, c);
+      GM(c, a) = make_double2(gv.x, -gv.y);
     }
-    GM(a, c) = make_double2(sr, si);
-    GM(c, a) = make_double2(sr, -si);
   }
-  for (int u = lane; u < N * N; u += LPB) VM(u / N, u % N) = make_double2(u / N == u % N ? 1.0 : 0.0, 0.0);
   __syncthreads();
   // 3. Jacobi sweeps
   const int Npl = N + (N & 1), n = Npl - 1, half = Npl / 2;
@@ -269,20 +298,27 @@ __global__ __launch_bounds__(64) void k_llr_block(float2 *__restrict__ X, LlrGeo
     GM(a, c) = make_double2(pr, pi);
   }
   __syncthreads();
-  // A P back to the image
-  for (int row = lane; row < Np; row += LPB) {
-    int o;
-    const long long vo = row_of(row, o);
-    for (int col = 0; col < N; col++) {
-      double xr = 0.0, xi = 0.0;
-      for (int a = 0; a < N; a++) {
-        const float2 av = A[(long long)a * LDA + row];
-        const double2 pv = GM(a, col);
-        xr += (double)av.x * pv.x - (double)av.y * pv.y;
-        xi += (double)av.x * pv.y + (double)av.y * pv.x;
+  // A P back to the image, chunk by chunk (a chunk's rows are read before
+  // they are written; blocks of a pass share no voxel)
+  for (int r0 = 0; r0 < Np; r0 += LPB) {
+    load_chunk(r0);
+    __syncthreads();
+    const int row = r0 + lane;
+    if (row < Np) {
+      int o;
+      const long long vo = row_of(row, o);
+      for (int col = 0; col < N; col++) {
+        double xr = 0.0, xi = 0.0;
+        for (int a = 0; a < N; a++) {
+          const float2 av = A[a * LDC + lane];
+          const double2 pv = GM(a, col);
+          xr += (double)av.x * pv.x - (double)av.y * pv.y;
+          xi += (double)av.x * pv.y + (double)av.y * pv.x;
+        }
+        if (live) X[frame_of(col, o) + vo] = make_float2((float)xr, (float)xi);
       }
-      if (live) X[frame_of(col, o) + vo] = make_float2((float)xr, (float)xi);
     }
+    __syncthreads();
   }
 }
 #undef GM
@@ -411,10 +447,12 @@ static int llr_geom(const mrn_llr_params *p, int Nt, int Ne, const int32_t N[3],
   g.lds_R = g.lds_V + 16 * g.N * (g.N + 1);
   g.lds_A = g.lds_R + 8 * (4 * (Npl / 2) > g.N ? 4 * (Npl / 2) : g.N);
   g.lds_A = (g.lds_A + 15) & ~15;
-  const long long total = g.lds_A + 8ll * (g.Np + 1) * g.N;
-  if (total > LLR_LDS) return (err = "low-rank block matrix too large for LDS (Np x N > about 4096)", MRN_EUNSUPPORTED);
-  g.lds_block = (int)((total + 15) & ~15ll);
-  g.lpb = (g.N <= 16 && 2ll * g.lds_block <= LLR_LDS) ? 32 : 64;
+  // one block's LDS with a chunk of lpb rows of A
+  auto block_bytes = [&](int lpb) { return (int)((g.lds_A + 8ll * (lpb + 1) * g.N + 15) & ~15ll); };
+  g.lpb = (g.N <= 16 && 2ll * block_bytes(32) <= LLR_LDS) ? 32 : 64;
+  g.lds_block = block_bytes(g.lpb);
+  if ((long long)g.lds_block * (64 / g.lpb) > LLR_LDS)
+    return (err = "low-rank block matrix too large for LDS", MRN_EUNSUPPORTED);
   g.nblk = (long long)g.nb[0] * g.nb[1] * g.nb[2];
   return MRN_OK;
 }

@@ -33,6 +33,9 @@ CASES = [
     (4, 2, (10, 9, 8), 0, (3, 5, 2), 3, 0.05),      # other block shapes
     (32, 1, (8, 8, 8), 0, (2, 2, 2), 1, 0.02),      # 32 frames, one unshifted pass
     (3, 1, (4, 4, 4), 0, (8, 8, 8), 2, 0.1),        # blocks clamped to the volume
+    # block rows stream through LDS in chunks of 32 or 64: more rows than fit at once
+    (4, 1, (16, 16, 16), 0, (16, 16, 16), 2, 0.1),  # one block of 4096 rows
+    (6, 2, (8, 8, 16), 0, (8, 8, 8), 2, 0.05),      # 2 encodes x 512 voxels per block row set
 ]
 
 
@@ -73,9 +76,6 @@ def test_llr_errors_and_flags():
     X = torch.zeros((40, 1, 8, 8, 8), dtype=torch.complex64, device="cuda")
     with pytest.raises(MrnError, match="32 frames"):
         llr_threshold(X, alpha_time=0.1)
-    X = torch.zeros((4, 1, 16, 16, 16), dtype=torch.complex64, device="cuda")
-    with pytest.raises(MrnError, match="LDS"):
-        llr_threshold(X, alpha_time=0.1, block=(16, 16, 16))
     p = llr_params(args=["-clear_nx", "3", "-clear_ny", "5", "-clear_nz", "2", "-clear_block_iter", "6",
                          "-clear_alpha_time", "0.25", "-clear_amp", "2", "-clear_t2", "9", "-x"])
     assert (tuple(p.block), p.block_iter, p.amp, p.t2) == ((3, 5, 2), 6, 2.0, 9.0)
