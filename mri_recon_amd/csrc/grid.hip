@@ -1152,8 +1152,9 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   for (int p = 0; p < 4; p++)
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[p][nb] = mrn_v4f{0.f, 0.f, 0.f, 0.f};
-  float4 kraw = make_float4(0.f, 0.f, 0.f, 0.f);
-  int inext = 0;  // indices of the batch after the current one
+  // coordinates of this batch and the next; visit indices of the next two
+  float4 kraw = make_float4(0.f, 0.f, 0.f, 0.f), kraw1 = kraw;
+  int inext = 0, inext2 = 0;
   auto batch = [&](int base, const float (&cur)[NG][NB], float (&nxt)[NG][NB]) {
     // ---- A: weights of the batch
     const bool ok = base + sl < vend;
@@ -1186,14 +1187,18 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     unsigned long long pm[4];
 #pragma unroll
     for (int p = 0; p < 4; p++) pm[p] = __ballot(any && h == 0 && p >= lo[2] && p <= hi[2]);
-    // ---- the next batch: coordinates and coil values (its indices came
-    // one batch earlier), and the indices of the batch after it
+    // ---- ahead: the next batch's coil values, the coordinates of the batch
+    // after it (step A of the next batch starts with coordinates loaded one
+    // batch earlier still), and the visit indices one batch further on
     const int nbase = base + SM_BATCH;
-    if (nbase < vend) {
-      kraw = *reinterpret_cast<const float4 *>(cb + (uint32_t)inext * 16u);
-      load_data(inext, nxt);
-      inext = visit(nbase + SM_BATCH + sl);
-    }
+    if (nbase < vend) load_data(inext, nxt);
+    float4 kraw2 = kraw1;
+    if (nbase + SM_BATCH < vend) kraw2 = *reinterpret_cast<const float4 *>(cb + (uint32_t)inext2 * 16u);
+    const int inext3 = visit(nbase + 2 * SM_BATCH + sl);
+    kraw = kraw1;
+    kraw1 = kraw2;
+    inext = inext2;
+    inext2 = inext3;
     wave_sync();
     // ---- B: accumulate, plane by plane
 #pragma unroll
@@ -1217,7 +1222,9 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   if (vbeg < vend) {
     const int i0 = visit(vbeg + sl);
     inext = visit(vbeg + SM_BATCH + sl);
+    inext2 = visit(vbeg + 2 * SM_BATCH + sl);
     kraw = *reinterpret_cast<const float4 *>(cb + (uint32_t)i0 * 16u);
+    kraw1 = *reinterpret_cast<const float4 *>(cb + (uint32_t)inext * 16u);
     load_data(i0, dA);
   }
   for (int base = vbeg; base < vend; base += 2 * SM_BATCH) {
