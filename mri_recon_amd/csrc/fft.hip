@@ -497,6 +497,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_post_z(
 // so that the 16 lines of a column read start in distinct banks.
 __host__ __device__ constexpr int fft4_ld(int S) { return S + (34 - S % 32) % 32; }
 __host__ __device__ constexpr bool fft4_ok(int S) { return S % 16 == 0 && S >= 64 && S <= 512; }
+// Line passes whose float2 transpose buffer would hold a CU to fewer than 3
+// workgroups (S >= 400) move it as floats, real parts then imaginary parts:
+// rows padded to 4 words mod 64 (16-byte aligned columns, the 16 lines'
+// column reads starting in distinct banks).
+__host__ __device__ constexpr bool fft4_split(int S) {
+  return (FFT_LINES * fft4_ld(S) * 8 + S * 8) * 3 > 160 * 1024;
+}
+__host__ __device__ constexpr int fft4_ldf(int S) { return S + (68 - S % 64) % 64; }
 
 // Stockham stages of a P-point DFT held in registers (natural order in and
 // out); W_P^m = tws[m TSC], tws the S-point table.
@@ -531,6 +539,19 @@ __device__ __forceinline__ void reg_fft(float2 *v, const float2 *tws) {
 }
 
 // Rows: DFT over j, twiddle, write column entries row[k1 * 16 + t].
+// Rows without the LDS write: DFT over j and twiddle, in place in v.
+template <int S, int DIR>
+__device__ __forceinline__ void fft4_rows_tw(float2 *v, int t, const float2 *tws) {
+  constexpr int P = S / 16;
+  reg_fft<P, DIR, 16, 0, 1>(v, tws);
+#pragma unroll
+  for (int k1 = 1; k1 < P; k1++) {
+    float2 tw = tws[t * k1];
+    if (DIR > 0) tw.y = -tw.y;
+    v[k1] = cmul(v[k1], tw);
+  }
+}
+
 template <int S, int DIR>
 __device__ __forceinline__ void fft4_rows(float2 *v, int t, float2 *row, const float2 *tws) {
   constexpr int P = S / 16;
@@ -580,8 +601,10 @@ __device__ __forceinline__ float2 group_sum(float2 t, int cc) {
 template <int S, int DIR>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__ grid, LinesArgs a,
                                                             const float2 *__restrict__ tw) {
-  constexpr int P = S / 16, LD = fft4_ld(S);
-  __shared__ float4 buf4[FFT_LINES * LD / 2];
+  constexpr int P = S / 16;
+  constexpr bool SPLIT = fft4_split(S);
+  constexpr int LD = SPLIT ? fft4_ldf(S) : fft4_ld(S);  // floats or float2 per line
+  __shared__ float4 buf4[SPLIT ? FFT_LINES * LD / 4 : FFT_LINES * LD / 2];
   __shared__ float2 tws[S];
   float2 *buf = reinterpret_cast<float2 *>(buf4);
   const int64_t base = a.off0 + (int64_t)blockIdx.y * a.ystride + (int64_t)blockIdx.z * a.zstride;
@@ -614,16 +637,71 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines4(float2 *__restrict__
   }
   load_twiddles<S>(tws, tw);
   __syncthreads();
-  fft4_rows<S, DIR>(v, t, buf + l * LD, tws);
-  __syncthreads();
-  for (int k1 = t; k1 < P; k1 += 16) {
-    float2 w[16];
-    fft4_cols<S, DIR>(w, buf + l * LD + k1 * 16, tws);
-    if (ok) {
+  if constexpr (!SPLIT) {
+    fft4_rows<S, DIR>(v, t, buf + l * LD, tws);
+    __syncthreads();
+    for (int k1 = t; k1 < P; k1 += 16) {
+      float2 w[16];
+      fft4_cols<S, DIR>(w, buf + l * LD + k1 * 16, tws);
+      if (ok) {
 #pragma unroll
-      for (int k2 = 0; k2 < 16; k2++) {
-        const int p = k1 + P * k2;
-        if (p >= st0 && p < st1) st_nt(ln + (int64_t)p * a.pstride, scl(w[k2], a.scale));
+        for (int k2 = 0; k2 < 16; k2++) {
+          const int p = k1 + P * k2;
+          if (p >= st0 && p < st1) st_nt(ln + (int64_t)p * a.pstride, scl(w[k2], a.scale));
+        }
+      }
+    }
+  } else {
+    // real parts through LDS, then imaginary parts; thread (l, u) keeps the
+    // real parts of its columns u and u + 16 between the two
+    constexpr int NCOL = (P + 15) / 16;
+    float *bf = reinterpret_cast<float *>(buf4) + l * LD;
+    fft4_rows_tw<S, DIR>(v, t, tws);
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) bf[k1 * 16 + t] = v[k1].x;
+    __syncthreads();
+    float re[NCOL][16];
+#pragma unroll
+    for (int c = 0; c < NCOL; c++) {
+      const int k1 = t + 16 * c;
+      if (k1 < P) {
+        const float4 *c4 = reinterpret_cast<const float4 *>(bf + k1 * 16);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float4 q = c4[n];
+          re[c][4 * n] = q.x;
+          re[c][4 * n + 1] = q.y;
+          re[c][4 * n + 2] = q.z;
+          re[c][4 * n + 3] = q.w;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) bf[k1 * 16 + t] = v[k1].y;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCOL; c++) {
+      const int k1 = t + 16 * c;
+      if (k1 < P) {
+        float2 w[16];
+        const float4 *c4 = reinterpret_cast<const float4 *>(bf + k1 * 16);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float4 q = c4[n];
+          w[4 * n] = make_float2(re[c][4 * n], q.x);
+          w[4 * n + 1] = make_float2(re[c][4 * n + 1], q.y);
+          w[4 * n + 2] = make_float2(re[c][4 * n + 2], q.z);
+          w[4 * n + 3] = make_float2(re[c][4 * n + 3], q.w);
+        }
+        reg_fft<16, DIR, S / 16, 0, 1>(w, tws);
+        if (ok) {
+#pragma unroll
+          for (int k2 = 0; k2 < 16; k2++) {
+            const int p = k1 + P * k2;
+            if (p >= st0 && p < st1) st_nt(ln + (int64_t)p * a.pstride, scl(w[k2], a.scale));
+          }
+        }
       }
     }
   }

@@ -113,6 +113,31 @@ def test_four_step_fft_vs_oracle_and_stockham(N, C, os_):
     assert errs[4] <= 1e-6 and errs[5] <= 1e-6
 
 
+def test_four_step_split_transpose_vs_stockham():
+    """Grid edges S >= 400 (here 480 and 432; 96 on z) move the four-step
+    line passes' transpose through LDS as floats, real parts then imaginary
+    parts, so that three workgroups fit a CU.  A and A^H against the LDS
+    Stockham passes (pinned to the oracle elsewhere), which sum in another
+    order: FP32 rounding, <= 1e-6; 8-coil blocks, where the x and y passes are
+    four-step."""
+    N, C = (320, 288, 64), 8
+    _, X, smaps, _ = problem(N, C, 63)
+    # spokes over the whole x-y extent (those leaving the 96-cell z extent are skipped)
+    traj = [t.numpy() for t in synth.radial_traj(400, 160, 320)]
+    d = crandn((C, traj[0].size), 64)
+    out = []
+    for stockham in (False, True):
+        op = GridFFT(N, ncoils=C, overgrid=1.5, dwin=2.5, fft_stockham=stockham)
+        assert tuple(op.S) == (480, 432, 96)
+        op.set_traj(*traj_dev(traj))
+        op.set_smaps(to_dev(smaps))
+        out.append((op.backward(to_dev(X)).cpu().numpy(), op.forward(to_dev(d), accumulate=False).cpu().numpy()))
+        op.close()
+    ea, eh = rel(out[0][0], out[1][0]), rel(out[0][1], out[1][1])
+    print("split transpose S=(480, 432, 96): A %.2e A^H %.2e vs Stockham" % (ea, eh))
+    assert ea <= 1e-6 and eh <= 1e-6
+
+
 @pytest.mark.parametrize("C", [16, 8, 3])
 def test_gather_coils_per_lane_same_bits(C):
     """The direct gather with 1, 2, 4 or 8 coils per lane (default 4; capped
