@@ -53,6 +53,13 @@ KERNEL_NAMES = {
 }
 
 
+# the config lines beside the headline, in the order the full run takes them
+CONFIG_LINES = {"sense_cg_128cube_8coil": "small_cg",
+                "gated_llr_fista_160cube_24coil_10frame": "gated_fista",
+                "cones_gradients_192cube_16coil": "cones_gradients",
+                "l1_fista_320cube_48coil": "large_l1_fista"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -140,6 +147,8 @@ def main():
                     help="gather of the dense tiles: 0 direct, 1 LDS-staged, 2 box matrix-core")
     ap.add_argument("--small", action="store_true", help="128^3 x 8 coils, 4096 x 64 radial (os 1.5, W=4)")
     ap.add_argument("--rocprof-summary", metavar="DIR", help="summarise rocprofv3 CSV output and exit")
+    ap.add_argument("--config", choices=sorted(CONFIG_LINES),
+                    help="only this config line (its JSON object), e.g. for rocprofv3")
     a = ap.parse_args()
     if a.rocprof_summary:
         rocprof_summary(a.rocprof_summary)
@@ -158,6 +167,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if a.config:
+        print(json.dumps({a.config: globals()[CONFIG_LINES[a.config]](a, log)}))
+        return
     dev = torch.device("cuda")
     if a.small:
         N, C, NSP, NPT, OS, DW = 128, 8, 4096, 64, 1.5, 2.0
@@ -346,10 +358,7 @@ def main():
         op.close()
         del data, smaps
         torch.cuda.empty_cache()
-        res["configs"] = {"sense_cg_128cube_8coil": small_cg(a, log),
-                          "gated_llr_fista_160cube_24coil_10frame": gated_fista(a, log),
-                          "cones_gradients_192cube_16coil": cones_gradients(a, log),
-                          "l1_fista_320cube_48coil": large_l1_fista(a, log)}
+        res["configs"] = {name: globals()[fn](a, log) for name, fn in CONFIG_LINES.items()}
 
     dom_b = b[dom]
     achieved = dom_b / (st[dom] * 1e-3) / 1e9
