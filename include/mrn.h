@@ -66,7 +66,8 @@ typedef struct mrn_params {
   int32_t tile[3];       /* scatter tile edge (x,y,z): 0 (auto) or 4; other edges are MRN_EUNSUPPORTED */
   int32_t max_visits;    /* samples per scatter work item; 0 = auto */
   int32_t bin_edge;      /* sample sort bin edge in cells (1 or 2); 0 = auto */
-  int32_t xcd_run;       /* scatter work items per XCD run (1 = hardware order); 0 = auto */
+  int32_t xcd_run;       /* scatter work items per XCD run (1 = hardware order); 0 = auto;
+                            -1 = each XCD takes one contiguous range of the work list, of equal cost */
   int32_t xcd_run_gather;/* gather workgroups per XCD run; 0 = auto */
   int32_t gather_tiles;  /* dense tiles of the gather: 1 = stage the grid in LDS, 2 = box matrix-core
                           * gather (16-coil blocks; default 0 = direct gather) */
