@@ -75,7 +75,7 @@ __device__ __forceinline__ float tap_dweight(const Geom &g, int a, const float *
 // exponential.  A sample with more than NT taps on an axis (exactly on the
 // window edge) takes the loop path.
 template <int NT, int NC>
-__global__ __launch_bounds__(256) void k_traj_grad(const float2 *__restrict__ grid,
+__global__ __launch_bounds__(256, 3) void k_traj_grad(const float2 *__restrict__ grid,
                                                    const float2 *__restrict__ gin,
                                                    const float4 *__restrict__ coords,
                                                    const int32_t *__restrict__ perm, int64_t Mv,
