@@ -29,6 +29,7 @@ int op_fill(mrn_op *, float2 *, int64_t, float2, hipStream_t);
 int op_copy(mrn_op *, float2 *, const float2 *, int64_t, hipStream_t);
 int op_smaps_in(mrn_op *, float2 *, const float2 *, hipStream_t);
 int op_permute_in(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
+int op_permute_in_weighted(mrn_op *, float2 *, const float2 *, const FrameData &, hipStream_t);
 int op_permute_out(mrn_op *, float2 *, const float2 *, const FrameData &, const float2 *, hipStream_t);
 int op_set_image_plain(mrn_op *, float2 *, const float2 *, const float2 *, const PrePostArgs &, hipStream_t);
 int op_accumulate_plain(mrn_op *, const float2 *, float2 *, const float2 *, const PrePostArgs &, hipStream_t,
@@ -650,8 +651,7 @@ int mrn_kspace_to_image(mrn_op *op, int f, const void *data, void *X, int accumu
   hipStream_t st = pick(op, stream);
   FrameData &fr = op->frames[f];
   if (batched(op)) {
-    MRN_TRY(op_permute_in(op, op->kwork, (const float2 *)data, fr, st));
-    MRN_TRY(launch_weight(op, op->kwork, op->kwork, fr, st));
+    MRN_TRY(op_permute_in_weighted(op, op->kwork, (const float2 *)data, fr, st));
     MRN_TRY(launch_scatter(op, op->grid, op->kwork, fr, st));
     return adj_grid(op, (float2 *)X, !accumulate, &fr, st);
   }
@@ -1071,8 +1071,7 @@ int mrn_chop_grid_forward(mrn_op *op, int f, const void *data, void *stream) {
     return launch_scatter_plain(op, op->cgrid, (const float2 *)data, fr.M, traw(fr, 0), traw(fr, 1), traw(fr, 2),
                                 traw(fr, 3), st);
   }
-  MRN_TRY(op_permute_in(op, op->kwork, (const float2 *)data, fr, st));
-  MRN_TRY(launch_weight(op, op->kwork, op->kwork, fr, st));
+  MRN_TRY(op_permute_in_weighted(op, op->kwork, (const float2 *)data, fr, st));
   return launch_scatter(op, op->grid, op->kwork, fr, st, true);
 }
 

@@ -97,19 +97,27 @@ __global__ void k_permute_out(float2 *data, const float2 *kint, const int32_t *i
 // consecutive samples, so the caller side is coalesced, and the internal side
 // moves one whole 8*CC-byte record per lane.  Every internal record belongs
 // to exactly one valid caller sample (inv is a bijection onto [0, Mv)).
-template <int CC>
+// W: the scatter's input weighting (d * kw) * taper of k_weight, on the way
+template <int CC, bool W>
 __global__ void k_permute_in_rec(float2 *__restrict__ kint, const float2 *__restrict__ data,
-                                 const int32_t *__restrict__ inv, int64_t M, int64_t Mv, int C) {
+                                 const int32_t *__restrict__ inv, int64_t M, int64_t Mv, int C,
+                                 const float4 *__restrict__ coords, const float *__restrict__ taper) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int ch = blockIdx.y;
   if (m >= M) return;
   const int i = inv[m];
   if (i < 0) return;
+  float w = 1.f, tp = 1.f;
+  if constexpr (W) {
+    w = coords[i].w;
+    tp = taper[i];
+  }
   float2 v[CC];
 #pragma unroll
   for (int cc = 0; cc < CC; cc++) {
     const int c = ch * CC + cc;
     v[cc] = c < C ? data[(int64_t)c * M + m] : make_float2(0.f, 0.f);
+    if constexpr (W) v[cc] = make_float2((v[cc].x * w) * tp, (v[cc].y * w) * tp);
   }
   float2 *dst = kint + ((int64_t)ch * Mv + i) * CC;
   if constexpr (CC == 1) {
@@ -485,18 +493,22 @@ static bool permute_by_record(const mrn_op *op, const void *kint, const FrameDat
   return fr.M > 0 && op->nch <= 65535 && (fr.M + 255) / 256 <= INT32_MAX &&
          (op->g.CC == 1 || ((uintptr_t)kint & 15) == 0);
 }
-int op_permute_in(mrn_op *op, float2 *kint, const float2 *data, const FrameData &fr, hipStream_t st) {
+static int permute_in_impl(mrn_op *op, float2 *kint, const float2 *data, const FrameData &fr, hipStream_t st,
+                           bool weighted) {
   const int64_t total = (int64_t)op->nch * fr.Mv * op->g.CC;
   if (total == 0) return MRN_OK;
   if (permute_by_record(op, kint, fr)) {
     const dim3 grid((unsigned)((fr.M + 255) / 256), (unsigned)op->nch);
-    switch (op->g.CC) {
-#define MRN_PIN(n)                                                                                      \
-  case n:                                                                                               \
-    hipLaunchKernelGGL(k_permute_in_rec<n>, grid, dim3(256), 0, st, kint, data, fr.inv, fr.M, fr.Mv, op->C); \
-    LAUNCH_CHECK(op);                                                                                   \
+    switch (op->g.CC * 2 + (weighted ? 1 : 0)) {
+#define MRN_PIN(n, w)                                                                                  \
+  case n * 2 + w:                                                                                      \
+    hipLaunchKernelGGL((k_permute_in_rec<n, w>), grid, dim3(256), 0, st, kint, data, fr.inv, fr.M, fr.Mv, \
+                       op->C, fr.coords, fr.taper);                                                    \
+    LAUNCH_CHECK(op);                                                                                  \
     return MRN_OK;
-      MRN_PIN(1) MRN_PIN(2) MRN_PIN(4) MRN_PIN(8) MRN_PIN(16)
+#define MRN_PIN2(n) MRN_PIN(n, 0) MRN_PIN(n, 1)
+      MRN_PIN2(1) MRN_PIN2(2) MRN_PIN2(4) MRN_PIN2(8) MRN_PIN2(16)
+#undef MRN_PIN2
 #undef MRN_PIN
       default:
         break;
@@ -505,7 +517,14 @@ int op_permute_in(mrn_op *op, float2 *kint, const float2 *data, const FrameData 
   hipLaunchKernelGGL(k_permute_in, dim3(gs(total)), dim3(256), 0, st, kint, data, fr.perm, fr.M,
                      fr.Mv, op->C, op->g.CC, op->g.lgCC, total);
   LAUNCH_CHECK(op);
-  return MRN_OK;
+  return weighted ? launch_weight(op, kint, kint, fr, st) : MRN_OK;
+}
+int op_permute_in(mrn_op *op, float2 *kint, const float2 *data, const FrameData &fr, hipStream_t st) {
+  return permute_in_impl(op, kint, data, fr, st, false);
+}
+// the same with the scatter's input weighting kw * taper applied
+int op_permute_in_weighted(mrn_op *op, float2 *kint, const float2 *data, const FrameData &fr, hipStream_t st) {
+  return permute_in_impl(op, kint, data, fr, st, true);
 }
 int op_permute_out(mrn_op *op, float2 *data, const float2 *kint, const FrameData &fr,
                    const float2 *sub, hipStream_t st) {
