@@ -138,6 +138,24 @@ __device__ __forceinline__ double lerp(const float *tab, float dk, int l, float 
   return __dadd_rn(__dmul_rn((double)tab[i], __dsub_rn(1.0, (double)d)), (double)__fmul_rn(tab[i + 1], d));
 }
 
+// Geometry of sample i for the forward gridder (DCFgridFFT.cpp:171-240): its
+// kw, kernel scale, window centre dk and tap range [s, e] per axis.  False
+// when the sample has no taps (i >= M, kw = 0, scale <= 0, or off the grid).
+__device__ __forceinline__ bool fwd_taps(const DcfGeom &g, const float *__restrict__ kw,
+                                         const float *__restrict__ kx, const float *__restrict__ ky,
+                                         const float *__restrict__ kz, int64_t i, int64_t M, float &temp,
+                                         float &scale, float dk[3], int s[3], int e[3]) {
+  if (i >= M) return false;
+  float k[3];
+  scale = scale_fwd(g, kr_of(g, kx[i], ky[i], kz[i], k));
+  temp = kw[i];
+  bool live = temp != 0.0f && scale > 0.0f;  // a scale <= 0 (acc < 1, large |k|) has no taps
+  live = live && axis(g, 0, k[0], scale, dk[0], s[0], e[0]);
+  live = live && axis(g, 1, k[1], scale, dk[1], s[1], e[1]);
+  live = live && axis(g, 2, k[2], scale, dk[2], s[2], e[2]);
+  return live;
+}
+
 __device__ __forceinline__ void load_table(float *s_tab, const float *tab, int len) {
   for (int i = threadIdx.x; i < len; i += blockDim.x) s_tab[i] = tab[i];
   __syncthreads();
@@ -157,9 +175,16 @@ __global__ void k_dcf_fill(float *v, int64_t n, float value) {
 // there with FP64 LDS atomics (ds_add_f64; FP32 LDS atomics are ~20x
 // slower on gfx950) and each touched cell gets one float atomic in the grid,
 // instead of one per tap.  Larger boxes (sparse regions) take the direct
-// path.  The grid sums were already unordered (float atomics); a box only
-// sums its part in double before rounding once.
+// path, DCF_XL lanes per sample.  Launched with 256 threads per workgroup.
+// The grid sums were already unordered (float atomics); a box only sums its
+// part in double before rounding once.
 constexpr int DCF_BOX = 8192;  // FP64 LDS box cells (64 KB): 221 ms vs 241 with 4096
+// Lanes per sample on the direct path.  The L2 serves float atomics per
+// 64-byte line request: about 21 G requests/s on gfx950 when every lane of a
+// wave has its own line, while lanes that share a line share its request
+// (16 lanes on one line: 333 G atomics/s).  With one lane per x tap of a
+// row, a row's 4-5 taps cost about one request instead of one each.
+constexpr int DCF_XL = 8;
 
 // float atomics the scatters of the last mrn_dcf_calc on this device issued
 // to the grid (diagnostic: one 64-bit atomic per wavefront and launch)
@@ -180,18 +205,9 @@ __global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, 
   __shared__ int s_red[6][256 / 64];
   load_table(s_tab, tab, g.len);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = false;
   float temp = 0.0f, scale = 0.0f, dk[3] = {0.f, 0.f, 0.f};
   int s[3] = {0, 0, 0}, e[3] = {-1, -1, -1};
-  if (i < M) {
-    float k[3];
-    scale = scale_fwd(g, kr_of(g, kx[i], ky[i], kz[i], k));
-    temp = kw[i];
-    live = temp != 0.0f && scale > 0.0f;  // a scale <= 0 (acc < 1, large |k|) has no taps
-    live = live && axis(g, 0, k[0], scale, dk[0], s[0], e[0]);
-    live = live && axis(g, 1, k[1], scale, dk[1], s[1], e[1]);
-    live = live && axis(g, 2, k[2], scale, dk[2], s[2], e[2]);
-  }
+  const bool live = fwd_taps(g, kw, kx, ky, kz, i, M, temp, scale, dk, s, e);
   // bounding box of the workgroup's taps
   int r6[6];
 #pragma unroll
@@ -223,42 +239,54 @@ __global__ __launch_bounds__(256) void k_dcf_scatter_box(float *__restrict__ X, 
   }
   const int64_t vol = (int64_t)ext[0] * ext[1] * ext[2];  // 0: no live sample
   if (vol == 0) return;                                     // (uniform)
-  const bool boxed = vol <= DCF_BOX;
   unsigned long long nat = 0;  // grid atomics of this thread
-  if (boxed) {
-    for (int q = threadIdx.x; q < vol; q += blockDim.x) s_acc[q] = 0.0;
-    __syncthreads();
-  }
-  if (live) {
-    for (int lz = s[2]; lz <= e[2]; lz++) {
-      const float wtz = (float)lerp(s_tab, dk[2], lz, scale);
-      for (int ly = s[1]; ly <= e[1]; ly++) {
-        const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk[1], ly, scale));
-        float *row = X + ((int64_t)lz * g.S[1] + ly) * g.S[0];
-        double *brow = s_acc + ((lz - lo[2]) * ext[1] + (ly - lo[1])) * ext[0] - lo[0];
-        for (int lx = s[0]; lx <= e[0]; lx++) {
-          const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
-          const float v = __fmul_rn(wtx, temp);
-          if (boxed) {
-            atomicAdd(brow + lx, (double)v);
-          } else {
-            atomicAdd(row + lx, v);
+  if (vol > DCF_BOX) {  // (uniform) direct path: DCF_XL lanes per sample, one per x tap
+    const int j = (int)(threadIdx.x % DCF_XL);
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x + threadIdx.x / DCF_XL;
+    for (int r = 0; r < DCF_XL; r++) {  // the workgroup's samples, 256 / DCF_XL at a time
+      float t2 = 0.0f, sc2 = 0.0f, dk2[3] = {0.f, 0.f, 0.f};
+      int s2[3] = {0, 0, 0}, e2[3] = {-1, -1, -1};
+      if (!fwd_taps(g, kw, kx, ky, kz, base + r * (256 / DCF_XL), M, t2, sc2, dk2, s2, e2)) continue;
+      for (int lz = s2[2]; lz <= e2[2]; lz++) {
+        const float wtz = (float)lerp(s_tab, dk2[2], lz, sc2);
+        for (int ly = s2[1]; ly <= e2[1]; ly++) {
+          const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk2[1], ly, sc2));
+          float *row = X + ((int64_t)lz * g.S[1] + ly) * g.S[0];
+          for (int lx = s2[0] + j; lx <= e2[0]; lx += DCF_XL) {
+            const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk2[0], lx, sc2));
+            atomicAdd(row + lx, __fmul_rn(wtx, t2));
             nat++;
           }
         }
       }
     }
+    count_atomics(nat);
+    return;
   }
-  if (boxed) {
-    __syncthreads();
-    for (int q = threadIdx.x; q < vol; q += blockDim.x) {
-      const double v = s_acc[q];
-      if (v == 0.0) continue;
-      const int x = q % ext[0], yz = q / ext[0];
-      const int y = yz % ext[1], z = yz / ext[1];
-      atomicAdd(X + ((int64_t)(lo[2] + z) * g.S[1] + (lo[1] + y)) * g.S[0] + lo[0] + x, (float)v);
-      nat++;
+  for (int q = threadIdx.x; q < vol; q += blockDim.x) s_acc[q] = 0.0;
+  __syncthreads();
+  if (live) {
+    for (int lz = s[2]; lz <= e[2]; lz++) {
+      const float wtz = (float)lerp(s_tab, dk[2], lz, scale);
+      for (int ly = s[1]; ly <= e[1]; ly++) {
+        const float wty = (float)__dmul_rn((double)wtz, lerp(s_tab, dk[1], ly, scale));
+        double *brow = s_acc + ((lz - lo[2]) * ext[1] + (ly - lo[1])) * ext[0] - lo[0];
+        for (int lx = s[0]; lx <= e[0]; lx++) {
+          const float wtx = (float)__dmul_rn((double)wty, lerp(s_tab, dk[0], lx, scale));
+          atomicAdd(brow + lx, (double)__fmul_rn(wtx, temp));
+        }
+      }
     }
+  }
+  __syncthreads();
+  // one float atomic per touched cell; consecutive lanes take consecutive x
+  for (int q = threadIdx.x; q < vol; q += blockDim.x) {
+    const double v = s_acc[q];
+    if (v == 0.0) continue;
+    const int x = q % ext[0], yz = q / ext[0];
+    const int y = yz % ext[1], z = yz / ext[1];
+    atomicAdd(X + ((int64_t)(lo[2] + z) * g.S[1] + (lo[1] + y)) * g.S[0] + lo[0] + x, (float)v);
+    nat++;
   }
   count_atomics(nat);
 }

@@ -47,10 +47,12 @@ def test_dcf_matches_oracle(iters, tol):
 @pytest.mark.parametrize("N,acc,kernel,dwin,os_", [((24, 16, 12), 1.0, "poly", 2.1, 2.1),
                                                   (32, 2.0, "poly", 2.1, 2.1),
                                                   (32, 1.0, "poly2", 1.5, 1.5),
-                                                  (20, 1.5, "poly", 3.0, 2.5)])
+                                                  (20, 1.5, "poly", 3.0, 2.5),
+                                                  (16, 3.0, "poly", 4.0, 2.1)])
 def test_dcf_settings_match_oracle(N, acc, kernel, dwin, os_):
     """Non-cubic sizes (the x extent follows Nz), kernel widening with |k|,
-    the second polynomial, other widths and oversampling."""
+    the second polynomial, other widths and oversampling; the last case has
+    rows of up to 10 taps (more than the scatter's 8 lanes per sample)."""
     n = N if isinstance(N, tuple) else (N, N, N)
     kx, ky, kz = radial(200, 16, min(n))
     kw = dcf_calc(dev(kx), dev(ky), dev(kz), N=n, iters=8, acc=acc, kernel=kernel, dwin=dwin, overgrid=os_)
