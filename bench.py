@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 METRIC = ("batched-graph vs plain SENSE-CG iter/s, 256³×32ch; gridding GB/s; "
           "rel err≤1e-4")
 HBM_PEAK_GBPS = 8000.0
+# The gather's other bound: every tap read passes the texture path (address
+# unit and vector L1), one 64-byte access per clock per CU; 256 CUs at the
+# 2.4 GHz peak engine clock
+CU_COUNT = 256
+ENGINE_GHZ = 2.4
+TEX_BYTES_PER_CLK = 64.0
 # The CPU leg's FFT, in the order of preference of the reference's build:
 # FFTW3f (absent from the image), MKL's FFTW3 interface (libmkl_rt, its
 # sequential layer per line under the oracle's OpenMP loop over lines), else
@@ -124,6 +130,14 @@ def roofline_of(st, b):
     return {"kernel": KERNEL_NAMES[dom], "stage": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS, "traffic": None, "algorithmic_bytes": b[dom],
             "avg_ms": st[dom]}
+
+
+def texture_path(taps_per_s):
+    """The gather's tap reads (8 bytes per tap and coil, every tap of every
+    sample) against the texture path's 64 bytes per clock per CU."""
+    per_clk = taps_per_s * 8.0 / (CU_COUNT * ENGINE_GHZ * 1e9)
+    return {"achieved": per_clk, "peak": TEX_BYTES_PER_CLK, "unit": "B/clk/CU", "frac": per_clk / TEX_BYTES_PER_CLK,
+            "bytes_model": "8 * taps * coils per sample, %d CUs at %.1f GHz" % (CU_COUNT, ENGINE_GHZ)}
 
 
 def main():
@@ -377,6 +391,8 @@ def main():
                      "algorithmic_bytes": dom_b, "avg_ms": st[dom]},
         "cpu_baseline": cpu,
     }
+    if dom == "gather":
+        out["roofline"]["texture_path"] = texture_path(res["gridding"]["gather_taps_per_s"])
     out.update(res)
     print(json.dumps(out))
 

@@ -1,7 +1,8 @@
 """bench.py's contract on the host: one GPU only (--gpus above 1 exits with a
 message and no JSON line), every configuration line it names exists, and the
 HBM traffic it reports for the headline roofline comes from the committed
-rocprofv3 counter summaries with their calibration factors."""
+rocprofv3 counter summaries with their calibration factors, and the gather's
+texture-path rate is bytes per clock per CU."""
 import os
 import subprocess
 import sys
@@ -25,6 +26,14 @@ def test_config_lines_name_functions():
     assert len(bench.CONFIG_LINES) == 4
     for name, fn in bench.CONFIG_LINES.items():
         assert callable(getattr(bench, fn, None)), name
+
+
+def test_texture_path_rate():
+    # 5.12 M samples x 32 coils x 125 taps in 5.585 ms: 29.3 TB/s of tap reads,
+    # 47.7 bytes per clock on each of 256 CUs at 2.4 GHz
+    t = bench.texture_path(5.12e6 * 32 * 125 / 5.585e-3)
+    assert abs(t["achieved"] - 47.74) < 0.01 and t["peak"] == 64.0
+    assert abs(t["frac"] - t["achieved"] / 64.0) < 1e-12
 
 
 def test_profiled_traffic_from_committed_summaries():
