@@ -52,9 +52,10 @@ namespace {
 
 constexpr int DCF_MOD = 600;          // table points per cell (DCFgridFFT.cpp:101-103)
 constexpr int DCF_MAX_TABLE = 12288;  // table floats held in LDS (48 KB)
-// sort brick edge in cells (5.12 M radial samples, 553^3: 4^3 bricks 221 ms
-// for 20 iterations, 2^3 249 ms, 8^3 with a 4096-cell box 262 ms)
-constexpr int DCF_BRICK = 4;
+// sort brick edge in cells (5.12 M radial samples, 553^3, 20 iterations:
+// 8^3 bricks 104.6 ms, 4^3 106.7, 2^3 119, 16^3 155; 8^3 with a 4096-cell
+// box 115)
+constexpr int DCF_BRICK = 8;
 
 struct DcfGeom {
   int S[3];
@@ -166,11 +167,10 @@ __global__ void k_dcf_fill(float *v, int64_t n, float value) {
     v[i] = value;
 }
 
-// DCFgridFFT::forward (DCFgridFFT.cpp:171-289), one thread per sample: kernel
-// weights with the reference's float/double promotions, grid += w * kw.
-// The samples run in 4^3-brick order, so
-// in the dense centre of k-space a workgroup's 256 samples reach a box of a
-// few thousand cells.  When the box (the bounding box of the taps of the
+// DCFgridFFT::forward (DCFgridFFT.cpp:171-289): kernel weights with the
+// reference's float/double promotions, grid += w * kw.  The samples run in
+// DCF_BRICK^3-brick order, so in the dense centre of k-space a workgroup's
+// 256 samples reach a box of a few thousand cells.  When the box (the bounding box of the taps of the
 // workgroup's samples) holds at most DCF_BOX cells, the taps are summed
 // there with FP64 LDS atomics (ds_add_f64; FP32 LDS atomics are ~20x
 // slower on gfx950) and each touched cell gets one float atomic in the grid,
@@ -178,7 +178,10 @@ __global__ void k_dcf_fill(float *v, int64_t n, float value) {
 // path, DCF_XL lanes per sample.  Launched with 256 threads per workgroup.
 // The grid sums were already unordered (float atomics); a box only sums its
 // part in double before rounding once.
-constexpr int DCF_BOX = 8192;  // FP64 LDS box cells (64 KB): 221 ms vs 241 with 4096
+// FP64 LDS box cells (64 KB).  20 iterations at 5.12 M radial samples, 553^3,
+// 4^3 bricks: 105 ms; 4096 cells 113, 2048 120, no box 153 (the dense centre
+// contends)
+constexpr int DCF_BOX = 8192;
 // Lanes per sample on the direct path.  The L2 serves float atomics per
 // 64-byte line request: about 21 G requests/s on gfx950 when every lane of a
 // wave has its own line, while lanes that share a line share its request
