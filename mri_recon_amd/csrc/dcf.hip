@@ -2,11 +2,11 @@
 // (recon_lib.cpp:854-917) over DCFgridFFT's real-valued polynomial-kernel
 // gridder (DCFgridFFT.cpp:96-440).
 //
-// The samples are first sorted by 8x8x8-cell brick of the grid (one device
-// radix sort): a wavefront's samples then touch neighbouring cells, and the
-// scatter's atomics and the gather's reads hit lines already in L2.  The
-// iterations run on the sorted copy; the weights are put back in the
-// caller's order at the end.
+// The samples are first sorted by 8x8x8-cell brick of the grid and 4x4x4
+// sub-brick (one device radix sort): a wavefront's samples then touch
+// neighbouring cells, and the scatter's atomics and the gather's reads hit
+// lines already in L2.  The iterations run on the sorted copy; the weights
+// are put back in the caller's order at the end.
 //
 // The trajectory stays resident; one call runs every iteration on the stream:
 //   kw = 1; repeat iters: X = 0; X += A^T kw (scatter); kw2 = A X (gather);
@@ -52,9 +52,9 @@ namespace {
 
 constexpr int DCF_MOD = 600;          // table points per cell (DCFgridFFT.cpp:101-103)
 constexpr int DCF_MAX_TABLE = 12288;  // table floats held in LDS (48 KB)
-// sort brick edge in cells (5.12 M radial samples, 553^3, 20 iterations:
-// 8^3 bricks 104.6 ms, 4^3 106.7, 2^3 119, 16^3 155; 8^3 with a 4096-cell
-// box 115)
+// sort brick edge in cells (5.12 M radial samples, 553^3, 20 iterations,
+// plain brick order: 8^3 bricks 104.6 ms, 4^3 106.7, 2^3 119, 16^3 155; 8^3
+// with a 4096-cell box 115; 8^3 with 4^3 sub-bricks 100.4 against 102.1)
 constexpr int DCF_BRICK = 8;
 
 struct DcfGeom {
@@ -359,7 +359,10 @@ __global__ void k_dcf_finish(float *__restrict__ kw, const float *__restrict__ k
   kw[i] = fabsf((float)__dmul_rn((double)kw[i], __dmul_rn(__dmul_rn(s, s), s)));
 }
 
-// sort key: the DCF_BRICK^3-cell brick of the sample's nearest grid cell
+// sort key: the DCF_BRICK^3-cell brick of the sample's nearest grid cell,
+// then its (DCF_BRICK / 2)^3 sub-brick: the scatter's workgroups take whole
+// bricks (their LDS boxes), the gather's wavefronts sub-bricks (their loads
+// stay closer together)
 __global__ void k_dcf_keys(const float *__restrict__ kx, const float *__restrict__ ky,
                            const float *__restrict__ kz, int64_t M, DcfGeom g, uint32_t *__restrict__ keys,
                            uint32_t *__restrict__ idx) {
@@ -367,14 +370,15 @@ __global__ void k_dcf_keys(const float *__restrict__ kx, const float *__restrict
   if (i >= M) return;
   float k[3];
   kr_of(g, kx[i], ky[i], kz[i], k);
-  uint32_t key = 0;
+  uint32_t key = 0, sub = 0;
   for (int a = 2; a >= 0; a--) {
     const float dk = __fadd_rn(__fmul_rn(k[a], g.gx), g.c[a]);
     int c = (int)floorf(dk);
     c = c < 0 ? 0 : (c > g.S[a] - 1 ? g.S[a] - 1 : c);
     key = key * (uint32_t)((g.S[a] + DCF_BRICK - 1) / DCF_BRICK) + (uint32_t)(c / DCF_BRICK);
+    sub = sub * 2u + (uint32_t)((c % DCF_BRICK) / (DCF_BRICK / 2));
   }
-  keys[i] = key;
+  keys[i] = key * 8u + sub;
   idx[i] = (uint32_t)i;
 }
 
@@ -507,7 +511,7 @@ int mrn_dcf_calc(const mrn_dcf_params *p, int64_t M, const float *kx, const floa
     DCF_HIP(hipMalloc(&srt, sizeof(float) * 4 * M));
     hipLaunchKernelGGL(k_dcf_keys, gs, dim3(256), 0, st, kx, ky, kz, M, g, keys, perm);
     DCF_HIP(hipGetLastError());
-    const uint64_t nkeys = (uint64_t)((g.S[0] + DCF_BRICK - 1) / DCF_BRICK) * ((g.S[1] + DCF_BRICK - 1) / DCF_BRICK) *
+    const uint64_t nkeys = 8ull * (uint64_t)((g.S[0] + DCF_BRICK - 1) / DCF_BRICK) * ((g.S[1] + DCF_BRICK - 1) / DCF_BRICK) *
                            ((g.S[2] + DCF_BRICK - 1) / DCF_BRICK);
     int bits = 1;
     while (bits < 32 && (1ull << bits) < nkeys) bits++;
