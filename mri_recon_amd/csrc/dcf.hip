@@ -56,6 +56,7 @@ constexpr int DCF_MAX_TABLE = 12288;  // table floats held in LDS (48 KB)
 // plain brick order: 8^3 bricks 104.6 ms, 4^3 106.7, 2^3 119, 16^3 155; 8^3
 // with a 4096-cell box 115; 8^3 with 4^3 sub-bricks 100.4 against 102.1)
 constexpr int DCF_BRICK = 8;
+static_assert(DCF_BRICK >= 2 && DCF_BRICK % 2 == 0, "the sort key splits each brick into 8 sub-bricks");
 
 struct DcfGeom {
   int S[3];
