@@ -1085,15 +1085,19 @@ __global__ void k_zero_tiles(float2 *grid, const int32_t *__restrict__ tiles, Ge
 // per cell.  Sums are FP32; only the order of the additions differs from a
 // per-sample loop.  Each tile is written once with plain stores (atomics
 // for a tile split over several work items).
+// PAIR (8-coil blocks): a work item takes coil blocks 2 y and 2 y + 1 as its
+// two 16-real blocks, so step A's weights serve both (the second is skipped
+// when nch is odd and y is the last pair).
 #define SM_BATCH 32
-template <int CCT, int KM>
+template <int CCT, int KM, bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     float2 *__restrict__ grid, const float2 *__restrict__ kin, const float4 *__restrict__ coords,
     const int64_t *__restrict__ vofs, const int32_t *__restrict__ vlist, const int4 *__restrict__ subs, int nsub,
     Geom g, Tiles t, const float *__restrict__ tables, int64_t chunk_grid, int64_t chunk_k, int accum,
-    const FrameDev *__restrict__ fdt, const int32_t *__restrict__ xs) {
+    const FrameDev *__restrict__ fdt, const int32_t *__restrict__ xs, int nch) {
   static_assert(CCT == 8 || CCT == 16, "matrix-core scatter needs 8 or 16 coils per cell");
-  constexpr int NB = CCT / 8;            // 16-real blocks of a cell's coil values
+  static_assert(!PAIR || CCT == 8, "coil blocks are paired only at 8 coils");
+  constexpr int NB = PAIR ? 2 : CCT / 8;  // 16-real blocks of a cell's coil values (or the pair's blocks)
   constexpr int NG = SM_BATCH / 4;       // sample groups of a batch
   __shared__ __align__(16) float s_w[4][SM_BATCH * 12];  // [sample][axis][offset] weights
   extern __shared__ float tab_s[];         // kernel tables (g.ntab floats)
@@ -1121,7 +1125,8 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   const int tile = sp.x;
   const int tix = tile % t.nt[0], tiy = (tile / t.nt[0]) % t.nt[1], tiz = tile / (t.nt[0] * t.nt[1]);
   const int t0[3] = {tix * 4, tiy * 4, tiz * 4};
-  const int ch = blockIdx.y;
+  const int ch = PAIR ? 2 * (int)blockIdx.y : (int)blockIdx.y;
+  const bool have1 = !PAIR || ch + 1 < nch;  // (uniform) the pair's second block exists
   const float *kch = reinterpret_cast<const float *>(kin + ch * chunk_k);
   const int64_t v0 = vofs[tile];
   const int32_t *vl = vlist + v0;
@@ -1137,14 +1142,20 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
   // 0.  Byte offsets are 32-bit (the launcher checks that a coil block's
   // k-space stays below 4 GiB).
   const char *kb = reinterpret_cast<const char *>(kch);
+  const char *kb1 = kb + chunk_k * (int64_t)sizeof(float2);  // PAIR: the second block's k-space
   const char *cb = reinterpret_cast<const char *>(coords);
   auto load_data = [&](int idx, float (&d)[NG][NB]) {
 #pragma unroll
     for (int q = 0; q < NG; q++) {
       const uint32_t j = (uint32_t)__shfl(idx, 2 * (4 * q + kq));
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++)
-        d[q][nb] = *reinterpret_cast<const float *>(kb + (j * (8u * CCT) + 4u * (16 * nb + ci)));
+      for (int nb = 0; nb < NB; nb++) {
+        if (PAIR)
+          d[q][nb] = (nb == 0 || have1) ? *reinterpret_cast<const float *>((nb ? kb1 : kb) + (j * 64u + 4u * ci))
+                                        : 0.0f;
+        else
+          d[q][nb] = *reinterpret_cast<const float *>(kb + (j * (8u * CCT) + 4u * (16 * nb + ci)));
+      }
     }
   };
   mrn_v4f acc[4][NB];
@@ -1212,7 +1223,8 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
           const float a = wz[p] * wyx;
 #pragma unroll
           for (int nb = 0; nb < NB; nb++)
-            acc[p][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, cur[q][nb], acc[p][nb], 0, 0, 0);
+            if (nb == 0 || have1)
+              acc[p][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, cur[q][nb], acc[p][nb], 0, 0, 0);
         }
       }
     }
@@ -1233,7 +1245,8 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
     batch(base + SM_BATCH, dB, dA);
   }
   // accumulator block (p, nb), register r of lane l: cell 16 p + 4 (l / 16) + r,
-  // real 16 nb + l % 16 of the cell's CCT coils
+  // real 16 nb + l % 16 of the cell's CCT coils (PAIR: real l % 16 of block
+  // ch + nb)
 #pragma unroll
   for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -1244,13 +1257,15 @@ __global__ __launch_bounds__(256, 4) void k_scatter_mm(
                                              (((int64_t)cz * g.S[1] + cy) * g.S[0] + cx) * CCT) + ci;
 #pragma unroll
       for (int nb = 0; nb < NB; nb++) {
+        if (nb > 0 && !have1) continue;
         const float v = acc[p][nb][r];
+        float *d = PAIR ? dst + nb * 2 * chunk_grid : dst + 16 * nb;
         if (sp.w) {
-          if (v != 0.f) atomicAdd(dst + 16 * nb, v);
+          if (v != 0.f) atomicAdd(d, v);
         } else if (accum) {  // the tile's only work item: a plain read-modify-write
-          dst[16 * nb] += v;
+          *d += v;
         } else {
-          dst[16 * nb] = v;
+          *d = v;
         }
       }
     }
@@ -1764,13 +1779,19 @@ int launch_scatter_tiles(mrn_op *op, float2 *grid, const float2 *kin, const mrn:
   // with 32-bit byte offsets
   const bool off32 = Mv * g.CC * 8 < (1ll << 32) && Mv * 16 < (1ll << 32);
   if ((g.CC == 16 || g.CC == 8) && !op->scatter_valu && off32) {
-#define MRN_MM(n, km)                                                                                  \
-  hipLaunchKernelGGL((k_scatter_mm<n, km>), g3, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,      \
-                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt, fr.xsplit)
-    if (g.CC == 16 && !exact) MRN_MM(16, 0);
-    else if (g.CC == 16) MRN_MM(16, 1);
-    else if (!exact) MRN_MM(8, 0);
-    else MRN_MM(8, 1);
+    // 8-coil blocks go in pairs (one work item, both blocks) when there are two or more
+    const bool pair = g.CC == 8 && op->nch >= 2;
+    dim3 gm = g3;
+    if (pair) gm.y = (unsigned)((op->nch + 1) / 2);
+#define MRN_MM(n, km, pr)                                                                                \
+  hipLaunchKernelGGL((k_scatter_mm<n, km, pr>), gm, dim3(256), sh, st, grid, kin, fr.coords, fr.vofs,   \
+                     fr.vlist, fr.subs, fr.nsub, g, t, op->tables_d, cg, ck, af, fdt, fr.xsplit, op->nch)
+    if (g.CC == 16 && !exact) MRN_MM(16, 0, false);
+    else if (g.CC == 16) MRN_MM(16, 1, false);
+    else if (pair && !exact) MRN_MM(8, 0, true);
+    else if (pair) MRN_MM(8, 1, true);
+    else if (!exact) MRN_MM(8, 0, false);
+    else MRN_MM(8, 1, false);
 #undef MRN_MM
   } else {
     switch (g.CC) {
