@@ -53,46 +53,62 @@ __device__ __forceinline__ int wmod(int i, int n) {
 
 // In-place one-level transform (INV = 0 analysis, 1 synthesis) of the lines
 // along `axis` of the sub-block [0,s0) x [0,s1) x [0,s2) of frame blockIdx.y.
-// The shift of the axis is read from sh[axis] (device memory).
+// The shift of the axis is read from sh[axis] (device memory; 0..3).  No
+// integer division per element: the workgroup's 16 line bases are computed
+// once, the x-axis walk steps by a precomputed quotient and remainder, and
+// the periodic index wraps with one add or subtract for lines of 4 or more
+// points (the indices reach [-3, pts + 2]).  Launched with 256 threads.
 template <int INV>
 __global__ __launch_bounds__(256) void k_wave_lines(float2 *X, int N0, int N1, int s0, int s1, int s2,
                                                     int axis, const int *__restrict__ sh, D4 f,
                                                     int64_t frame_stride) {
   extern __shared__ float2 wbuf[];  // [WL_LINES][pts]
+  __shared__ int64_t s_base[WL_LINES];
   const int sz[3] = {s0, s1, s2};
   const int pts = sz[axis], h = pts >> 1;
   const int b = axis == 0 ? 1 : 0, c = axis == 2 ? 1 : 2;
   const int64_t st[3] = {1, N0, (int64_t)N0 * N1};
+  const int64_t sa = st[axis];
   const int nlines = sz[b] * sz[c];
   const int l0 = blockIdx.x * WL_LINES;
   float2 *Xf = X + (int64_t)blockIdx.y * frame_stride;
   const int shift = sh[axis];
   const int tot = WL_LINES * pts;
-  // element e -> (line, point): an x line is contiguous; along y and z the 16
-  // lines of a point are neighbouring x (or y) cells
-  auto where = [&](int e, int &ln, int &p) {
+  if (threadIdx.x < WL_LINES) {
+    const int L = l0 + (int)threadIdx.x, u = L % sz[b], v = L / sz[b];
+    s_base[threadIdx.x] = (int64_t)u * st[b] + (int64_t)v * st[c];
+  }
+  // element e = threadIdx.x + k * blockDim.x as (line, point): an x line is
+  // contiguous (ln = e / pts, p = e % pts); along y and z the 16 lines of a
+  // point are neighbouring x (or y) cells (p = e / 16, ln = e % 16)
+  const int dq = (int)blockDim.x / pts, dr = (int)blockDim.x - dq * pts;
+  const int q0 = (int)threadIdx.x / pts, r0 = (int)threadIdx.x - q0 * pts;
+  auto walk = [&](auto &&fn) {
     if (axis == 0) {
-      ln = e / pts;
-      p = e - ln * pts;
+      int ln = q0, p = r0;
+      for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        fn(ln, p);
+        p += dr;
+        ln += dq;
+        if (p >= pts) {
+          p -= pts;
+          ln++;
+        }
+      }
     } else {
-      p = e / WL_LINES;
-      ln = e - p * WL_LINES;
+      const int ln = (int)threadIdx.x % WL_LINES;
+      for (int p = (int)threadIdx.x / WL_LINES; p < pts; p += (int)blockDim.x / WL_LINES) fn(ln, p);
     }
   };
-  auto addr = [&](int ln, int p) -> int64_t {
-    const int L = l0 + ln, u = L % sz[b], v = L / sz[b];
-    return (int64_t)u * st[b] + (int64_t)v * st[c] + (int64_t)p * st[axis];
-  };
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    int ln, p;
-    where(e, ln, p);
-    if (l0 + ln < nlines) wbuf[ln * pts + p] = Xf[addr(ln, p)];
-  }
+  const bool near = pts >= 4;
+  auto wrap = [&](int i) { return near ? (i < 0 ? i + pts : (i >= pts ? i - pts : i)) : wmod(i, pts); };
   __syncthreads();
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    int ln, p;
-    where(e, ln, p);
-    if (l0 + ln >= nlines) continue;
+  walk([&](int ln, int p) {
+    if (l0 + ln < nlines) wbuf[ln * pts + p] = Xf[s_base[ln] + p * sa];
+  });
+  __syncthreads();
+  walk([&](int ln, int p) {
+    if (l0 + ln >= nlines) return;
     const float2 *in = wbuf + ln * pts;
     float2 o = make_float2(0.f, 0.f);
     if (!INV) {  // wave1D: out[pos] = sum lpf * in, out[pos + h] = sum hpf * in
@@ -100,7 +116,7 @@ __global__ __launch_bounds__(256) void k_wave_lines(float2 *X, int N0, int N1, i
       const int pos = lo ? p : p - h;
 #pragma unroll
       for (int fp = 0; fp < 4; fp++) {
-        const float2 v = in[wmod(2 * pos - fp + shift, pts)];
+        const float2 v = in[wrap(2 * pos - fp + shift)];
         const float w = lo ? f.lpf[fp] : f.hpf[fp];
         o.x += w * v.x;
         o.y += w * v.y;
@@ -110,7 +126,7 @@ __global__ __launch_bounds__(256) void k_wave_lines(float2 *X, int N0, int N1, i
       for (int fp = 0; fp < 4; fp++) {
         const int t = p - fp + 3 - shift;
         if (t & 1) continue;
-        const int pos = wmod(t, pts) >> 1;
+        const int pos = wrap(t) >> 1;
         const float2 a = in[pos], d = in[pos + h];
         o.x += a.x * f.slpf[fp];
         o.y += a.y * f.slpf[fp];
@@ -118,8 +134,8 @@ __global__ __launch_bounds__(256) void k_wave_lines(float2 *X, int N0, int N1, i
         o.y += d.y * f.shpf[fp];
       }
     }
-    Xf[addr(ln, p)] = o;
-  }
+    Xf[s_base[ln] + p * sa] = o;
+  });
 }
 
 // ------------------------------------------------------------ threshold
@@ -332,7 +348,8 @@ static int wave_axis(mrn_op *op, float2 *X, int frames, const int s[3], int axis
   const int nlines = s[b] * s[c];
   dim3 g3((unsigned)((nlines + WL_LINES - 1) / WL_LINES), (unsigned)frames);
   const size_t shm = sizeof(float2) * WL_LINES * (size_t)pts;
-  if (shm > 160 * 1024) return fail(op, MRN_EUNSUPPORTED, "wavelet line too long");
+  if (shm + sizeof(int64_t) * WL_LINES > 160 * 1024)  // + the kernel's line bases
+    return fail(op, MRN_EUNSUPPORTED, "wavelet line too long");
   if (inv)
     hipLaunchKernelGGL(k_wave_lines<1>, g3, dim3(256), shm, st, X, N[0], N[1], s[0], s[1], s[2], axis, sh, f,
                        op->nvox);
