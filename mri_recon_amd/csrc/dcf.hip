@@ -171,14 +171,14 @@ __global__ void k_dcf_fill(float *v, int64_t n, float value) {
 // DCFgridFFT::forward (DCFgridFFT.cpp:171-289): kernel weights with the
 // reference's float/double promotions, grid += w * kw.  The samples run in
 // DCF_BRICK^3-brick order, so in the dense centre of k-space a workgroup's
-// 256 samples reach a box of a few thousand cells.  When the box (the bounding box of the taps of the
-// workgroup's samples) holds at most DCF_BOX cells, the taps are summed
-// there with FP64 LDS atomics (ds_add_f64; FP32 LDS atomics are ~20x
-// slower on gfx950) and each touched cell gets one float atomic in the grid,
-// instead of one per tap.  Larger boxes (sparse regions) take the direct
-// path, DCF_XL lanes per sample.  Launched with 256 threads per workgroup.
-// The grid sums were already unordered (float atomics); a box only sums its
-// part in double before rounding once.
+// 256 samples reach a box of a few thousand cells.  When the box (the
+// bounding box of the taps of the workgroup's samples) holds at most DCF_BOX
+// cells, the taps are summed there with FP64 LDS atomics (ds_add_f64; FP32
+// LDS atomics are ~20x slower on gfx950) and each touched cell gets one
+// float atomic in the grid, instead of one per tap.  Larger boxes (sparse
+// regions) take the direct path, DCF_XL lanes per sample.  Launched with 256
+// threads per workgroup.  The grid sums were already unordered (float
+// atomics); a box only sums its part in double before rounding once.
 // FP64 LDS box cells (64 KB).  20 iterations at 5.12 M radial samples, 553^3,
 // 4^3 bricks: 105 ms; 4096 cells 113, 2048 120, no box 153 (the dense centre
 // contends)
