@@ -158,7 +158,8 @@ __global__ void k_widen(const int32_t *__restrict__ v, int n, int64_t *out) {
 // order, each row one contiguous run of internal indices) at
 // vlist[vofs[t] ...], so that the scatter reads visit v of a tile with one
 // load.  One wavefront per tile, one lane per bin row: the lanes' run
-// lengths are loaded together and scanned, then each lane writes its run.
+// lengths are loaded together and scanned, then the wavefront writes the
+// runs one after the other, 64 consecutive entries per store.
 __global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, const int64_t *__restrict__ vofs,
                               int32_t *vlist) {
   const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -185,18 +186,13 @@ __global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, co
       const int y = __shfl_up(x, off);
       if (lane >= off) x += y;
     }
-    // Visit p of this group of rows belongs to the row r whose inclusive
-    // prefix is the first above p (binary search over the lanes' prefixes),
-    // so consecutive lanes write consecutive entries.
+    // row k's run (start sk, length nk, at offset ek of the group), written
+    // by the whole wavefront: lane q takes entries q, q + 64, ...
     const int tot = __shfl(x, 63);
-    for (int p0 = 0; p0 < tot; p0 += 64) {
-      const int p = p0 + lane;
-      int r = 0;
-#pragma unroll
-      for (int step = 32; step; step >>= 1)
-        if (__shfl(x, r + step - 1) <= p) r += step;
-      const int ex = __shfl(x, r) - __shfl(n, r), base = __shfl(s0, r);
-      if (p < tot) vlist[o + p] = base + (p - ex);
+    const int nr = min(64, nrows - r0);
+    for (int k = 0; k < nr; k++) {
+      const int nk = __shfl(n, k), sk = __shfl(s0, k), ek = __shfl(x, k) - nk;
+      for (int q = lane; q < nk; q += 64) vlist[o + ek + q] = sk + q;
     }
     o += tot;
   }
