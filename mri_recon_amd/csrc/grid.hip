@@ -157,22 +157,29 @@ __global__ void k_widen(const int32_t *__restrict__ v, int n, int64_t *out) {
 // Visit list: tile t's visits (the samples of its bin rows, rows in (bz, by)
 // order, each row one contiguous run of internal indices) at
 // vlist[vofs[t] ...], so that the scatter reads visit v of a tile with one
-// load.  One wavefront per tile, one lane per bin row: the lanes' run
-// lengths are loaded together and scanned, then the wavefront writes the
-// runs one after the other, 64 consecutive entries per store.
+// load.  One wavefront per scatter work item (a tile's visits [sp.y, sp.z),
+// at most maxv of them, so that a dense tile's list is written by several
+// wavefronts), one lane per bin row: the lanes' run lengths are loaded
+// together and scanned, then the wavefront writes the runs' parts inside its
+// range one after the other, 64 consecutive entries per store.
 __global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, const int64_t *__restrict__ vofs,
-                              int32_t *vlist) {
-  const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+                              const int4 *__restrict__ subs, int nsub, int32_t *vlist) {
+  const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (tile >= t.ntiles) return;
+  if (item >= nsub) return;
+  const int4 sp = subs[item];
+  const int tile = sp.x;
+  int64_t o = vofs[tile];
+  const int vb = max(sp.y, 0), ve = min(sp.z, (int)(vofs[tile + 1] - o));
+  if (ve <= vb) return;  // (uniform) nothing to write
   const int tx = tile % t.nt[0], ty = (tile / t.nt[0]) % t.nt[1], tz = tile / (t.nt[0] * t.nt[1]);
   int bx0, bx1, by0, by1, bz0, bz1;
   tile_bins(t, 0, tx, bx0, bx1);
   tile_bins(t, 1, ty, by0, by1);
   tile_bins(t, 2, tz, bz0, bz1);
   const int nry = by1 - by0 + 1, nrows = nry * (bz1 - bz0 + 1);
-  int64_t o = vofs[tile];
-  for (int r0 = 0; r0 < nrows; r0 += 64) {
+  int g0 = 0;  // the tile's visits before this group of rows
+  for (int r0 = 0; r0 < nrows && g0 < ve; r0 += 64) {
     const int r = r0 + lane;
     int s0 = 0, n = 0;
     if (r < nrows) {
@@ -186,15 +193,16 @@ __global__ void k_fill_visits(const int32_t *__restrict__ bin_start, Tiles t, co
       const int y = __shfl_up(x, off);
       if (lane >= off) x += y;
     }
-    // row k's run (start sk, length nk, at offset ek of the group), written
-    // by the whole wavefront: lane q takes entries q, q + 64, ...
-    const int tot = __shfl(x, 63);
+    // row k's run: start sk, length nk, tile visits [ek, ek + nk); the part
+    // inside [vb, ve) is written by the whole wavefront, lane q taking
+    // entries q, q + 64, ... of it
     const int nr = min(64, nrows - r0);
     for (int k = 0; k < nr; k++) {
-      const int nk = __shfl(n, k), sk = __shfl(s0, k), ek = __shfl(x, k) - nk;
-      for (int q = lane; q < nk; q += 64) vlist[o + ek + q] = sk + q;
+      const int nk = __shfl(n, k), sk = __shfl(s0, k), ek = g0 + __shfl(x, k) - nk;
+      const int lo = max(ek, vb), hi = min(ek + nk, ve);
+      for (int q = lo + lane; q < hi; q += 64) vlist[o + q] = sk + (q - ek);
     }
-    o += tot;
+    g0 += __shfl(x, 63);
   }
 }
 
@@ -2090,7 +2098,9 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipLaunchKernelGGL(k_tile_nsub, dim3(nblk(nt, 256)), dim3(256), 0, st, fr.bin_start, t, nsub, visits,
                      fft_x_pruned(op) ? fr.xrng : nullptr, g.S[1], g.S[2]);
   MRN_BH(hipGetLastError());
-  // visit list (offsets in 64 bits: about 8 visits per sample)
+  // visit list (offsets in 64 bits: about 8 visits per sample); it is
+  // filled once the work items exist
+  int64_t nvisits = 0;
   {
     MRN_BH(grow(fr.vofs, cp.vofs, (int64_t)nt + 1));
     MRN_BH(grow(b.vw, b.c_vw, (int64_t)nt + 1));
@@ -2103,10 +2113,7 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
     MRN_BH(hipMemcpyAsync(&nv, fr.vofs + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     MRN_BH(hipStreamSynchronize(st));
     MRN_BH(grow(fr.vlist, cp.vlist, nv));
-    if (nv > 0) {
-      hipLaunchKernelGGL(k_fill_visits, dim3(nblk(nt, 4)), dim3(256), 0, st, fr.bin_start, t, fr.vofs, fr.vlist);
-      MRN_BH(hipGetLastError());
-    }
+    nvisits = nv;
   }
   MRN_BH(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, nsub, off, nt + 1, st));
   int32_t nsub_total = 0;
@@ -2117,6 +2124,11 @@ int build_bins(mrn_op *op, mrn::FrameData &fr, int64_t M, const float *kx, const
   hipLaunchKernelGGL(k_fill_subs, dim3(nblk(nt, 256)), dim3(256), 0, st, nsub, off, visits, t,
                      fr.subs, hflag);
   MRN_BH(hipGetLastError());
+  if (nvisits > 0 && nsub_total > 0) {  // the visit list, one wavefront per work item
+    hipLaunchKernelGGL(k_fill_visits, dim3(nblk(nsub_total, 4)), dim3(256), 0, st, fr.bin_start, t, fr.vofs,
+                       fr.subs, nsub_total, fr.vlist);
+    MRN_BH(hipGetLastError());
+  }
   // the work list cut into MRN_XCDS ranges of equal cost (xcd_run < 0)
   {
     const int64_t n = nsub_total;
