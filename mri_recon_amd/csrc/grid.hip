@@ -44,10 +44,10 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
 __global__ void k_traj_keys(const float *__restrict__ kx, const float *__restrict__ ky,
                             const float *__restrict__ kz, const float *__restrict__ kw, int64_t M,
                             Geom g, Tiles t, uint32_t *keys, int32_t *vals, int32_t *counts) {
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  uint32_t key = (uint32_t)t.nbins;
-  if (kw[m] != 0.0f) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = m < M;
+  uint32_t key = (uint32_t)t.nbins;  // no bin (skipped, or past M)
+  if (in && kw[m] != 0.0f) {
     const float k[3] = {kx[m], ky[m], kz[m]};
     bool ok = true;
     int b[3];
@@ -59,9 +59,21 @@ __global__ void k_traj_keys(const float *__restrict__ kx, const float *__restric
     }
     if (ok) key = (uint32_t)((b[2] * t.nb[1] + b[1]) * t.nb[0] + b[0]);
   }
-  keys[m] = key;
-  vals[m] = (int32_t)m;
-  if (key < (uint32_t)t.nbins) atomicAdd(&counts[key], 1);
+  if (in) {
+    keys[m] = key;
+    vals[m] = (int32_t)m;
+  }
+  // Consecutive samples of a trajectory mostly share a bin, and the centre
+  // bins take a sample of every interleave: each run of equal keys among the
+  // wavefront's lanes adds its length with one atomic from its first lane.
+  // (Every lane reaches this point: blockDim.x is a multiple of 64.)
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long after = lane == 63 ? 0ull : heads >> (lane + 1);
+  const int run = after ? __builtin_ctzll(after) + 1 : 64 - lane;
+  if (head && key < (uint32_t)t.nbins) atomicAdd(&counts[key], run);
 }
 
 __global__ void k_traj_finish(const float *__restrict__ kx, const float *__restrict__ ky,
